@@ -1,0 +1,180 @@
+/*
+ * cloudsc_hip.h -- C-ABI of the MI355X (gfx950) CLOUDSC drop-in.
+ *
+ * Reference interface replaced: SUBROUTINE CLOUDSC
+ *   ifs-source/arpifs/phys_ec/cloudsc.F90:14-44 (dummy arguments),
+ *   declarations :194-277, called from CLOUD_LAYER
+ *   ifs-source/arpifs/phys_ec/cloud_layer.F90:169-198.
+ *
+ * Conventions
+ *   - Every array argument is a plain pointer to Fortran-ordered storage
+ *     (JL, the column index, fastest).  Shapes follow cloudsc.F90:207-277:
+ *     "KLEV array" = (KLON,KLEV), "half-level array" = (KLON,KLEV+1),
+ *     PTENDENCY_*_CLD / PCLV = (KLON,KLEV,NCLV=5), PLUDELI = (KLON,KLEV,4),
+ *     PSNDE = (KLON,KLEV,2), PEXTRA = (KLON,KLEV,KFLDX).
+ *   - The seven derived-type arguments (YDECLDP, YDECUMF, YDEPHLI, YDERAD,
+ *     YDEPHY, YDVDF, YDSPP_CONFIG) are flattened into one cloudsc_params_t.
+ *   - LOGICAL arguments (LDCUM(KLON), LDDIAG, LDSLPHY) are passed as int32
+ *     (0 = .FALSE., nonzero = .TRUE.).
+ *   - Arguments that the default configuration never reads (budget-only,
+ *     aerosol-only, SPP-only, inhomogeneity-only: PVFA, PDYN*, PHRSW, PHRLW,
+ *     PVERVEL, PGEOM, PGAW, KCTOP, KPBLTYPE, PGP2DSPP, PLCRIT_AER, PICRIT_AER,
+ *     PRE_ICE, PCCN, PNICE, PEXTRA) are kept in the list so the call is a
+ *     positional drop-in; they may be NULL.  PFSD is written only when
+ *     LRAD_CLOUD_INHOMOG is set, which this build rejects, so it may be NULL.
+ *   - Write set is the reference's: PTENDENCY_LOC_CLD(:,:,5) is not written;
+ *     PCOVPTOT is written only for JK >= NCLDTOP; PRAINFRAC_TOPRFZ is zeroed
+ *     for all KLON columns; PLUDE is rescaled in place for NCLDTOP<=JK<KLEV
+ *     (cloudsc.F90:971-977, 3815, 1010, 1367/1384).
+ *   - Return value: 0 on success, a negative CLOUDSC_E* code otherwise.  The
+ *     reference has no status argument; its only fatal path is ABOR1
+ *     (budget diagnostics), so a Fortran shim maps nonzero to ABOR1.
+ *
+ * Three arithmetic variants share one argument list:
+ *   _dp    : FP64 storage and arithmetic (the ifs_dp build).
+ *   _sp    : FP32 storage and arithmetic (the ifs_sp build).
+ *   _mixed : FP32 storage and bulk arithmetic; FP64 for CLOUD_SUPERSATCHECK
+ *            (cloud_supersatcheck.F90) and for the no-pivot 5x5 elimination
+ *            with its back-substitution (cloudsc.F90:3546-3628).
+ */
+#ifndef CLOUDSC_HIP_H
+#define CLOUDSC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLOUDSC_NCLV 5
+#define CLOUDSC_NCLDQL 1
+#define CLOUDSC_NCLDQI 2
+#define CLOUDSC_NCLDQR 3
+#define CLOUDSC_NCLDQS 4
+#define CLOUDSC_NCLDQV 5
+
+/* Status codes */
+#define CLOUDSC_OK 0
+#define CLOUDSC_EDIMS (-1)        /* inconsistent KIDIA/KFDIA/KLON/KLEV/NCLDTOP */
+#define CLOUDSC_EUNSUPPORTED (-2) /* a configuration switch this build does not implement */
+#define CLOUDSC_ENULL (-3)        /* a pointer the default path reads or writes is NULL */
+#define CLOUDSC_EHIP (-4)         /* a HIP runtime call failed */
+#define CLOUDSC_ENODEVICE (-5)    /* no gfx950 device visible */
+#define CLOUDSC_ENOMEM (-6)       /* device allocation failed */
+
+/*
+ * Flattened configuration: every TECLDP / TTHF / TCST / TECUMF / TERAD /
+ * TEPHY / TVDF / TSPP_CONFIG component CLOUDSC and CLOUD_SUPERSATCHECK read.
+ * Names are the reference's component names in lower case
+ * (yoecldp.F90:112-296, yoethf.F90:102-128, yomcst.F90:508-629).
+ * Values are FP64; the FP32 variants round them once at launch.
+ */
+typedef struct cloudsc_params_t {
+  /* TCST (yomcst.F90) */
+  double rg, rd, rv, rcpd, retv, rtt, rlvtt, rlstt, rlmlt, ra, rpi;
+  /* TTHF (suphec.F90:231-254) */
+  double r2es, r3les, r3ies, r4les, r4ies, r5les, r5ies, r5alvcp, r5alscp;
+  double ralvdcp, ralsdcp, ralfdcp, rtwat, rtice, rtwat_rtice_r, rkoop1, rkoop2;
+  /* TECLDP (sucldp.F90:243-655) */
+  double ramid, ramin, rlmin, rcldiff, rcldiff_convi, rclcrit_sea, rclcrit_land;
+  double rkconv, rprc1, rcl_inhomogaut, rcl_inhomogacc;
+  double rsnowlin1, rsnowlin2, rlcritsnow, rvice, rvrain, rvsnow;
+  double rxfallspeedrain, rxfallspeedice, rnice, rccn;
+  double rcldtopcf, rdepliqrefrate, rdepliqrefdepth, rcl_overlapliqice, rcl_effrime;
+  double riceinit, rthomo, rprecrhmax, rpecons, rvrfactor, rcovpmin;
+  double rtaumel, rkooptau, rssicefactor, rdenswat, rdensref;
+  double rcl_dr, rcl_x1r, rcl_x2r, rcl_x1s, rcl_lam1r, rcl_lam2r, rcl_lam1s, rcl_lam2s;
+  double rcl_cdenom1, rcl_cdenom2, rcl_cdenom3, rcl_ka273;
+  double rcl_const1r, rcl_const2r, rcl_const3r, rcl_const4r, rcl_const5r;
+  double rcl_const6r, rcl_const7r, rcl_const9r, rcl_const10r, rcl_eff_racw;
+  double rcl_kkaac, rcl_kkbac, rcl_kkaau, rcl_kkbauq, rcl_kkbaun;
+  double rcl_kk_cloud_num_sea, rcl_kk_cloud_num_land, rcl_fzrab;
+  double rcl_const1i, rcl_const2i, rcl_const3i, rcl_const4i, rcl_const5i, rcl_const6i, rcl_const7i;
+  double rcl_apb1, rcl_apb2, rcl_apb3;
+  double rcl_const1s, rcl_const2s, rcl_const3s, rcl_const4s, rcl_const5s;
+  double rcl_const6s, rcl_const7s, rcl_const8s;
+  /* TECUMF (sucumf.F90:203,228-229) */
+  double rmfadvw, rmfadvwdd;
+  /* TERAD (suecrad.F90:745) and TVDF (suvdf.F90:135) */
+  double rcloud_frac_std, reisthsc;
+  /* integer / logical switches */
+  int32_t ncldtop;          /* first level the cloud scheme treats (sucldp.F90:631-634) */
+  int32_t lmfdsnow;         /* TECUMF: convective snow/rain detrainment */
+  int32_t lcloud_inhomog;   /* TECLDP: must be 0 (regime-dependent FSD not built) */
+  int32_t lrad_cloud_inhomog; /* TEPHY: must be 0 (ice FSD / PFSD not built) */
+  int32_t laeraware;        /* any of LAERLIQAUTOLSP/LAERLIQCOLL/LAERICESED/LAERICEAUTO: must be 0 */
+  int32_t lspp;             /* TSPP_CONFIG%LSPP: must be 0 */
+  int32_t lcldbud;          /* any LCLDBUD* / LCLDBUDGET / LBUD23 diagnostics: must be 0 */
+  int32_t pad_;
+} cloudsc_params_t;
+
+/*
+ * Fill *p with the reference defaults of SUPHEC/SUCLDP/SUCUMF/SU0PHY/SUECRAD/
+ * SUVDF for physics time step ptsphy (RSSICEFACTOR = MIN(TSPHY/RKOOPTAU,1),
+ * sucldp.F90:655) and NCLDTOP from a standard-atmosphere full-level pressure
+ * profile stpre[0..nflevg-1] (Pa) with RCLDTOPP = 100 Pa (sucldp.F90:631-634).
+ * Replaces the setup call chain SUPHEC -> SUCLDP (suphec.F90:406).
+ */
+int cloudsc_params_default(cloudsc_params_t* p, double ptsphy, int nflevg, const double* stpre);
+
+/* ---------------------------------------------------------------------------
+ * Per-block drop-in (host pointers).  One call = one NPROMA block, exactly
+ * like the reference call from CLOUD_LAYER.  Copies the block to the device,
+ * runs the kernel and copies the outputs back.  Reentrant: OpenMP threads of
+ * the EC_PHYS_DRV block loop (ec_phys_drv.F90:430-439) may call concurrently.
+ * ------------------------------------------------------------------------- */
+#define CLOUDSC_ARGS(R)                                                                                 \
+  const cloudsc_params_t *ydparams, int kidia, int kfdia, int klon, int klev, int lddiag, R ptsphy,     \
+      const R *pt, const R *pq, const R *ptendency_cml_t, const R *ptendency_cml_q,                     \
+      const R *ptendency_cml_a, const R *ptendency_cml_cld, R *ptendency_loc_t, R *ptendency_loc_q,     \
+      R *ptendency_loc_a, R *ptendency_loc_cld, const R *pvfa, const R *pvfl, const R *pvfi,            \
+      const R *pdyna, const R *pdynl, const R *pdyni, const R *pdynr, const R *pdyns, const R *phrsw,  \
+      const R *phrlw, const R *pvervel, const R *pap, const R *paph, const R *pgeom, const R *plsm,     \
+      const R *pgaw, const int32_t *ldcum, const int32_t *kctop, const int32_t *ktype,                  \
+      const int32_t *kpbltype, R *peis, const R *plu, R *plude, const R *pludeli, const R *psnde,       \
+      const R *pmfu, const R *pmfd, const R *pgp2dspp, int ldslphy, const R *pa, const R *pclv,         \
+      const R *plcrit_aer, const R *picrit_aer, const R *pre_ice, const R *pccn, const R *pnice,        \
+      R *pcovptot, R *pfsd, R *prainfrac_toprfz, R *pfsqlf, R *pfsqif, R *pfcqnng, R *pfcqlng,          \
+      R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur, R *pfplsl, R *pfplsn,     \
+      R *pfhpsl, R *pfhpsn, R *pextra, int kfldx
+
+/* Replaces CLOUDSC of the ifs_dp build (JPRB = FP64). */
+int cloudsc_dp_c(CLOUDSC_ARGS(double));
+/* Replaces CLOUDSC of the ifs_sp build (JPRB = FP32). */
+int cloudsc_sp_c(CLOUDSC_ARGS(float));
+/* Replaces CLOUDSC of the ifs_sp build, FP64 islands (supersaturation check, 5x5 solve). */
+int cloudsc_mixed_c(CLOUDSC_ARGS(float));
+
+/* ---------------------------------------------------------------------------
+ * Blocked, device-resident entry: one launch over all NGPBLKS blocks of the
+ * EC_PHYS_DRV loop (ec_phys_drv.F90:430-441).  Every pointer is a device
+ * pointer.  Arrays are the IFS blocked fields (ec_phys_drv.F90:146):
+ * (NPROMA,KLEV,NGPBLKS), (NPROMA,KLEV+1,NGPBLKS), (NPROMA,KLEV,n,NGPBLKS),
+ * per-column arrays (NPROMA,NGPBLKS).  Block IBL covers global columns
+ * (IBL-1)*NPROMA+1 .. MIN(IBL*NPROMA, NGPTOT) with KIDIA = 1 and
+ * KFDIA = MIN(NPROMA, NGPTOT-(IBL-1)*NPROMA) (ec_phys_drv.F90:441-465).
+ * stream: a hipStream_t (NULL = the null stream).  The call is asynchronous.
+ * ------------------------------------------------------------------------- */
+#define CLOUDSC_BLOCKED_ARGS(R)                                                                         \
+  const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,            \
+      const R *pt, const R *pq, const R *ptendency_cml_t, const R *ptendency_cml_q,                     \
+      const R *ptendency_cml_a, const R *ptendency_cml_cld, R *ptendency_loc_t, R *ptendency_loc_q,     \
+      R *ptendency_loc_a, R *ptendency_loc_cld, const R *pvfl, const R *pvfi, const R *pap,             \
+      const R *paph, const R *plsm, const int32_t *ldcum, const int32_t *ktype, R *peis,                \
+      const R *plu, R *plude, const R *pludeli, const R *psnde, const R *pmfu, const R *pmfd,           \
+      const R *pa, const R *pclv, R *pcovptot, R *prainfrac_toprfz, R *pfsqlf, R *pfsqif,              \
+      R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur,   \
+      R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, void *stream
+
+int cloudsc_blocked_dp_c(CLOUDSC_BLOCKED_ARGS(double));
+int cloudsc_blocked_sp_c(CLOUDSC_BLOCKED_ARGS(float));
+int cloudsc_blocked_mixed_c(CLOUDSC_BLOCKED_ARGS(float));
+
+/* Last HIP error text of the calling thread (empty string if none). */
+const char *cloudsc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLOUDSC_HIP_H */

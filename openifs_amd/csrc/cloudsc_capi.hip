@@ -1,0 +1,266 @@
+// extern "C" entry points of libcloudsc_hip.so (declared in include/cloudsc_hip.h).
+//
+//   cloudsc_blocked_{dp,sp,mixed}_c : device pointers, one launch over all
+//       NGPBLKS blocks (the production path).
+//   cloudsc_{dp,sp,mixed}_c         : host pointers, one NPROMA block, the
+//       positional drop-in for SUBROUTINE CLOUDSC (cloudsc.F90:14-44).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cloudsc_hip.h"
+#include "cloudsc_kernel.h"
+
+namespace cloudsc_gpu {
+template <typename R, typename RD>
+hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
+                          hipStream_t stream);
+}
+
+using cloudsc_gpu::KDims;
+using cloudsc_gpu::KPtrs;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* what, hipError_t e = hipSuccess) {
+  g_last_error = what;
+  if (e != hipSuccess) {
+    g_last_error += ": ";
+    g_last_error += hipGetErrorString(e);
+  }
+  return code;
+}
+
+int check_config(const cloudsc_params_t* p) {
+  if (!p) return fail(CLOUDSC_ENULL, "cloudsc: NULL parameter block");
+  if (p->lcloud_inhomog || p->lrad_cloud_inhomog || p->laeraware || p->lspp || p->lcldbud)
+    return fail(CLOUDSC_EUNSUPPORTED,
+                "cloudsc: LCLOUD_INHOMOG/LRAD_CLOUD_INHOMOG/aerosol/SPP/budget switches are not implemented");
+  return CLOUDSC_OK;
+}
+
+// The kernel is built for gfx950 only; refuse any other device up front.
+int check_device() {
+  static std::mutex m;
+  static std::vector<int> ok(64, -1);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(CLOUDSC_ENODEVICE, "cloudsc: hipGetDevice", e);
+  std::lock_guard<std::mutex> lk(m);
+  if (dev < 0 || dev >= (int)ok.size()) return fail(CLOUDSC_ENODEVICE, "cloudsc: device index out of range");
+  if (ok[dev] < 0) {
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return fail(CLOUDSC_ENODEVICE, "cloudsc: hipGetDeviceProperties", e);
+    ok[dev] = std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+  }
+  if (!ok[dev]) return fail(CLOUDSC_ENODEVICE, "cloudsc: device is not gfx950 (MI355X)");
+  return CLOUDSC_OK;
+}
+
+template <typename R, typename RD>
+int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngpblks, R ptsphy, const KPtrs<R>& p,
+            void* stream) {
+  int rc = check_config(q);
+  if (rc) return rc;
+  if (nproma < 1 || klev < 2 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma)
+    return fail(CLOUDSC_EDIMS, "cloudsc_blocked: need NGPBLKS = ceil(NGPTOT/NPROMA), KLEV >= 2");
+  if (q->ncldtop < 2 || q->ncldtop > klev) return fail(CLOUDSC_EDIMS, "cloudsc_blocked: NCLDTOP outside [2,KLEV]");
+  const void* req[] = {p.pt, p.pq, p.tcml_t, p.tcml_q, p.tcml_a, p.tcml_cld, p.tloc_t, p.tloc_q, p.tloc_a,
+                       p.tloc_cld, p.pvfl, p.pvfi, p.pap, p.paph, p.plsm, p.ldcum, p.ktype, p.peis, p.plu,
+                       p.plude, p.pludeli, p.psnde, p.pmfu, p.pmfd, p.pa, p.pclv, p.pcovptot,
+                       p.prainfrac_toprfz, p.pfsqlf, p.pfsqif, p.pfcqnng, p.pfcqlng, p.pfsqrf, p.pfsqsf,
+                       p.pfcqrng, p.pfcqsng, p.pfsqltur, p.pfsqitur, p.pfplsl, p.pfplsn, p.pfhpsl, p.pfhpsn};
+  for (const void* x : req)
+    if (!x) return fail(CLOUDSC_ENULL, "cloudsc_blocked: NULL array on the default path");
+  rc = check_device();
+  if (rc) return rc;
+  KDims d{ngptot, nproma, klev, ngpblks, 1, nproma};
+  hipError_t e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, 0, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_blocked: kernel launch", e);
+  return CLOUDSC_OK;
+}
+
+// ---- per-block host entry: a per-thread device workspace and stream ----
+struct Workspace {
+  void* dev = nullptr;
+  size_t bytes = 0;
+  hipStream_t stream = nullptr;
+  int device = -1;
+};
+thread_local Workspace g_ws;
+
+int workspace(size_t bytes, Workspace** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: hipGetDevice", e);
+  Workspace& w = g_ws;
+  if (w.device != dev) {  // first call on this thread, or the thread switched devices
+    w = Workspace{};
+    w.device = dev;
+    e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: hipStreamCreate", e);
+  }
+  if (w.bytes < bytes) {
+    if (w.dev) (void)hipFree(w.dev);
+    w.dev = nullptr;
+    w.bytes = 0;
+    e = hipMalloc(&w.dev, bytes);
+    if (e != hipSuccess) return fail(CLOUDSC_ENOMEM, "cloudsc: hipMalloc workspace", e);
+    w.bytes = bytes;
+  }
+  *out = &w;
+  return CLOUDSC_OK;
+}
+
+template <typename R, typename RD>
+int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int klev, R ptsphy, const R* pt,
+              const R* pq, const R* tcml_t, const R* tcml_q, const R* tcml_a, const R* tcml_cld, R* tloc_t, R* tloc_q,
+              R* tloc_a, R* tloc_cld, const R* pvfl, const R* pvfi, const R* pap, const R* paph, const R* plsm,
+              const int32_t* ldcum, const int32_t* ktype, R* peis, const R* plu, R* plude, const R* pludeli,
+              const R* psnde, const R* pmfu, const R* pmfd, const R* pa, const R* pclv, R* pcovptot,
+              R* prainfrac_toprfz, R* pfsqlf, R* pfsqif, R* pfcqnng, R* pfcqlng, R* pfsqrf, R* pfsqsf, R* pfcqrng,
+              R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn, R* pfhpsl, R* pfhpsn) {
+  int rc = check_config(q);
+  if (rc) return rc;
+  if (klon < 1 || klev < 2 || kidia < 1 || kfdia < kidia || kfdia > klon || kidia + kfdia - 1 > klon)
+    return fail(CLOUDSC_EDIMS, "cloudsc: need 1 <= KIDIA <= KFDIA <= KLON and KIDIA+KFDIA-1 <= KLON");
+  if (q->ncldtop < 2 || q->ncldtop > klev) return fail(CLOUDSC_EDIMS, "cloudsc: NCLDTOP outside [2,KLEV]");
+  const void* req[] = {pt, pq, tcml_t, tcml_q, tcml_a, tcml_cld, tloc_t, tloc_q, tloc_a, tloc_cld, pvfl, pvfi,
+                       pap, paph, plsm, ldcum, ktype, peis, plu, plude, pludeli, psnde, pmfu, pmfd, pa, pclv,
+                       pcovptot, prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng,
+                       pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn};
+  for (const void* x : req)
+    if (!x) return fail(CLOUDSC_ENULL, "cloudsc: NULL array on the default path");
+  rc = check_device();
+  if (rc) return rc;
+
+  const size_t n2 = (size_t)klon * klev, nh = (size_t)klon * (klev + 1), n1 = klon;
+  // every array the kernel touches, with its element count; outputs are copied in too so that
+  // columns outside KIDIA:KFDIA and levels the reference leaves untouched come back unchanged
+  struct Arr {
+    const void* host;
+    void* host_out;
+    size_t n, esz;
+  };
+  Arr arrs[] = {{pt, nullptr, n2, sizeof(R)},          {pq, nullptr, n2, sizeof(R)},
+                {tcml_t, nullptr, n2, sizeof(R)},      {tcml_q, nullptr, n2, sizeof(R)},
+                {tcml_a, nullptr, n2, sizeof(R)},      {tcml_cld, nullptr, 5 * n2, sizeof(R)},
+                {tloc_t, tloc_t, n2, sizeof(R)},       {tloc_q, tloc_q, n2, sizeof(R)},
+                {tloc_a, tloc_a, n2, sizeof(R)},       {tloc_cld, tloc_cld, 5 * n2, sizeof(R)},
+                {pvfl, nullptr, n2, sizeof(R)},        {pvfi, nullptr, n2, sizeof(R)},
+                {pap, nullptr, n2, sizeof(R)},         {paph, nullptr, nh, sizeof(R)},
+                {plsm, nullptr, n1, sizeof(R)},        {ldcum, nullptr, n1, sizeof(int32_t)},
+                {ktype, nullptr, n1, sizeof(int32_t)}, {peis, nullptr, n1, sizeof(R)},
+                {plu, nullptr, n2, sizeof(R)},         {plude, plude, n2, sizeof(R)},
+                {pludeli, nullptr, 4 * n2, sizeof(R)}, {psnde, nullptr, 2 * n2, sizeof(R)},
+                {pmfu, nullptr, n2, sizeof(R)},        {pmfd, nullptr, n2, sizeof(R)},
+                {pa, nullptr, n2, sizeof(R)},          {pclv, nullptr, 5 * n2, sizeof(R)},
+                {pcovptot, pcovptot, n2, sizeof(R)},   {prainfrac_toprfz, prainfrac_toprfz, n1, sizeof(R)},
+                {pfsqlf, pfsqlf, nh, sizeof(R)},       {pfsqif, pfsqif, nh, sizeof(R)},
+                {pfcqnng, pfcqnng, nh, sizeof(R)},     {pfcqlng, pfcqlng, nh, sizeof(R)},
+                {pfsqrf, pfsqrf, nh, sizeof(R)},       {pfsqsf, pfsqsf, nh, sizeof(R)},
+                {pfcqrng, pfcqrng, nh, sizeof(R)},     {pfcqsng, pfcqsng, nh, sizeof(R)},
+                {pfsqltur, pfsqltur, nh, sizeof(R)},   {pfsqitur, pfsqitur, nh, sizeof(R)},
+                {pfplsl, pfplsl, nh, sizeof(R)},       {pfplsn, pfplsn, nh, sizeof(R)},
+                {pfhpsl, pfhpsl, nh, sizeof(R)},       {pfhpsn, pfhpsn, nh, sizeof(R)}};
+  const int narr = sizeof(arrs) / sizeof(arrs[0]);
+  size_t off[sizeof(arrs) / sizeof(arrs[0])];
+  size_t total = 0;
+  for (int i = 0; i < narr; ++i) {
+    off[i] = total;
+    total += (arrs[i].n * arrs[i].esz + 255) & ~(size_t)255;
+  }
+  Workspace* w = nullptr;
+  rc = workspace(total, &w);
+  if (rc) return rc;
+  char* base = (char*)w->dev;
+  hipError_t e = hipSuccess;
+  for (int i = 0; i < narr && e == hipSuccess; ++i)
+    e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * arrs[i].esz, hipMemcpyHostToDevice, w->stream);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: copy to device", e);
+  auto D = [&](int i) { return (R*)(base + off[i]); };
+  KPtrs<R> p;
+  p.pt = D(0); p.pq = D(1); p.tcml_t = D(2); p.tcml_q = D(3); p.tcml_a = D(4); p.tcml_cld = D(5);
+  p.tloc_t = D(6); p.tloc_q = D(7); p.tloc_a = D(8); p.tloc_cld = D(9); p.pvfl = D(10); p.pvfi = D(11);
+  p.pap = D(12); p.paph = D(13); p.plsm = D(14); p.ldcum = (const int32_t*)D(15); p.ktype = (const int32_t*)D(16);
+  p.peis = D(17); p.plu = D(18); p.plude = D(19); p.pludeli = D(20); p.psnde = D(21); p.pmfu = D(22);
+  p.pmfd = D(23); p.pa = D(24); p.pclv = D(25); p.pcovptot = D(26); p.prainfrac_toprfz = D(27);
+  p.pfsqlf = D(28); p.pfsqif = D(29); p.pfcqnng = D(30); p.pfcqlng = D(31); p.pfsqrf = D(32); p.pfsqsf = D(33);
+  p.pfcqrng = D(34); p.pfcqsng = D(35); p.pfsqltur = D(36); p.pfsqitur = D(37); p.pfplsl = D(38);
+  p.pfplsn = D(39); p.pfhpsl = D(40); p.pfhpsn = D(41);
+  // one block: KFDIA bounds the active columns, KIDIA-1 shifts the pressure section of the supersaturation check
+  KDims d{kfdia, klon, klev, 1, kidia, kfdia};
+  e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, kidia - 1, w->stream);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: kernel launch", e);
+  for (int i = 0; i < narr && e == hipSuccess; ++i)
+    if (arrs[i].host_out)
+      e = hipMemcpyAsync(arrs[i].host_out, base + off[i], arrs[i].n * arrs[i].esz, hipMemcpyDeviceToHost, w->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: copy to host", e);
+  return CLOUDSC_OK;
+}
+
+template <typename R>
+KPtrs<R> blocked_ptrs(const R* pt, const R* pq, const R* tcml_t, const R* tcml_q, const R* tcml_a, const R* tcml_cld,
+                      R* tloc_t, R* tloc_q, R* tloc_a, R* tloc_cld, const R* pvfl, const R* pvfi, const R* pap,
+                      const R* paph, const R* plsm, const int32_t* ldcum, const int32_t* ktype, R* peis, const R* plu,
+                      R* plude, const R* pludeli, const R* psnde, const R* pmfu, const R* pmfd, const R* pa,
+                      const R* pclv, R* pcovptot, R* prainfrac_toprfz, R* pfsqlf, R* pfsqif, R* pfcqnng, R* pfcqlng,
+                      R* pfsqrf, R* pfsqsf, R* pfcqrng, R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn,
+                      R* pfhpsl, R* pfhpsn) {
+  KPtrs<R> p;
+  p.pt = pt; p.pq = pq; p.tcml_t = tcml_t; p.tcml_q = tcml_q; p.tcml_a = tcml_a; p.tcml_cld = tcml_cld;
+  p.tloc_t = tloc_t; p.tloc_q = tloc_q; p.tloc_a = tloc_a; p.tloc_cld = tloc_cld; p.pvfl = pvfl; p.pvfi = pvfi;
+  p.pap = pap; p.paph = paph; p.plsm = plsm; p.ldcum = ldcum; p.ktype = ktype; p.peis = peis; p.plu = plu;
+  p.plude = plude; p.pludeli = pludeli; p.psnde = psnde; p.pmfu = pmfu; p.pmfd = pmfd; p.pa = pa; p.pclv = pclv;
+  p.pcovptot = pcovptot; p.prainfrac_toprfz = prainfrac_toprfz; p.pfsqlf = pfsqlf; p.pfsqif = pfsqif;
+  p.pfcqnng = pfcqnng; p.pfcqlng = pfcqlng; p.pfsqrf = pfsqrf; p.pfsqsf = pfsqsf; p.pfcqrng = pfcqrng;
+  p.pfcqsng = pfcqsng; p.pfsqltur = pfsqltur; p.pfsqitur = pfsqitur; p.pfplsl = pfplsl; p.pfplsn = pfplsn;
+  p.pfhpsl = pfhpsl; p.pfhpsn = pfhpsn;
+  return p;
+}
+
+}  // namespace
+
+#define PER_BLOCK_CALL(R, RD)                                                                                   \
+  (void)lddiag, (void)pvfa, (void)pdyna, (void)pdynl, (void)pdyni, (void)pdynr, (void)pdyns, (void)phrsw,       \
+      (void)phrlw, (void)pvervel, (void)pgeom, (void)pgaw, (void)kctop, (void)kpbltype, (void)pgp2dspp,           \
+      (void)ldslphy, (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice, (void)pfsd,      \
+      (void)pextra, (void)kfldx;                                                                                  \
+  return per_block<R, RD>(ydparams, kidia, kfdia, klon, klev, ptsphy, pt, pq, ptendency_cml_t, ptendency_cml_q,  \
+                          ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a,   \
+                          ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis, plu, plude, pludeli, \
+                          psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng,        \
+                          pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl,   \
+                          pfhpsn)
+
+#define BLOCKED_CALL(R, RD)                                                                                      \
+  return blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy,                                          \
+                        blocked_ptrs<R>(pt, pq, ptendency_cml_t, ptendency_cml_q, ptendency_cml_a,                \
+                                        ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a,     \
+                                        ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis, plu,  \
+                                        plude, pludeli, psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz,  \
+                                        pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng,       \
+                                        pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn),                      \
+                        stream)
+
+extern "C" {
+
+int cloudsc_dp_c(CLOUDSC_ARGS(double)) { PER_BLOCK_CALL(double, double); }
+int cloudsc_sp_c(CLOUDSC_ARGS(float)) { PER_BLOCK_CALL(float, float); }
+int cloudsc_mixed_c(CLOUDSC_ARGS(float)) { PER_BLOCK_CALL(float, double); }
+
+int cloudsc_blocked_dp_c(CLOUDSC_BLOCKED_ARGS(double)) { BLOCKED_CALL(double, double); }
+int cloudsc_blocked_sp_c(CLOUDSC_BLOCKED_ARGS(float)) { BLOCKED_CALL(float, float); }
+int cloudsc_blocked_mixed_c(CLOUDSC_BLOCKED_ARGS(float)) { BLOCKED_CALL(float, double); }
+
+const char* cloudsc_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// CLOUDSC column kernel for gfx950: one column per lane, levels swept top to
+// bottom with all level-to-level state in registers.
+//
+// Restates the default path of ifs-source/arpifs/phys_ec/cloudsc.F90 and
+// cloud_supersatcheck.F90 (hard-wired options at cloudsc.F90:727-793).
+// Layout: IFS blocked fields (NPROMA, KLEV[+1], n, NGPBLKS), JL fastest, so
+// lane JL of a wavefront reads consecutive words at every level.
+//
+// Register-level reformulation (same arithmetic as the reference):
+//  * ZSOLQA is antisymmetric off the diagonal on every path the default
+//    configuration takes (each process adds +x to (a,b) and -x to (b,a)), and
+//    only 7 off-diagonal pairs and 4 diagonal entries are ever nonzero, so the
+//    kernel keeps 11 scalars instead of 25 (cloudsc.F90:1335-3408).
+//  * ZSOLQB has 5 nonzero entries (L,L), (I,I), (R,L), (S,I), (S,L), which
+//    makes ZQLHS lower triangular: the no-pivot LU of cloudsc.F90:3600-3628
+//    reduces to 3 multipliers and 5 divisions with the same operations on the
+//    nonzero entries (the skipped ones only add or subtract exact zeros).
+//  * Level JK-1 quantities (tidied ZA, ZQX(r,s), ZTP1, PAP) and level JK+1
+//    inputs (PLU, PMFU, PMFD) are carried in registers instead of KLEV arrays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <limits>
+
+#include "../../include/cloudsc_hip.h"
+
+namespace cloudsc_gpu {
+
+// Constants in the kernel's working precision T.
+template <typename T>
+struct KConst {
+  T rg, rd, rv_rd, retv, rtt, rlvtt, rlstt, rpi, r2es, r3les, r3ies, r4les, r4ies, r5les, r5ies;
+  T r5alvcp, r5alscp, ralvdcp, ralsdcp, rtwat, rtice, rtwat_rtice_r, rkoop1, rkoop2, rv, rd_rv_dummy;
+  T ramid, ramin, rlmin, rcldiff, rcldiff_convi, rkconv;
+  T rsnowlin1, rsnowlin2, rlcritsnow, rvice, rvrain, rvsnow, rxfallspeedrain;
+  T rcldtopcf, rdepliqrefrate, rdepliqrefdepth, rcl_overlapliqice, rcl_effrime, riceinit, rthomo;
+  T rprecrhmax, rpecons, rvrfactor, rcovpmin, rtaumel, rssicefactor, rdenswat, rdensref;
+  T rcl_dr, rcl_x1r, rcl_x2r, rcl_x1s, rcl_lam1r, rcl_lam2r, rcl_lam1s, rcl_lam2s;
+  T rcl_cdenom1, rcl_cdenom2, rcl_cdenom3, rcl_const1r, rcl_const2r, rcl_const3r, rcl_const4r;
+  T rcl_const5r, rcl_const6r, rcl_const7r, rcl_kkaac, rcl_kkbac, rcl_kkbauq, rcl_kkbaun;
+  T rcl_kk_cloud_num_sea, rcl_kk_cloud_num_land, rcl_fzrab, rcl_apb1, rcl_apb2, rcl_apb3;
+  T rcl_const1s, rcl_const2s, rcl_const3s, rcl_const4s, rcl_const5s, rcl_const6s, rcl_const7s, rcl_const8s;
+  T rmfadvw, rmfadvwdd;
+  // derived scalars (cloudsc.F90:798-804), all evaluated as the reference does
+  T ptsphy, zqtmst, zrdcp, zrg_r, zrldcp, zepsilon, rdtg_r;
+  T c_aut, c_acc;  // RCL_INHOMOGAUT*PTSPHY*RCL_KKAAU, RCL_INHOMOGACC*PTSPHY*RCL_KKAAC
+  int32_t ncldtop, lmfdsnow;
+};
+
+// Device pointers of one launch, blocked layout.
+template <typename R>
+struct KPtrs {
+  const R *pt, *pq, *tcml_t, *tcml_q, *tcml_a, *tcml_cld;
+  R *tloc_t, *tloc_q, *tloc_a, *tloc_cld;
+  const R *pvfl, *pvfi, *pap, *paph, *plsm;
+  const int32_t *ldcum, *ktype;
+  const R *peis, *plu;
+  R* plude;
+  const R *pludeli, *psnde, *pmfu, *pmfd, *pa, *pclv;
+  R *pcovptot, *prainfrac_toprfz;
+  R *pfsqlf, *pfsqif, *pfcqnng, *pfcqlng, *pfsqrf, *pfsqsf, *pfcqrng, *pfcqsng, *pfsqltur, *pfsqitur;
+  R *pfplsl, *pfplsn, *pfhpsl, *pfhpsn;
+};
+
+struct KDims {
+  int32_t ngptot;   // columns in total (blocked) or KFDIA (one block)
+  int32_t nproma;   // KLON
+  int32_t klev;
+  int32_t ngpblks;
+  int32_t kidia;    // first active column of every block (1-based)
+  int32_t kfdia;    // last active column of a full block (1-based)
+};
+
+}  // namespace cloudsc_gpu

@@ -1,0 +1,916 @@
+// CLOUDSC for gfx950 -- the column kernel.  See cloudsc_kernel.h for the
+// data layout and the register-level reformulation.  Line citations refer to
+// ifs-source/arpifs/phys_ec/cloudsc.F90 unless another file is named.
+#include <hip/hip_runtime.h>
+
+#include "cloudsc_kernel.h"
+
+namespace cloudsc_gpu {
+
+__device__ __forceinline__ float m_exp(float x) { return expf(x); }
+__device__ __forceinline__ double m_exp(double x) { return exp(x); }
+__device__ __forceinline__ float m_pow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
+template <typename T>
+__device__ __forceinline__ T m_max(T a, T b) { return a < b ? b : a; }
+template <typename T>
+__device__ __forceinline__ T m_min(T a, T b) { return b < a ? b : a; }
+template <typename T>
+__device__ __forceinline__ T m_abs(T a) { return a < T(0) ? -a : a; }
+
+// Tetens saturation vapour pressures (fcttre.func.h:84-93,165-166).
+template <typename T>
+struct Sat {
+  T alfa, el, ei;  // FOEALFA, EXP(R3LES*(T-RTT)/(T-R4LES)), EXP(R3IES*(T-RTT)/(T-R4IES))
+};
+template <typename T>
+__device__ __forceinline__ Sat<T> saturation(const KConst<T>& c, T t) {
+  Sat<T> s;
+  T x = (m_max(c.rtice, m_min(c.rtwat, t)) - c.rtice) * c.rtwat_rtice_r;
+  s.alfa = m_min(T(1), x * x);
+  s.el = m_exp(c.r3les * (t - c.rtt) / (t - c.r4les));
+  s.ei = m_exp(c.r3ies * (t - c.rtt) / (t - c.r4ies));
+  return s;
+}
+
+// CLOUD_SUPERSATCHECK for one point, KFTLIQICE = 1 (cloud_supersatcheck.F90:111-226).
+template <typename T>
+__device__ __forceinline__ void supersatcheck(const KConst<T>& c, T pt, T pq, T pa, T pp, T& t_adj, T& q_adj,
+                                              T& a_adj, T& l_adj, T& i_adj) {
+  const T zepsec = T(1.e-14);
+  T zda = T(0);
+  T zt = m_max(pt, T(160));
+  T za = m_max(m_min(pa, T(1)), T(0));
+  const T zp_r = T(1) / pp;
+  const Sat<T> s = saturation<T>(c, zt);
+  const T foeeliq = c.r2es * s.el;
+  const T zfoeeliq = m_min(foeeliq * zp_r, T(0.5));
+  // FOEEW: water above RTT, ice below (FOEDELTA = MAX(0,SIGN(1,T-RTT)))
+  const T foeew = (zt - c.rtt >= T(0)) ? c.r2es * s.el : c.r2es * s.ei;
+  const T zfoeeice = m_min(foeew * zp_r, T(0.5));
+  const T zfokoop = m_min(zfoeeice * (c.rkoop1 - c.rkoop2 * zt), zfoeeliq);
+  T zqsliqk = zfokoop / (T(1) - c.retv * zfokoop);
+  const T zfoeemix = m_min(c.r2es * (s.alfa * s.el + (T(1) - s.alfa) * s.ei) * zp_r, T(0.5));
+  const T zcor = T(1) / (T(1) - c.retv * zfoeemix);
+  const T zqsmix = zfoeemix * zcor;
+  const T dl = zt - c.r4les, di = zt - c.r4ies;
+  const T foedem = s.alfa * c.r5alvcp * (T(1) / (dl * dl)) + (T(1) - s.alfa) * c.r5alscp * (T(1) / (di * di));
+  const T zcorqsmix = T(1) + zqsmix * zcor * foedem;
+  zqsliqk = T(0.5) * (zqsliqk + zqsmix);
+  T zqslim = za * zqsmix + (T(1) - za) * zqsliqk;
+  T zcond = m_max(pq - zqslim, T(0));
+  if (zcond > zepsec) {
+    const T zfaci = s.alfa + (T(1) - s.alfa) * c.rssicefactor;
+    zda = (T(1) - za) * zfaci;
+    const T za_upd = za + zda;
+    zqslim = za_upd * zqsmix + (T(1) - za_upd) * zqsliqk;
+    zcond = (pq - zqslim) / zcorqsmix;
+  }
+  const T foeldcpm = s.alfa * c.ralvdcp + (T(1) - s.alfa) * c.ralsdcp;
+  t_adj = foeldcpm * zcond;
+  q_adj = -zcond;
+  a_adj = zda;
+  l_adj = zcond * s.alfa;
+  i_adj = zcond * (T(1) - s.alfa);
+}
+
+// Scale factor selection for the explicit-sink truncation (:3500-3536):
+// a stored off-diagonal value x = ZSOLQA(a,b) = -ZSOLQA(b,a) is scaled when
+// the row being processed is a and x < 0, or the row is b and x > 0.
+template <typename T>
+__device__ __forceinline__ T trunc_scale(T x, int jo, int a, int b, T zz) {
+  const bool hit = (jo == a && x < T(0)) || (jo == b && x > T(0));
+  return hit ? x * zz : x;
+}
+
+template <typename R, typename RD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c, const KConst<RD> cd,
+                                                                const KPtrs<R> p, const KDims d, int ssc_off) {
+  const int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const int64_t ncols = (int64_t)d.ngpblks * d.nproma;
+  if (g >= ncols) return;
+  const int64_t b = g / d.nproma;
+  const int jl = (int)(g - b * d.nproma);  // 0-based JL-1
+  const int nproma = d.nproma, klev = d.klev;
+  // PRAINFRAC_TOPRFZ is zeroed for all KLON columns of the block (:1010)
+  p.prainfrac_toprfz[b * nproma + jl] = R(0);
+  const int kfdia = (int)min((int64_t)d.kfdia, (int64_t)d.ngptot - b * nproma);
+  if (jl < d.kidia - 1 || jl >= kfdia) return;
+
+  const int64_t s2 = (int64_t)nproma * klev;
+  const int64_t b2 = b * s2 + jl;                                  // (NPROMA,KLEV) level 0
+  const int64_t bh = b * (int64_t)nproma * (klev + 1) + jl;        // (NPROMA,KLEV+1) level 0
+  const int64_t b5 = b * s2 * 5 + jl, b4 = b * s2 * 4 + jl, b2x = b * s2 * 2 + jl;
+  const int64_t b1 = b * nproma + jl;
+
+  const R ptsphy = c.ptsphy, zqtmst = c.zqtmst, zepsec = R(1.e-14);
+  const R rlmin = c.rlmin;
+
+  // Per-column inputs
+  const R plsm = p.plsm[b1];
+  const int32_t ktype = p.ktype[b1];
+  const bool ldcum = p.ldcum[b1] != 0;
+  const R peis = p.peis[b1];
+  const R paph_surf = p.paph[bh + (int64_t)klev * nproma];
+  const R zadvw = (ktype == 1 && c.rmfadvw > R(0)) ? R(1) - c.rmfadvw : R(1);                  // :696-703
+  const R zadvwd = (ktype == 1 && c.rmfadvw > R(0)) ? R(1) - c.rmfadvw * c.rmfadvwdd : R(1);
+  const R zconst = plsm > R(0.5) ? c.rcl_kk_cloud_num_land : c.rcl_kk_cloud_num_sea;          // :2063-2069
+  const R zconst_pow = m_pow(zconst, c.rcl_kkbaun);
+  const int ncldtop0 = c.ncldtop - 1;  // 0-based first level of the cloud scheme
+
+  // Level-carried state (:1205-1211 and :1011)
+  R qxnm1_l = R(0), qxnm1_i = R(0), qxnm1_r = R(0), qxnm1_s = R(0);
+  R zanewm1 = R(0), zcovptot = R(0), zcovpmax = R(0), zcldtopdist = R(0);
+  bool llrainliq = true;
+  R prainfrac = R(0);
+  R pf_i = R(0), pf_r = R(0), pf_s = R(0);  // ZPFPLSX(JK) of the falling species
+  R za_prev = R(0), qr_prev = R(0), qs_prev = R(0), t_prev = R(0), pap_prev = R(0);
+  // DDH flux running sums at the upper half level (:4231-4242)
+  R sqlf = R(0), sqif = R(0), cqlng = R(0), cqnng = R(0), sqltur = R(0), sqitur = R(0);
+  p.pfsqlf[bh] = R(0);
+  p.pfsqif[bh] = R(0);
+  p.pfsqrf[bh] = R(0);
+  p.pfsqsf[bh] = R(0);
+  p.pfcqlng[bh] = R(0);
+  p.pfcqnng[bh] = R(0);
+  p.pfcqrng[bh] = R(0);
+  p.pfcqsng[bh] = R(0);
+  p.pfsqltur[bh] = R(0);
+  p.pfsqitur[bh] = R(0);
+
+  R paph_k = p.paph[bh];
+  R pmfu_k = p.pmfu[b2], pmfd_k = p.pmfd[b2];
+
+  for (int k = 0; k < klev; ++k) {
+    const int64_t i2 = b2 + (int64_t)k * nproma;
+    const int64_t ih1 = bh + (int64_t)(k + 1) * nproma;
+    const int64_t i5 = b5 + (int64_t)k * nproma;
+    const bool has_next = k + 1 < klev;
+    const R pmfu_n = has_next ? p.pmfu[i2 + nproma] : R(0);
+    const R pmfd_n = has_next ? p.pmfd[i2 + nproma] : R(0);
+    const R plu_n = has_next ? p.plu[i2 + nproma] : R(0);
+    const R paph_k1 = p.paph[ih1];
+    const R pap = p.pap[i2];
+    const R pt = p.pt[i2], pq = p.pq[i2], pa = p.pa[i2];
+    const R tcml_t = p.tcml_t[i2], tcml_q = p.tcml_q[i2], tcml_a = p.tcml_a[i2];
+    const R pvfl = p.pvfl[i2], pvfi = p.pvfi[i2];
+    const R pludeli1 = p.pludeli[b4 + (int64_t)k * nproma];
+    const R pludeli2 = p.pludeli[b4 + (int64_t)k * nproma + s2];
+
+    // ---- 1. initial values (:982-1002) ----
+    const R ztp1 = pt + ptsphy * tcml_t;
+    R qv = pq + ptsphy * tcml_q;
+    R za = pa + ptsphy * tcml_a;
+    const R zaorig = za;
+    const R ql0 = p.pclv[i5] + ptsphy * p.tcml_cld[i5];
+    const R qi0 = p.pclv[i5 + s2] + ptsphy * p.tcml_cld[i5 + s2];
+    const R qr0 = p.pclv[i5 + 2 * s2] + ptsphy * p.tcml_cld[i5 + 2 * s2];
+    const R qs0 = p.pclv[i5 + 3 * s2] + ptsphy * p.tcml_cld[i5 + 3 * s2];
+    R ql = ql0, qi = qi0, qr = qr0, qs = qs0;
+
+    // saturation values (:1087-1115)
+    const Sat<R> sat = saturation<R>(c, ztp1);
+    const R foeeliq = c.r2es * sat.el, foeeice = c.r2es * sat.ei;
+    const R zfoeewm = m_min(c.r2es * (sat.alfa * sat.el + (R(1) - sat.alfa) * sat.ei) / pap, R(0.5));
+    const R zqsmix = zfoeewm / (R(1) - c.retv * zfoeewm);
+    const R zdelta = (ztp1 - c.rtt >= R(0)) ? R(1) : R(0);
+    const R zfoeew = m_min((zdelta * foeeliq + (R(1) - zdelta) * foeeice) / pap, R(0.5));
+    const R zqsice = zfoeew / (R(1) - c.retv * zfoeew);
+    const R zfoeeliqt = m_min(foeeliq / pap, R(0.5));
+    const R zqsliq = zfoeeliqt / (R(1) - c.retv * zfoeeliqt);
+
+    // ---- 2. tidy up (:1127-1196) ----
+    R tt = R(0), tq = R(0);
+    R zlneg_l = R(0), zlneg_i = R(0), zlneg_r = R(0), zlneg_s = R(0);
+    if (ql + qi < rlmin || za < c.ramin) {
+      zlneg_l = zlneg_l + ql;
+      R zqadj = ql * zqtmst;
+      tq = tq + zqadj;
+      tt = tt - c.ralvdcp * zqadj;
+      qv = qv + ql;
+      ql = R(0);
+      zlneg_i = zlneg_i + qi;
+      zqadj = qi * zqtmst;
+      tq = tq + zqadj;
+      tt = tt - c.ralsdcp * zqadj;
+      qv = qv + qi;
+      qi = R(0);
+      za = R(0);
+    }
+#define CLOUDSC_TIDY(x, zlneg, rlat)   \
+  if (x < rlmin) {                     \
+    zlneg = zlneg + x;                 \
+    const R zqadj = x * zqtmst;        \
+    tq = tq + zqadj;                   \
+    tt = tt - rlat * zqadj;            \
+    qv = qv + x;                       \
+    x = R(0);                          \
+  }
+    CLOUDSC_TIDY(ql, zlneg_l, c.ralvdcp)
+    CLOUDSC_TIDY(qi, zlneg_i, c.ralsdcp)
+    CLOUDSC_TIDY(qr, zlneg_r, c.ralvdcp)
+    CLOUDSC_TIDY(qs, zlneg_s, c.ralsdcp)
+#undef CLOUDSC_TIDY
+    za = m_max(R(0), m_min(R(1), za));
+    const R zli = ql + qi;
+    R zliqfrac = R(0), zicefrac = R(0);
+    if (zli > rlmin) {
+      zliqfrac = ql / zli;
+      zicefrac = R(1) - zliqfrac;
+    }
+
+    // Outputs of this level; defaults are those of levels above NCLDTOP.
+    R qxn_l = R(0), qxn_i = R(0), qxn_r = R(0), qxn_s = R(0);
+    R tcld_l = R(0), tcld_i = R(0), tcld_r = R(0), tcld_s = R(0), ta = R(0);
+    // PFPLSL/PFPLSN/PFHPSL/PFHPSN at the upper half level (:4218-4224,4297-4302)
+    {
+      const R fl = pf_r + R(0), fn = pf_s + pf_i;
+      p.pfplsl[bh + (int64_t)k * nproma] = fl;
+      p.pfplsn[bh + (int64_t)k * nproma] = fn;
+      p.pfhpsl[bh + (int64_t)k * nproma] = -c.rlvtt * fl;
+      p.pfhpsn[bh + (int64_t)k * nproma] = -c.rlstt * fn;
+    }
+
+    if (k >= ncldtop0) {
+      const bool above_top = k > ncldtop0;  // JK > NCLDTOP
+      const R plude_in = p.plude[i2];
+      // derived variables (:1281-1324)
+      const R zdp = paph_k1 - paph_k;
+      const R zgdp = c.rg / zdp;
+      const R zrho = pap / (c.rd * ztp1);
+      const R zdtgdp = ptsphy * zgdp;
+      const R zrdtgdp = zdp * c.rdtg_r;
+      const R tl = ztp1 - c.r4les, ti = ztp1 - c.r4ies;
+      const R zfacw = c.r5les / (tl * tl);
+      R zcor = R(1) / (R(1) - c.retv * zfoeeliqt);
+      const R zdqsliqdt = zfacw * zcor * zqsliq;
+      const R zcorqsliq = R(1) + c.ralvdcp * zdqsliqdt;
+      const R zfaci = c.r5ies / (ti * ti);
+      zcor = R(1) / (R(1) - c.retv * zfoeew);
+      const R zdqsicedt = zfaci * zcor * zqsice;
+      const R zcorqsice = R(1) + c.ralsdcp * zdqsicedt;
+      const R zfac = sat.alfa * zfacw + (R(1) - sat.alfa) * zfaci;
+      zcor = R(1) / (R(1) - c.retv * zfoeewm);
+      const R zdqsmixdt = zfac * zcor * zqsmix;
+      const R foeldcpm = sat.alfa * c.ralvdcp + (R(1) - sat.alfa) * c.ralsdcp;
+      const R zcorqsmix = R(1) + foeldcpm * zdqsmixdt;
+      const R zevaplimmix = m_max((zqsmix - qv) / zcorqsmix, R(0));
+      const R zevaplimice = m_max((zqsice - qv) / zcorqsice, R(0));
+      (void)zevaplimice;
+      const R ztmpa = R(1) / m_max(za, R(0.01));
+      R zliqcld = ql * ztmpa;
+      R zicecld = qi * ztmpa;
+      const R zlicld = zliqcld + zicecld;
+
+      // Explicit (ZSOLQA) entries: vX = ZSOLQA(V,X) = -ZSOLQA(X,V); il = ZSOLQA(I,L);
+      // ri = ZSOLQA(R,I); rs = ZSOLQA(R,S); dX = ZSOLQA(X,X).  Implicit (ZSOLQB) entries qbXY.
+      R vl = R(0), vi = R(0), vr = R(0), vs = R(0), il = R(0), ri = R(0), rs = R(0);
+      R dl = R(0), di = R(0), dr = R(0), ds = R(0);
+      R qbll = R(0), qbii = R(0), qbrl = R(0), qbsi = R(0), qbsl = R(0);
+      R zsolab = R(0), zsolac = R(0);
+      R csrc_l = R(0), csrc_i = R(0), csrc_r = R(0), csrc_s = R(0), csink = R(0);
+      R fsrc_i = R(0), fsrc_r = R(0), fsrc_s = R(0);
+      R qxfg_l = ql, qxfg_i = qi, qxfg_r = qr, qxfg_s = qs;
+      R zqpretot = R(0);
+
+      if (ql < rlmin) vl = ql;  // :1334-1342
+      if (qi < rlmin) vi = qi;
+
+      // 3.2 detrainment (:1362-1398)
+      R plude = plude_in;
+      if (has_next) {
+        plude = plude_in * zdtgdp;
+        if (ldcum && plude > zepsec && plu_n > zepsec) {
+          zsolac = zsolac + plude / plu_n;
+          csrc_l = pludeli1 * zdtgdp;
+          csrc_i = pludeli2 * zdtgdp;
+          dl = dl + csrc_l;
+          di = di + csrc_i;
+        } else {
+          plude = R(0);
+        }
+        if (c.lmfdsnow) {
+          csrc_r = p.psnde[b2x + (int64_t)k * nproma] * zdtgdp;
+          csrc_s = p.psnde[b2x + (int64_t)k * nproma + s2] * zdtgdp;
+          dr = dr + csrc_r;
+          ds = ds + csrc_s;
+        }
+        p.plude[i2] = plude;
+      }
+
+      // 3.3 subsidence from the level above (:1419-1492)
+      if (above_top) {
+        const R zmf = m_max(R(0), (pmfu_k * zadvw + pmfd_k * zadvwd) * zdtgdp);
+        const R zacust = zmf * zanewm1;
+        const R lcust_l = zmf * qxnm1_l, lcust_i = zmf * qxnm1_i;
+        csrc_l = csrc_l + lcust_l;
+        csrc_i = csrc_i + lcust_i;
+        const R zdtdp = c.zrdcp * R(0.5) * (t_prev + ztp1) / paph_k;
+        const R zwtot = m_min(c.rg * (pmfu_k * zadvw + pmfd_k * zadvwd), (pap - pap_prev) / ptsphy);
+        const R zdtforc = zdtdp * ptsphy * zwtot;
+        const R zdqs = zanewm1 * zdtforc * zdqsmixdt;
+        R zlfinalsum = R(0);
+        {
+          R zlfinal = m_max(R(0), lcust_l - zdqs);
+          const R zevap = m_min(lcust_l - zlfinal, zevaplimmix);
+          zlfinal = lcust_l - zevap;
+          zlfinalsum = zlfinalsum + zlfinal;
+          dl = dl + lcust_l;
+          vl = vl + zevap;
+        }
+        {
+          R zlfinal = m_max(R(0), lcust_i - zdqs);
+          const R zevap = m_min(lcust_i - zlfinal, zevaplimmix);
+          zlfinal = lcust_i - zevap;
+          zlfinalsum = zlfinalsum + zlfinal;
+          di = di + lcust_i;
+          vi = vi + zevap;
+        }
+        zsolac = zsolac + zacust;
+        if (zlfinalsum < zepsec) zsolac = zsolac - zacust;
+      }
+      // subsidence sink to the level below, implicit (:1499-1514)
+      if (has_next) {
+        const R zmfdn = m_max(R(0), (pmfu_n * zadvw + pmfd_n * zadvwd) * zdtgdp);
+        zsolab = zsolab + zmfdn;
+        qbll = qbll + zmfdn;
+        qbii = qbii + zmfdn;
+        csink = zmfdn;
+      }
+
+      // 3.4 turbulent erosion, ITURBEROSION = 3 (:1689-1754)
+      {
+        R zldifdt = c.rcldiff * ptsphy;
+        if (ktype >= 2 && plude > zepsec && peis < R(10.0)) {
+          zldifdt = R(20.0) * c.rcldiff_convi * zldifdt;
+        } else if (ktype == 1 && plude > zepsec) {
+          zldifdt = R(2.0) * c.rcldiff_convi * zldifdt;
+        }
+        if (zli > zepsec) {
+          const R zqe = (qv - za * zqsmix) / m_max(zepsec, R(1) - za);
+          const R ze = m_max(zqsmix - zqe, R(0));
+          R zleros = R(0.333) * za * (R(1) - za) * zldifdt * ze;
+          zleros = m_min(zleros, zevaplimmix);
+          zleros = m_min(zleros, zli);
+          const R zaeros = R(0.5) * zleros / zlicld;
+          zsolac = zsolac - zaeros;
+          vl = vl + zliqfrac * zleros;
+          vi = vi + zicefrac * zleros;
+        }
+      }
+
+      // 4.1 sedimentation (:1783-1877)
+      const R zfallcorr = m_pow(c.rdensref / zrho, R(0.4));
+      const R ztemp_r = R(0.5) * (qr + qxnm1_r);
+      const R zrlambda = m_pow(zrho * ztemp_r / c.rcl_lam1r, c.rcl_lam2r);
+      const R vfall_r = c.rxfallspeedrain * zfallcorr * c.rcl_const7r * m_pow(zrlambda, c.rcl_dr);
+      const R vfall_i = c.rvice, vfall_s = c.rvsnow;
+      if (above_top) {
+        fsrc_i = pf_i * zdtgdp;
+        di = di + fsrc_i;
+        qxfg_i = qxfg_i + fsrc_i;
+        zqpretot = zqpretot + qxfg_i;
+        fsrc_r = pf_r * zdtgdp;
+        dr = dr + fsrc_r;
+        qxfg_r = qxfg_r + fsrc_r;
+        zqpretot = zqpretot + qxfg_r;
+        fsrc_s = pf_s * zdtgdp;
+        ds = ds + fsrc_s;
+        qxfg_s = qxfg_s + fsrc_s;
+        zqpretot = zqpretot + qxfg_s;
+      }
+      const R fsink_i = zdtgdp * zrho * vfall_i;
+      const R fsink_r = zdtgdp * zrho * vfall_r;
+      const R fsink_s = zdtgdp * zrho * vfall_s;
+
+      // 4.2 precipitation fraction (:1898-1919)
+      R zcovpclr = R(0), zraincld = R(0), zraincldm1 = R(0), zsnowcld = R(0);
+      if (zqpretot > zepsec) {
+        zcovptot = R(1) - ((R(1) - zcovptot) * (R(1) - m_max(za, za_prev)) /
+                           (R(1) - m_min(za_prev, R(1) - R(1.E-06))));
+        zcovptot = m_max(zcovptot, c.rcovpmin);
+        zcovpclr = m_max(R(0), zcovptot - za);
+        zraincld = R(0.5) * (qr + qr_prev) / zcovptot;
+        zraincldm1 = R(0.5) * (qr + qxnm1_r) / zcovptot;
+        zsnowcld = R(0.5) * (qs + qs_prev) / zcovptot;
+        zcovpmax = m_max(zcovptot, zcovpmax);
+      } else {
+        zcovptot = R(0);
+        zcovpmax = R(0);
+      }
+
+      // 4.3 warm rain, implicit KK2000 (:2050-2254)
+      if (zliqcld > zepsec) {
+        const R zrainaut = c.c_aut * m_pow(zliqcld, c.rcl_kkbauq - R(1)) * zconst_pow;
+        qbrl = qbrl + zrainaut;
+        R zrainacc = R(0);
+        if (zraincld > zepsec)
+          zrainacc = c.c_acc * m_pow(zliqcld, c.rcl_kkbac - R(1)) * m_pow(zraincld, c.rcl_kkbac);
+        qbrl = qbrl + zrainacc;
+      }
+
+      // 4.4 rain evaporation (:2271-2378)
+      {
+        R zzrh = c.rprecrhmax + (R(1) - c.rprecrhmax) * zcovpmax / m_max(zepsec, R(1) - za);
+        zzrh = m_min(m_max(zzrh, c.rprecrhmax), R(1));
+        R zrhc = c.ramid;
+        const R zsigk = pap / paph_surf;
+        if (zsigk > R(0.8)) {
+          const R s = (zsigk - R(0.8)) / R(0.2);
+          zrhc = c.ramid + (R(1) - c.ramid) * (s * s);
+        }
+        zzrh = m_min(zrhc, zzrh);
+        zzrh = m_min(R(0.9), zzrh);
+        const R zqe = m_max(R(0), m_min(qv, zqsliq));
+        if (zcovpclr > zepsec && zraincldm1 > zepsec && zqe < zzrh * zqsliq) {
+          const R zpreclr = zraincldm1;
+          const R zfallcorr2 = m_pow(c.rdensref / zrho, R(0.4f));
+          const R zesatliq = c.rv_rd * foeeliq;
+          const R zlambda = m_pow(c.rcl_lam1r / (zrho * zpreclr), c.rcl_lam2r);
+          const R t = ztp1;
+          const R zevap_denom = c.rcl_cdenom1 * zesatliq - c.rcl_cdenom2 * t * zesatliq + c.rcl_cdenom3 * (t * t * t) * pap;
+          const R zcorr2 = m_pow(t / R(273.), R(1.5)) * R(393.) / (t + R(120.));
+          const R zsubsat = m_max(zzrh * zqsliq - zqe, R(0));
+          const R zbeta = (R(0.15) / zqsliq) * (t * t) * zesatliq * c.rcl_const1r * (zcorr2 / zevap_denom) *
+                          (R(0.78) / m_pow(zlambda, c.rcl_const4r) +
+                           c.rcl_const2r * m_sqrt(zrho * zfallcorr2) / (m_sqrt(zcorr2) * m_pow(zlambda, c.rcl_const3r)));
+          const R zdenom = R(1) + zbeta * ptsphy * zcorqsliq;
+          const R zdpevap = zcovpclr * zbeta * ptsphy * zsubsat / zdenom;
+          const R zevap = m_min(zdpevap, qxfg_r);
+          vr = vr + zevap;
+          qxfg_r = qxfg_r - zevap;
+        }
+      }
+
+      // 4.5 ice deposition, IDEPICE = 1 (:2404-2499)
+      const R zvpice = foeeice * c.rv / c.rd;
+      {
+        if (za_prev < c.rcldtopcf && za >= c.rcldtopcf) {
+          zcldtopdist = R(0);
+        } else {
+          zcldtopdist = zcldtopdist + zdp / (zrho * c.rg);
+        }
+        if (ztp1 < c.rtt - R(5.) && qxfg_l > rlmin) {
+          const R zsupersatice = (zqsliq - zqsice) / zqsice;
+          const R zicenuclei = R(1000.0) * m_exp(R(12.96) * zsupersatice - R(0.639));
+          const R t = ztp1;
+          const R zadd = c.rlstt * (c.rlstt / (c.rv * t) - R(1)) / (R(2.4E-2) * t);
+          const R zbdd = c.rv * t * pap / (R(2.21) * zvpice);
+          const R zcvds = R(7.8) * m_pow(zicenuclei / zrho, R(0.666)) * zsupersatice / (R(8.87) * (zadd + zbdd));
+          const R zice0 = m_max(zicecld, zicenuclei * c.riceinit / zrho);
+          const R zinew = m_pow(R(0.666) * zcvds * ptsphy + m_pow(zice0, R(0.666)), R(1.5));
+          R zdepos = m_max(c.rcl_overlapliqice * za * (zinew - zice0), R(0));
+          zdepos = m_min(zdepos, qxfg_l);
+          const R zinfactor = m_min(zicenuclei / R(15000.), R(1));
+          zdepos = zdepos * m_min(zinfactor + (R(1) - zinfactor) * (c.rdepliqrefrate + zcldtopdist / c.rdepliqrefdepth), R(1));
+          il = il + zdepos;
+          qxfg_i = qxfg_i + zdepos;
+          qxfg_l = qxfg_l - zdepos;
+        }
+      }
+
+      // 4.7 deposition onto snow, IDEPSNOW = 1 (:2700-2745)
+      {
+        const R zsupersatice = m_max((qv - zqsmix) / zqsmix, R(0));
+        if (ztp1 < c.rtt && zsnowcld > zepsec && zsupersatice > R(0)) {
+          const R t = ztp1;
+          const R zaplusb = c.rcl_apb1 * zvpice - c.rcl_apb2 * zvpice * t + pap * c.rcl_apb3 * (t * t * t);
+          const R zcorrfac = m_sqrt(R(1) / zrho);
+          const R zcorrfac2 = m_pow(t / R(273.0), R(1.5)) * (R(393.0) / (t + R(120.0)));
+          const R zpr02 = zrho * zsnowcld * c.rcl_const1s;
+          const R zterm1 = zsupersatice * (t * t) * zvpice * zcorrfac2 * c.rcl_const2s / (zrho * zaplusb);
+          const R zterm2 = R(0.65) * c.rcl_const6s * m_pow(zpr02, c.rcl_const4s) +
+                           c.rcl_const3s * m_sqrt(zcorrfac) * m_sqrt(zrho) * m_pow(zpr02, c.rcl_const5s) / m_sqrt(zcorrfac2);
+          R zpsdep = m_max(zcovptot * zterm1 * zterm2 * ptsphy, R(0));
+          zpsdep = m_min(zpsdep, qv - zqsmix) / zcorqsmix;
+          vs = vs - zpsdep;  // ZSOLQA(S,V) += ZPSDEP, ZSOLQA(V,S) -= ZPSDEP
+        }
+      }
+
+      // revised in-cloud condensate (:2750-2754)
+      zliqcld = qxfg_l * ztmpa;
+      zicecld = qxfg_i * ztmpa;
+
+      // 4.8 ice -> snow autoconversion (:2769-2800)
+      if (ztp1 <= c.rtt && zicecld > zepsec) {
+        const R zzco = ptsphy * c.rsnowlin1 * m_exp(c.rsnowlin2 * (ztp1 - c.rtt));
+        const R r = zicecld / c.rlcritsnow;
+        const R ztemp = R(-1.) * (r * r);
+        const R zsnowaut = (ztemp < R(50.0)) ? zzco * (R(1) - m_exp(ztemp)) : zzco;
+        qbsi = qbsi + zsnowaut;
+      }
+
+      // 4.12 melting of ice and snow (:2815-2862)
+      {
+        const R zicetot = qxfg_i + qxfg_s;
+        R zmeltmax = R(0);
+        if (zicetot > zepsec && ztp1 > c.rtt) {
+          const R zsubsat = m_max(zqsice - qv, R(0));
+          const R ztdmtw0 = ztp1 - c.rtt - zsubsat * (R(1329.31) + R(0.0074615) * (pap - R(0.85E5)) - R(40.637) * (ztp1 - R(275.0)));
+          const R zcons1 = m_abs(ptsphy * (R(1) + R(0.5) * ztdmtw0) / c.rtaumel);
+          zmeltmax = m_max(ztdmtw0 * zcons1 * c.zrldcp, R(0));
+        }
+        if (zmeltmax > zepsec && zicetot > zepsec) {
+          R zalfa = qxfg_i / zicetot;
+          R zmelt = m_min(qxfg_i, zalfa * zmeltmax);
+          qxfg_i = qxfg_i - zmelt;
+          qxfg_r = qxfg_r + zmelt;
+          ri = ri + zmelt;  // ZSOLQA(R,I) += melt
+          zalfa = qxfg_s / zicetot;
+          zmelt = m_min(qxfg_s, zalfa * zmeltmax);
+          qxfg_s = qxfg_s - zmelt;
+          qxfg_r = qxfg_r + zmelt;
+          rs = rs + zmelt;  // ZSOLQA(R,S) += melt
+        }
+      }
+
+      // 4.13 freezing of rain (:2877-2933)
+      if (qr > zepsec) {
+        if (ztp1 <= c.rtt && t_prev > c.rtt) {
+          const R zqp = m_max(qs + qr, zepsec);
+          prainfrac = qr / zqp;
+          llrainliq = prainfrac > R(0.8f);
+        }
+        if (ztp1 < c.rtt) {
+          R zfrzmax;
+          if (llrainliq) {
+            const R zlambda = m_pow(c.rcl_lam1r / (zrho * qr), c.rcl_lam2r);
+            const R ztemp = m_min(c.rcl_fzrab * (ztp1 - c.rtt), R(50.));
+            const R zfrz = ptsphy * (c.rcl_const5r / zrho) * (m_exp(ztemp) - R(1.)) * m_pow(zlambda, c.rcl_const6r);
+            zfrzmax = m_max(zfrz, R(0));
+          } else {
+            const R zcons1 = m_abs(ptsphy * (R(1) + R(0.5) * (c.rtt - ztp1)) / c.rtaumel);
+            zfrzmax = m_max((c.rtt - ztp1) * zcons1 * c.zrldcp, R(0));
+          }
+          if (zfrzmax > zepsec) {
+            const R zfrz = m_min(qr, zfrzmax);
+            rs = rs - zfrz;  // ZSOLQA(S,R) += frz
+          }
+        }
+      }
+
+      // 4.14 homogeneous freezing of cloud liquid (:2948-2960)
+      {
+        const R zfrzmax = m_max((c.rthomo - ztp1) * c.zrldcp, R(0));
+        if (zfrzmax > zepsec && qxfg_l > zepsec) {
+          const R zfrz = m_min(qxfg_l, zfrzmax);
+          il = il + zfrz;
+        }
+      }
+
+      // 4.9 riming of snow (:2975-3017)
+      if (ztp1 <= c.rtt && zliqcld > zepsec) {
+        if (zsnowcld > zepsec && zcovptot > R(0.01)) {
+          const R zfallcorr3 = m_pow(c.rdensref / zrho, R(0.4f));
+          R zsnowrime = c.rcl_effrime * zcovptot * ptsphy * c.rcl_const7s * zfallcorr3 *
+                        m_pow(zrho * zsnowcld * c.rcl_const1s, c.rcl_const8s);
+          zsnowrime = m_min(zsnowrime, R(1));
+          qbsl = qbsl + zsnowrime;
+        }
+      }
+
+      // PSD slopes, PSACR and PIACR (:3029-3198)
+      if (ztp1 <= c.rtt && prainfrac < zepsec && zraincld > zepsec) {
+        const R zlamr = m_pow(c.rcl_lam1r / (zrho * zraincld), c.rcl_lam2r);
+        const R zn0r = c.rcl_x1r * m_pow(zlamr, c.rcl_x2r);
+        const R lr3 = zlamr * zlamr * zlamr;
+        if (zsnowcld > zepsec) {
+          const R zlams = m_pow(c.rcl_lam1s / (zrho * zsnowcld), c.rcl_lam2s);
+          const R zfalldiff = m_max(m_abs(vfall_r - vfall_s), (vfall_r + vfall_s) / R(8.));
+          const R zfac = c.rpi * c.rpi * c.rdenswat * R(1.0) * zrho * zn0r * c.rcl_x1s / (zlams * lr3);
+          const R zrat = zlamr / zlams;
+          const R zpsacr = zcovptot * ptsphy * zfac * zfalldiff * ((R(0.5) * zrat + R(2.)) * zrat + R(5.)) / lr3;
+          rs = rs - zpsacr;  // ZSOLQA(S,R) += psacr
+        }
+        if (zicecld > zepsec) {
+          const R zlami = m_pow(c.rcl_lam1s / (zrho * zicecld), c.rcl_lam2s);
+          const R zfalldiff = m_max(m_abs(vfall_r - vfall_i), (vfall_r + vfall_i) / R(8.));
+          const R zfac = c.rpi * c.rpi * c.rdenswat * R(0.1) * zrho * zn0r * c.rcl_x1s / (zlami * lr3);
+          const R zrat = zlamr / zlami;
+          const R zpiacr = zcovptot * ptsphy * zfac * zfalldiff * ((R(0.5) * zrat + R(2.)) * zrat + R(5.)) / lr3;
+          ri = ri - zpiacr;  // ZSOLQA(I,R) += piacr
+        }
+      }
+
+      // 4.15 snow sublimation, ISUBLSNOW = 1 (:3254-3318)
+      {
+        R zzrh = c.rprecrhmax + (R(1) - c.rprecrhmax) * zcovpmax / m_max(zepsec, R(1) - za);
+        zzrh = m_min(m_max(zzrh, c.rprecrhmax), R(1));
+        R zqe = (qv - za * zqsice) / m_max(zepsec, R(1) - za);
+        zqe = m_max(R(0), m_min(zqe, zqsice));
+        if (zcovpclr > zepsec && qxfg_s > zepsec && zqe < zzrh * zqsice) {
+          const R dd = zcovptot * zdtgdp;
+          const R den = m_max(m_abs(dd), c.zepsilon);
+          const R zpreclr = qxfg_s * zcovpclr / (dd < R(0) ? -den : den);
+          const R zbeta1 = m_sqrt(pap / paph_surf) / c.rvrfactor * zpreclr / m_max(zcovpclr, zepsec);
+          const R zbeta = c.rg * c.rpecons * m_pow(zbeta1, R(0.5777));
+          const R zdenom = R(1) + zbeta * ptsphy * zcorqsice;
+          const R zdpr = zcovpclr * zbeta * (zqsice - zqe) / zdenom * zdp * c.zrg_r;
+          const R zdpevap = zdpr * zdtgdp;
+          const R zevap = m_min(zdpevap, qxfg_s);
+          vs = vs + zevap;
+          zcovptot = m_max(c.rcovpmin, zcovptot - m_max(R(0), (zcovptot - za) * zevap / qxfg_s));
+          qxfg_s = qxfg_s - zevap;
+        }
+      }
+
+      // evaporate small precipitation amounts (:3402-3412)
+      if (qxfg_i < rlmin) vi = vi + qxfg_i;
+      if (qxfg_r < rlmin) vr = vr + qxfg_r;
+      if (qxfg_s < rlmin) vs = vs + qxfg_s;
+
+      // 5.1 cloud cover (:3429-3438)
+      R zanew = (za + zsolac) / (R(1) + zsolab);
+      zanew = m_min(zanew, R(1));
+      if (zanew < c.ramin) zanew = R(0);
+      const R zda = zanew - zaorig;
+      zanewm1 = zanew;
+
+      // 5.2.1 truncation of explicit sinks (:3455-3536).  Row sums of ZSOLQA
+      // (minus sign: sinks positive), in the reference's column order L,I,R,S,V.
+      {
+        const R qxs[5] = {ql, qi, qr, qs, qv};
+        R rsum[5];
+        rsum[0] = ((R(0) - dl) + il) + vl;        // row L: (L,L)=dl (L,I)=-il (L,V)=-vl
+        rsum[1] = (((R(0) - il) - di) + ri) + vi; // row I: il, di, (I,R)=-ri, (I,V)=-vi
+        rsum[2] = (((R(0) - ri) - dr) - rs) + vr; // row R: ri, dr, rs, (R,V)=-vr
+        rsum[3] = ((R(0) + rs) - ds) + vs;        // row S: (S,R)=-rs, ds, (S,V)=-vs
+        rsum[4] = (((R(0) - vl) - vi) - vr) - vs; // row V: vl, vi, vr, vs
+        R zratio[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          const R zmax = m_max(qxs[m], zepsec);
+          zratio[m] = zmax / m_max(rsum[m], zmax);
+        }
+        bool any_trunc = false;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) any_trunc = any_trunc || (zratio[m] < R(1));
+        if (any_trunc) {
+          // stable ascending order of zratio (insertion sort of :3479-3495)
+          int rank[5];
+#pragma unroll
+          for (int m = 0; m < 5; ++m) {
+            int r = 0;
+#pragma unroll
+            for (int n = 0; n < 5; ++n) r += (zratio[n] < zratio[m] || (n < m && zratio[n] == zratio[m])) ? 1 : 0;
+            rank[m] = r;
+          }
+#pragma unroll
+          for (int s = 0; s < 5; ++s) {
+            int jo = 0;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) jo = (rank[m] == s) ? m : jo;
+            R ssum;
+            R zmm;
+            if (jo == 0) {
+              ssum = ((R(0) - dl) + il) + vl;
+              zmm = m_max(ql, zepsec);
+            } else if (jo == 1) {
+              ssum = (((R(0) - il) - di) + ri) + vi;
+              zmm = m_max(qi, zepsec);
+            } else if (jo == 2) {
+              ssum = (((R(0) - ri) - dr) - rs) + vr;
+              zmm = m_max(qr, zepsec);
+            } else if (jo == 3) {
+              ssum = ((R(0) + rs) - ds) + vs;
+              zmm = m_max(qs, zepsec);
+            } else {
+              ssum = (((R(0) - vl) - vi) - vr) - vs;
+              zmm = m_max(qv, zepsec);
+            }
+            const R zz = zmm / m_max(ssum, zmm);
+            // off-diagonal pairs (a,b) with stored x = ZSOLQA(a,b)
+            vl = trunc_scale(vl, jo, 4, 0, zz);
+            vi = trunc_scale(vi, jo, 4, 1, zz);
+            vr = trunc_scale(vr, jo, 4, 2, zz);
+            vs = trunc_scale(vs, jo, 4, 3, zz);
+            il = trunc_scale(il, jo, 1, 0, zz);
+            ri = trunc_scale(ri, jo, 2, 1, zz);
+            rs = trunc_scale(rs, jo, 2, 3, zz);
+            // a negative diagonal entry is scaled twice (:3531-3532)
+            if (jo == 0 && dl < R(0)) dl = (dl * zz) * zz;
+            if (jo == 1 && di < R(0)) di = (di * zz) * zz;
+            if (jo == 2 && dr < R(0)) dr = (dr * zz) * zz;
+            if (jo == 3 && ds < R(0)) ds = (ds * zz) * zz;
+          }
+        }
+      }
+
+      // 5.2.2 implicit solve (:3546-3628) in RD; ZQLHS is lower triangular here.
+      R qxn_v_solve;
+      {
+        const RD one = RD(1);
+        const RD dL = ((one + RD(qbll)) + RD(qbrl)) + RD(qbsl);
+        const RD dI = ((one + RD(fsink_i)) + RD(qbii)) + RD(qbsi);
+        const RD dR = one + RD(fsink_r);
+        const RD dS = one + RD(fsink_s);
+        const RD xL = RD(ql) + ((RD(dl) - RD(il)) - RD(vl));
+        const RD xI = RD(qi) + (((RD(il) + RD(di)) - RD(ri)) - RD(vi));
+        const RD xR = RD(qr) + (((RD(ri) + RD(dr)) + RD(rs)) - RD(vr));
+        const RD xS = RD(qs) + ((-RD(rs) + RD(ds)) - RD(vs));
+        const RD xV = RD(qv) + (((RD(vl) + RD(vi)) + RD(vr)) + RD(vs));
+        const RD lRL = -RD(qbrl) / dL, lSL = -RD(qbsl) / dL, lSI = -RD(qbsi) / dI;
+        const RD yR = xR - lRL * xL;
+        const RD yS = (xS - lSL * xL) - lSI * xI;
+        qxn_l = R(xL / dL);
+        qxn_i = R(xI / dI);
+        qxn_r = R(yR / dR);
+        qxn_s = R(yS / dS);
+        qxn_v_solve = R(xV);  // row V of ZQLHS is the identity
+      }
+      R qxn_v = qxn_v_solve;
+      // clip small values (:3633-3640)
+      if (qxn_l < zepsec) { qxn_v = qxn_v + qxn_l; qxn_l = R(0); }
+      if (qxn_i < zepsec) { qxn_v = qxn_v + qxn_i; qxn_i = R(0); }
+      if (qxn_r < zepsec) { qxn_v = qxn_v + qxn_r; qxn_r = R(0); }
+      if (qxn_s < zepsec) { qxn_v = qxn_v + qxn_s; qxn_s = R(0); }
+      qxnm1_l = qxn_l;
+      qxnm1_i = qxn_i;
+      qxnm1_r = qxn_r;
+      qxnm1_s = qxn_s;
+
+      // 5.3 fluxes to the next level (:3658-3672)
+      pf_i = fsink_i * qxn_i * zrdtgdp;
+      pf_r = fsink_r * qxn_r * zrdtgdp;
+      pf_s = fsink_s * qxn_s * zrdtgdp;
+      if (pf_s + pf_r < zepsec) zcovptot = R(0);
+
+      // 6.1-6.3 tendencies (:3713-3758)
+      {
+        const R fq_l = csrc_l - csink * qxn_l;
+        const R fq_i = (csrc_i + fsrc_i) - (fsink_i + csink) * qxn_i;
+        const R fq_r = (csrc_r + fsrc_r) - fsink_r * qxn_r;
+        const R fq_s = (csrc_s + fsrc_s) - fsink_s * qxn_s;
+        tt = tt + c.ralvdcp * (qxn_l - ql - fq_l) * zqtmst;
+        tcld_l = (qxn_l - ql0) * zqtmst;
+        tt = tt + c.ralsdcp * (qxn_i - qi - fq_i) * zqtmst;
+        tcld_i = (qxn_i - qi0) * zqtmst;
+        tt = tt + c.ralvdcp * (qxn_r - qr - fq_r) * zqtmst;
+        tcld_r = (qxn_r - qr0) * zqtmst;
+        tt = tt + c.ralsdcp * (qxn_s - qs - fq_s) * zqtmst;
+        tcld_s = (qxn_s - qs0) * zqtmst;
+        tq = tq + (qxn_v - qv) * zqtmst;
+        ta = zda * zqtmst;
+      }
+
+      // 6.4 CLOUD_SUPERSATCHECK on the updated state, FP64 islands in RD (:3765-3777)
+      {
+        const RD zq_upd = RD(pq) + RD(ptsphy) * (RD(tq) + RD(tcml_q));
+        const RD zt_upd = RD(pt) + RD(ptsphy) * (RD(tt) + RD(tcml_t));
+        const RD za_upd = RD(pa) + RD(ptsphy) * (RD(ta) + RD(tcml_a));
+        const R pp = ssc_off ? p.pap[i2 + ssc_off] : pap;
+        RD t_adj, q_adj, a_adj, l_adj, i_adj;
+        supersatcheck<RD>(cd, zt_upd, zq_upd, za_upd, RD(pp), t_adj, q_adj, a_adj, l_adj, i_adj);
+        tq = tq + R(q_adj) * zqtmst;
+        tt = tt + R(t_adj) * zqtmst;
+        ta = ta + R(a_adj) * zqtmst;
+        tcld_l = tcld_l + R(l_adj) * zqtmst;
+        tcld_i = tcld_i + R(i_adj) * zqtmst;
+      }
+      // negative clean-up (:3793-3807)
+      {
+        const R zl = ql0 + tcld_l * ptsphy;
+        if (zl < R(0)) {
+          tcld_l = tcld_l - zl * zqtmst;
+          tq = tq + zl * zqtmst;
+          tt = tt - c.ralvdcp * zl * zqtmst;
+        }
+        const R zi = qi0 + tcld_i * ptsphy;
+        if (zi < R(0)) {
+          tcld_i = tcld_i - zi * zqtmst;
+          tq = tq + zi * zqtmst;
+          tt = tt - c.ralsdcp * zi * zqtmst;
+        }
+      }
+      p.pcovptot[i2] = zcovptot;  // (:3814-3816)
+    }
+
+    // store tendencies of this level (levels above NCLDTOP keep the tidy-up terms only)
+    p.tloc_t[i2] = tt;
+    p.tloc_q[i2] = tq;
+    p.tloc_a[i2] = ta;
+    p.tloc_cld[i5] = tcld_l;
+    p.tloc_cld[i5 + s2] = tcld_i;
+    p.tloc_cld[i5 + 2 * s2] = tcld_r;
+    p.tloc_cld[i5 + 3 * s2] = tcld_s;
+
+    // DDH fluxes at the lower half level (:4244-4291)
+    {
+      const R zgdph_r = -c.zrg_r * (paph_k1 - paph_k) * zqtmst;
+      const R sqrf = sqlf + (qxn_r - qr0) * zgdph_r;
+      const R sqsf = sqif + (qxn_s - qs0) * zgdph_r;
+      const R cqrng = cqlng + zlneg_r * zgdph_r;
+      const R cqsng = cqnng + zlneg_s * zgdph_r;
+      sqlf = sqlf + (qxn_l - ql0 + pvfl * ptsphy) * zgdph_r + pludeli1;
+      cqlng = cqlng + zlneg_l * zgdph_r;
+      sqltur = sqltur + pvfl * ptsphy * zgdph_r;
+      sqif = sqif + (qxn_i - qi0 + pvfi * ptsphy) * zgdph_r + pludeli2;
+      cqnng = cqnng + zlneg_i * zgdph_r;
+      sqitur = sqitur + pvfi * ptsphy * zgdph_r;
+      p.pfsqlf[ih1] = sqlf;
+      p.pfsqif[ih1] = sqif;
+      p.pfsqrf[ih1] = sqrf;
+      p.pfsqsf[ih1] = sqsf;
+      p.pfcqlng[ih1] = cqlng;
+      p.pfcqnng[ih1] = cqnng;
+      p.pfcqrng[ih1] = cqrng;
+      p.pfcqsng[ih1] = cqsng;
+      p.pfsqltur[ih1] = sqltur;
+      p.pfsqitur[ih1] = sqitur;
+    }
+
+    // carry to the next level
+    za_prev = za;
+    qr_prev = qr;
+    qs_prev = qs;
+    t_prev = ztp1;
+    pap_prev = pap;
+    paph_k = paph_k1;
+    pmfu_k = pmfu_n;
+    pmfd_k = pmfd_n;
+  }
+  // precipitation fluxes at the surface half level
+  {
+    const int64_t ih = bh + (int64_t)klev * nproma;
+    const R fl = pf_r + R(0), fn = pf_s + pf_i;
+    p.pfplsl[ih] = fl;
+    p.pfplsn[ih] = fn;
+    p.pfhpsl[ih] = -c.rlvtt * fl;
+    p.pfhpsn[ih] = -c.rlstt * fn;
+  }
+  p.prainfrac_toprfz[b1] = prainfrac;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (declared in cloudsc_launch.h)
+// ---------------------------------------------------------------------------
+template <typename T>
+KConst<T> make_kconst(const cloudsc_params_t& q, double ptsphy_in) {
+  KConst<T> c{};
+  const T ptsphy = T(ptsphy_in);
+  c.rg = T(q.rg); c.rd = T(q.rd); c.rv = T(q.rv); c.retv = T(q.retv); c.rtt = T(q.rtt);
+  c.rlvtt = T(q.rlvtt); c.rlstt = T(q.rlstt); c.rpi = T(q.rpi);
+  c.rv_rd = c.rv / c.rd;
+  c.r2es = T(q.r2es); c.r3les = T(q.r3les); c.r3ies = T(q.r3ies); c.r4les = T(q.r4les); c.r4ies = T(q.r4ies);
+  c.r5les = T(q.r5les); c.r5ies = T(q.r5ies); c.r5alvcp = T(q.r5alvcp); c.r5alscp = T(q.r5alscp);
+  c.ralvdcp = T(q.ralvdcp); c.ralsdcp = T(q.ralsdcp); c.rtwat = T(q.rtwat); c.rtice = T(q.rtice);
+  c.rtwat_rtice_r = T(q.rtwat_rtice_r); c.rkoop1 = T(q.rkoop1); c.rkoop2 = T(q.rkoop2);
+  c.ramid = T(q.ramid); c.ramin = T(q.ramin); c.rlmin = T(q.rlmin); c.rcldiff = T(q.rcldiff);
+  c.rcldiff_convi = T(q.rcldiff_convi); c.rkconv = T(q.rkconv);
+  c.rsnowlin1 = T(q.rsnowlin1); c.rsnowlin2 = T(q.rsnowlin2); c.rlcritsnow = T(q.rlcritsnow);
+  c.rvice = T(q.rvice); c.rvrain = T(q.rvrain); c.rvsnow = T(q.rvsnow); c.rxfallspeedrain = T(q.rxfallspeedrain);
+  c.rcldtopcf = T(q.rcldtopcf); c.rdepliqrefrate = T(q.rdepliqrefrate); c.rdepliqrefdepth = T(q.rdepliqrefdepth);
+  c.rcl_overlapliqice = T(q.rcl_overlapliqice); c.rcl_effrime = T(q.rcl_effrime); c.riceinit = T(q.riceinit);
+  c.rthomo = T(q.rthomo); c.rprecrhmax = T(q.rprecrhmax); c.rpecons = T(q.rpecons); c.rvrfactor = T(q.rvrfactor);
+  c.rcovpmin = T(q.rcovpmin); c.rtaumel = T(q.rtaumel); c.rssicefactor = T(q.rssicefactor);
+  c.rdenswat = T(q.rdenswat); c.rdensref = T(q.rdensref);
+  c.rcl_dr = T(q.rcl_dr); c.rcl_x1r = T(q.rcl_x1r); c.rcl_x2r = T(q.rcl_x2r); c.rcl_x1s = T(q.rcl_x1s);
+  c.rcl_lam1r = T(q.rcl_lam1r); c.rcl_lam2r = T(q.rcl_lam2r); c.rcl_lam1s = T(q.rcl_lam1s); c.rcl_lam2s = T(q.rcl_lam2s);
+  c.rcl_cdenom1 = T(q.rcl_cdenom1); c.rcl_cdenom2 = T(q.rcl_cdenom2); c.rcl_cdenom3 = T(q.rcl_cdenom3);
+  c.rcl_const1r = T(q.rcl_const1r); c.rcl_const2r = T(q.rcl_const2r); c.rcl_const3r = T(q.rcl_const3r);
+  c.rcl_const4r = T(q.rcl_const4r); c.rcl_const5r = T(q.rcl_const5r); c.rcl_const6r = T(q.rcl_const6r);
+  c.rcl_const7r = T(q.rcl_const7r); c.rcl_kkaac = T(q.rcl_kkaac); c.rcl_kkbac = T(q.rcl_kkbac);
+  c.rcl_kkbauq = T(q.rcl_kkbauq); c.rcl_kkbaun = T(q.rcl_kkbaun);
+  c.rcl_kk_cloud_num_sea = T(q.rcl_kk_cloud_num_sea); c.rcl_kk_cloud_num_land = T(q.rcl_kk_cloud_num_land);
+  c.rcl_fzrab = T(q.rcl_fzrab); c.rcl_apb1 = T(q.rcl_apb1); c.rcl_apb2 = T(q.rcl_apb2); c.rcl_apb3 = T(q.rcl_apb3);
+  c.rcl_const1s = T(q.rcl_const1s); c.rcl_const2s = T(q.rcl_const2s); c.rcl_const3s = T(q.rcl_const3s);
+  c.rcl_const4s = T(q.rcl_const4s); c.rcl_const5s = T(q.rcl_const5s); c.rcl_const6s = T(q.rcl_const6s);
+  c.rcl_const7s = T(q.rcl_const7s); c.rcl_const8s = T(q.rcl_const8s);
+  c.rmfadvw = T(q.rmfadvw); c.rmfadvwdd = T(q.rmfadvwdd);
+  c.ptsphy = ptsphy;
+  c.zqtmst = T(1) / ptsphy;                   // :798
+  c.zrdcp = c.rd / T(q.rcpd);                 // :800
+  c.zrg_r = T(1) / c.rg;                      // :803
+  c.zrldcp = T(1) / (c.ralsdcp - c.ralvdcp);  // :804
+  c.zepsilon = T(100) * std::numeric_limits<T>::epsilon();  // :693
+  c.rdtg_r = T(1) / (ptsphy * c.rg);          // :1285
+  c.c_aut = T(q.rcl_inhomogaut) * ptsphy * T(q.rcl_kkaau);
+  c.c_acc = T(q.rcl_inhomogacc) * ptsphy * T(q.rcl_kkaac);
+  c.ncldtop = q.ncldtop;
+  c.lmfdsnow = q.lmfdsnow;
+  return c;
+}
+
+template <typename R, typename RD>
+hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
+                          hipStream_t stream) {
+  constexpr int BLOCK = 256;
+  const KConst<R> c = make_kconst<R>(q, ptsphy);
+  const KConst<RD> cd = make_kconst<RD>(q, ptsphy);
+  const int64_t ncols = (int64_t)d.ngpblks * d.nproma;
+  const int64_t nblk = (ncols + BLOCK - 1) / BLOCK;
+  hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, BLOCK>), dim3((unsigned)nblk), dim3(BLOCK), 0, stream, c, cd, p, d,
+                     ssc_off);
+  return hipGetLastError();
+}
+
+template hipError_t launch_cloudsc<double, double>(const cloudsc_params_t&, double, const KPtrs<double>&,
+                                                   const KDims&, int, hipStream_t);
+template hipError_t launch_cloudsc<float, float>(const cloudsc_params_t&, double, const KPtrs<float>&, const KDims&,
+                                                 int, hipStream_t);
+template hipError_t launch_cloudsc<float, double>(const cloudsc_params_t&, double, const KPtrs<float>&, const KDims&,
+                                                  int, hipStream_t);
+
+}  // namespace cloudsc_gpu

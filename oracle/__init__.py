@@ -1,0 +1,109 @@
+"""Python binding of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+The oracle (cloudsc_oracle.hpp) restates the reference CLOUDSC default path
+on the CPU.  It is the checker of the HIP kernels: only tests/,
+__graft_entry__.smoke() and the cpu_baseline leg of bench.py import this
+module.  The product (openifs_amd, libcloudsc_hip.so) never does.
+
+Parity status of each oracle function (what pins it; see tests/test_oracle_*.py):
+  Thermo statement functions     pinned: closed forms and Clausius-Clapeyron derivative checks
+  Lanczos Gamma / PSD constants  pinned: math.gamma, hand-derived closed forms of SUCLDP
+  CLOUD_SUPERSATCHECK            pinned: invariants (q+l+i conserved, enthalpy, exact
+                                 saturation after adjustment to first order), no-op cases
+  lu5_solve                      pinned: numpy.linalg.solve on diagonally dominant systems
+  truncate_sinks                 pinned: brute-force positivity and ordering properties
+  cloudsc (whole column)         pinned only by column water/enthalpy conservation, the
+                                 reference's write set and special cases (dry, clear-sky,
+                                 warm); process-rate formulas of section 4 are
+                                 parity unpinned (no reference run or golden vector exists)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_cloudsc.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (make -C oracle)")
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def cloudsc_block(precision, params, kidia, kfdia, klon, klev, ptsphy, arrays):
+    """One CLOUDSC block on the CPU, same argument list as the product's cloudsc_*_c."""
+    from openifs_amd.cloudsc import call_block
+
+    f = getattr(lib(), f"oracle_cloudsc_{precision}")
+    rc = call_block(f, params, kidia, kfdia, klon, klev, ptsphy, arrays, precision)
+    if rc != 0:
+        raise RuntimeError(f"oracle_cloudsc_{precision} returned {rc}")
+    return arrays
+
+
+def cloudsc_blocked(precision, params, ngptot, nproma, klev, ptsphy, arrays, nthreads=0):
+    """All blocks of a blocked field set (dict of C-order numpy arrays, see openifs_amd.cloudsc.blocked_shape)."""
+    from openifs_amd.cloudsc import BLOCKED_ARRAYS
+
+    ngpblks = (ngptot + nproma - 1) // nproma
+    real = ctypes.c_double if precision == "dp" else ctypes.c_float
+    args = [ctypes.byref(params), ctypes.c_int(ngptot), ctypes.c_int(nproma), ctypes.c_int(klev),
+            ctypes.c_int(ngpblks), real(ptsphy)]
+    for name, kind, role in BLOCKED_ARRAYS:
+        a = arrays[name]
+        want = np.int32 if kind == "1di" else (np.float64 if precision == "dp" else np.float32)
+        if a.dtype != want or not a.flags.c_contiguous:
+            raise TypeError(f"{name}: need a C-contiguous {np.dtype(want).name} array")
+        args.append(ctypes.c_void_p(a.ctypes.data))
+    args.append(ctypes.c_int(nthreads))
+    rc = getattr(lib(), f"oracle_cloudsc_blocked_{precision}")(*args)
+    if rc != 0:
+        raise RuntimeError(f"oracle_cloudsc_blocked_{precision} returned {rc}")
+    return arrays
+
+
+def supersatcheck(params, t, q, a, pp, psurf, precision="dp"):
+    dt = np.float64 if precision == "dp" else np.float32
+    t, q, a, pp, psurf = (np.ascontiguousarray(x, dtype=dt) for x in (t, q, a, pp, psurf))
+    n = t.size
+    outs = [np.zeros(n, dtype=dt) for _ in range(5)]
+    f = getattr(lib(), f"oracle_supersatcheck_{precision}")
+    f(ctypes.byref(params), ctypes.c_int(n), *(ctypes.c_void_p(x.ctypes.data) for x in (t, q, a, pp, psurf)),
+      *(ctypes.c_void_p(x.ctypes.data) for x in outs))
+    return outs  # t_adj, q_adj, a_adj, l_adj, i_adj
+
+
+def lu5(lhs, rhs):
+    lhs = np.ascontiguousarray(lhs, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.zeros(5)
+    lib().oracle_lu5_dp(ctypes.c_void_p(lhs.ctypes.data), ctypes.c_void_p(rhs.ctypes.data),
+                        ctypes.c_void_p(x.ctypes.data))
+    return x
+
+
+def truncate_sinks(qa, qx):
+    qa = np.ascontiguousarray(qa, dtype=np.float64).copy()
+    qx = np.ascontiguousarray(qx, dtype=np.float64)
+    lib().oracle_truncate_sinks_dp(ctypes.c_void_p(qa.ctypes.data), ctypes.c_void_p(qx.ctypes.data))
+    return qa
+
+
+THERMO = {"foedelta": 0, "foeew": 1, "foealfa": 2, "foeewm": 3, "foedem": 4, "foeldcpm": 5, "foeeliq": 6,
+          "foeeice": 7, "foeeliq2ice": 8, "fokoop": 9}
+
+
+def thermo(params, name, t):
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = np.zeros_like(t)
+    rc = lib().oracle_thermo_dp(ctypes.byref(params), ctypes.c_int(THERMO[name]), ctypes.c_int(t.size),
+                                ctypes.c_void_p(t.ctypes.data), ctypes.c_void_p(out.ctypes.data))
+    if rc != 0:
+        raise RuntimeError(name)
+    return out

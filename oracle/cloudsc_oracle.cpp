@@ -1,0 +1,233 @@
+// C entry points of the CPU oracle -- TEST INFRASTRUCTURE ONLY (see
+// cloudsc_oracle.hpp).  Built into oracle/liboracle_cloudsc.so by
+// oracle/Makefile; loaded by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py, never by the product library.
+//
+//   oracle_cloudsc_{dp,sp,mixed}        one NPROMA block, CLOUDSC's argument list
+//   oracle_cloudsc_blocked_{dp,sp,mixed} all blocks, OpenMP over blocks with
+//       SCHEDULE(DYNAMIC,1) as the reference driver (ec_phys_drv.F90:430-439)
+//   oracle_supersatcheck_*, oracle_lu5_dp, oracle_truncate_sinks_dp,
+//   oracle_thermo_dp                     unit-level entry points for the tests
+#include <omp.h>
+
+#include "cloudsc_oracle.hpp"
+
+namespace {
+
+int check_config(const cloudsc_params_t* p) {
+  if (!p) return CLOUDSC_ENULL;
+  if (p->lcloud_inhomog || p->lrad_cloud_inhomog || p->laeraware || p->lspp || p->lcldbud)
+    return CLOUDSC_EUNSUPPORTED;
+  return CLOUDSC_OK;
+}
+
+int check_dims(const cloudsc_params_t* p, int kidia, int kfdia, int klon, int klev) {
+  if (klon < 1 || klev < 2 || kidia < 1 || kfdia < kidia || kfdia > klon) return CLOUDSC_EDIMS;
+  // CLOUD_SUPERSATCHECK reads the pressure section from KIDIA (cloudsc.F90:3776): stay inside KLON.
+  if (kidia + kfdia - 1 > klon) return CLOUDSC_EDIMS;
+  if (p->ncldtop < 2 || p->ncldtop > klev) return CLOUDSC_EDIMS;
+  return CLOUDSC_OK;
+}
+
+template <typename R, typename RD>
+int run_block(const cloudsc_params_t* p, int kidia, int kfdia, int klon, int klev, R ptsphy, const R* pt,
+              const R* pq, const R* tcml_t, const R* tcml_q, const R* tcml_a, const R* tcml_cld, R* tloc_t,
+              R* tloc_q, R* tloc_a, R* tloc_cld, const R* pvfl, const R* pvfi, const R* pap, const R* paph,
+              const R* plsm, const int32_t* ldcum, const int32_t* ktype, const R* peis, const R* plu, R* plude,
+              const R* pludeli, const R* psnde, const R* pmfu, const R* pmfd, const R* pa, const R* pclv,
+              R* pcovptot, R* prainfrac_toprfz, R* pfsqlf, R* pfsqif, R* pfcqnng, R* pfcqlng, R* pfsqrf,
+              R* pfsqsf, R* pfcqrng, R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn, R* pfhpsl,
+              R* pfhpsn) {
+  int rc = check_config(p);
+  if (rc) return rc;
+  rc = check_dims(p, kidia, kfdia, klon, klev);
+  if (rc) return rc;
+  oracle::Args<R> a{kidia,  kfdia,  klon,    klev,    ptsphy,  pt,       pq,      tcml_t,  tcml_q,  tcml_a,
+                    tcml_cld, tloc_t, tloc_q, tloc_a, tloc_cld, pvfl, pvfi, pap, paph, plsm,
+                    ldcum,  ktype,  peis,    plu,     plude,   pludeli,  psnde,   pmfu,    pmfd,    pa,
+                    pclv,   pcovptot, prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf,
+                    pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn};
+  oracle::cloudsc<R, RD>(*p, a);
+  return CLOUDSC_OK;
+}
+
+#define ORACLE_BLOCK_CALL(R, RD)                                                                              \
+  (void)lddiag, (void)pvfa, (void)pdyna, (void)pdynl, (void)pdyni, (void)pdynr, (void)pdyns, (void)phrsw,     \
+      (void)phrlw, (void)pvervel, (void)pgeom, (void)pgaw, (void)kctop, (void)kpbltype, (void)pgp2dspp,         \
+      (void)ldslphy, (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice, (void)pfsd,    \
+      (void)pextra, (void)kfldx;                                                                                \
+  return run_block<R, RD>(ydparams, kidia, kfdia, klon, klev, ptsphy, pt, pq, ptendency_cml_t, ptendency_cml_q, \
+                          ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a, \
+                          ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis, plu, plude,        \
+                          pludeli, psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz, pfsqlf, pfsqif,      \
+                          pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl,        \
+                          pfplsn, pfhpsl, pfhpsn)
+
+template <typename R, typename RD>
+int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int ngpblks, R ptsphy, const R* pt,
+                const R* pq, const R* tcml_t, const R* tcml_q, const R* tcml_a, const R* tcml_cld, R* tloc_t,
+                R* tloc_q, R* tloc_a, R* tloc_cld, const R* pvfl, const R* pvfi, const R* pap, const R* paph,
+                const R* plsm, const int32_t* ldcum, const int32_t* ktype, const R* peis, const R* plu, R* plude,
+                const R* pludeli, const R* psnde, const R* pmfu, const R* pmfd, const R* pa, const R* pclv,
+                R* pcovptot, R* prainfrac_toprfz, R* pfsqlf, R* pfsqif, R* pfcqnng, R* pfcqlng, R* pfsqrf,
+                R* pfsqsf, R* pfcqrng, R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn, R* pfhpsl,
+                R* pfhpsn, int nthreads) {
+  int rc = check_config(p);
+  if (rc) return rc;
+  if (nproma < 1 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma) return CLOUDSC_EDIMS;
+  rc = check_dims(p, 1, nproma, nproma, klev);
+  if (rc) return rc;
+  const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1), s1 = nproma;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int ibl = 0; ibl < ngpblks; ++ibl) {
+    const int kfdia = std::min(nproma, ngptot - ibl * nproma);
+    const size_t o2 = ibl * s2, oh = ibl * sh, o1 = ibl * s1, o5 = ibl * s2 * 5, o4 = ibl * s2 * 4,
+                 o2b = ibl * s2 * 2;
+    oracle::Args<R> a{1,
+                      kfdia,
+                      nproma,
+                      klev,
+                      ptsphy,
+                      pt + o2,
+                      pq + o2,
+                      tcml_t + o2,
+                      tcml_q + o2,
+                      tcml_a + o2,
+                      tcml_cld + o5,
+                      tloc_t + o2,
+                      tloc_q + o2,
+                      tloc_a + o2,
+                      tloc_cld + o5,
+                      pvfl + o2,
+                      pvfi + o2,
+                      pap + o2,
+                      paph + oh,
+                      plsm + o1,
+                      ldcum + o1,
+                      ktype + o1,
+                      peis + o1,
+                      plu + o2,
+                      plude + o2,
+                      pludeli + o4,
+                      psnde + o2b,
+                      pmfu + o2,
+                      pmfd + o2,
+                      pa + o2,
+                      pclv + o5,
+                      pcovptot + o2,
+                      prainfrac_toprfz + o1,
+                      pfsqlf + oh,
+                      pfsqif + oh,
+                      pfcqnng + oh,
+                      pfcqlng + oh,
+                      pfsqrf + oh,
+                      pfsqsf + oh,
+                      pfcqrng + oh,
+                      pfcqsng + oh,
+                      pfsqltur + oh,
+                      pfsqitur + oh,
+                      pfplsl + oh,
+                      pfplsn + oh,
+                      pfhpsl + oh,
+                      pfhpsn + oh};
+    oracle::cloudsc<R, RD>(*p, a);
+  }
+  return CLOUDSC_OK;
+}
+
+#define ORACLE_BLOCKED_CALL(R, RD)                                                                              \
+  return run_blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy, pt, pq, ptendency_cml_t,          \
+                            ptendency_cml_q, ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, \
+                            ptendency_loc_a, ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis,  \
+                            plu, plude, pludeli, psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz, pfsqlf, \
+                            pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng, pfsqltur, pfsqitur,       \
+                            pfplsl, pfplsn, pfhpsl, pfhpsn, nthreads)
+
+#define ORACLE_BLOCKED_ARGS(R)                                                                          \
+  const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,            \
+      const R *pt, const R *pq, const R *ptendency_cml_t, const R *ptendency_cml_q,                     \
+      const R *ptendency_cml_a, const R *ptendency_cml_cld, R *ptendency_loc_t, R *ptendency_loc_q,     \
+      R *ptendency_loc_a, R *ptendency_loc_cld, const R *pvfl, const R *pvfi, const R *pap,             \
+      const R *paph, const R *plsm, const int32_t *ldcum, const int32_t *ktype, R *peis,                \
+      const R *plu, R *plude, const R *pludeli, const R *psnde, const R *pmfu, const R *pmfd,           \
+      const R *pa, const R *pclv, R *pcovptot, R *prainfrac_toprfz, R *pfsqlf, R *pfsqif,              \
+      R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur,   \
+      R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, int nthreads
+
+}  // namespace
+
+extern "C" {
+
+int oracle_cloudsc_dp(CLOUDSC_ARGS(double)) { ORACLE_BLOCK_CALL(double, double); }
+int oracle_cloudsc_sp(CLOUDSC_ARGS(float)) { ORACLE_BLOCK_CALL(float, float); }
+int oracle_cloudsc_mixed(CLOUDSC_ARGS(float)) { ORACLE_BLOCK_CALL(float, double); }
+
+int oracle_cloudsc_blocked_dp(ORACLE_BLOCKED_ARGS(double)) { ORACLE_BLOCKED_CALL(double, double); }
+int oracle_cloudsc_blocked_sp(ORACLE_BLOCKED_ARGS(float)) { ORACLE_BLOCKED_CALL(float, float); }
+int oracle_cloudsc_blocked_mixed(ORACLE_BLOCKED_ARGS(float)) { ORACLE_BLOCKED_CALL(float, double); }
+
+// CLOUD_SUPERSATCHECK on n points (KIDIA = 1, KFDIA = n), outputs in RD.
+int oracle_supersatcheck_dp(const cloudsc_params_t* p, int n, const double* t, const double* q, const double* a,
+                            const double* pp, const double* psurf, double* t_adj, double* q_adj, double* a_adj,
+                            double* l_adj, double* i_adj) {
+  oracle::supersatcheck<double, double>(*p, 1, n, t, q, a, pp, psurf, t_adj, q_adj, a_adj, l_adj, i_adj);
+  return 0;
+}
+int oracle_supersatcheck_sp(const cloudsc_params_t* p, int n, const float* t, const float* q, const float* a,
+                            const float* pp, const float* psurf, float* t_adj, float* q_adj, float* a_adj,
+                            float* l_adj, float* i_adj) {
+  oracle::supersatcheck<float, float>(*p, 1, n, t, q, a, pp, psurf, t_adj, q_adj, a_adj, l_adj, i_adj);
+  return 0;
+}
+
+// No-pivot LU solve of one 5x5 system (row-major lhs[a*5+b] = ZQLHS(a,b)).
+int oracle_lu5_dp(const double* lhs_in, const double* rhs, double* x_out) {
+  double lhs[5][5], x[5];
+  for (int i = 0; i < 5; ++i) {
+    x[i] = rhs[i];
+    for (int j = 0; j < 5; ++j) lhs[i][j] = lhs_in[i * 5 + j];
+  }
+  oracle::lu5_solve<double>(lhs, x);
+  for (int i = 0; i < 5; ++i) x_out[i] = x[i];
+  return 0;
+}
+
+// Sink truncation on one column (row-major qa[m*5+n] = ZSOLQA(m,n)), in place.
+int oracle_truncate_sinks_dp(double* qa_io, const double* qx) {
+  double qa[5][5];
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) qa[i][j] = qa_io[i * 5 + j];
+  oracle::truncate_sinks<double>(qa, qx, 1.e-14);
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) qa_io[i * 5 + j] = qa[i][j];
+  return 0;
+}
+
+// Thermodynamic statement functions on n temperatures:
+// fn 0 FOEDELTA, 1 FOEEW, 2 FOEALFA, 3 FOEEWM, 4 FOEDEM, 5 FOELDCPM,
+//    6 FOEELIQ, 7 FOEEICE, 8 FOEELIQ2ICE, 9 FOKOOP
+int oracle_thermo_dp(const cloudsc_params_t* p, int fn, int n, const double* t, double* out) {
+  const oracle::Thermo<double> th(*p);
+  for (int i = 0; i < n; ++i) {
+    const double x = t[i];
+    double y;
+    switch (fn) {
+      case 0: y = th.foedelta(x); break;
+      case 1: y = th.foeew(x); break;
+      case 2: y = th.foealfa(x); break;
+      case 3: y = th.foeewm(x); break;
+      case 4: y = th.foedem(x); break;
+      case 5: y = th.foeldcpm(x); break;
+      case 6: y = th.foeeliq(x); break;
+      case 7: y = th.foeeice(x); break;
+      case 8: y = th.foeeliq2ice(x); break;
+      case 9: y = th.fokoop(x); break;
+      default: return -1;
+    }
+    out[i] = y;
+  }
+  return 0;
+}
+
+}  // extern "C"
