@@ -1,0 +1,200 @@
+"""Benchmark of the plain FP32 CLOUDSC kernel on one MI355X.
+
+Workload: plain CLOUDSC in FP32, NLEV = 137, 655,360 columns (a TCo399-sized
+grid) in IFS blocked layout, seeded synthetic atmospheres generated on the
+device.  One step = one launch of cloudsc_blocked_sp_c over all columns,
+preceded by the device copy that restores PLUDE (CLOUDSC rescales it in place,
+so every step starts from the same inputs).  Inputs are resident in HBM
+before the timed region starts.
+
+Reported besides the throughput:
+  roofline      HBM-bound: algorithmic bytes per launch (4*(49*NLEV+20) per
+                column) / average kernel duration from HIP events on the
+                kernel's stream; traffic from the rocprofv3 FETCH_SIZE /
+                WRITE_SIZE summary committed under profiles/ (null if absent)
+  err_vs_fp64   per-field relative RMS of this FP32 result against the FP64
+                CPU oracle on the first 2048 columns
+  cpu_baseline  the CPU oracle (C++/OpenMP restatement, FP32) on a sample of
+                the same workload, OpenMP over NPROMA=32 blocks, SCHEDULE(DYNAMIC,1)
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+NGPTOT, KLEV = 655360, 137
+METRIC = "columns/sec at NLEV=137 (plain FP32 vs surrogate-hybrid) + % HBM peak, err vs FP64"
+
+
+def bytes_per_column(klev, word=4):
+    # 26*KLEV+5 words read, 23*KLEV+15 written on the default path
+    return word * (49 * klev + 20)
+
+
+def read_traffic(nproma):
+    """HBM bytes per dispatch measured by rocprofv3 (profiles/cloudsc_sp_l137_655360_traffic.md)."""
+    path = os.path.join(ROOT, "profiles", "cloudsc_sp_l137_655360_traffic.md")
+    if not os.path.exists(path):
+        return None
+    txt = open(path).read()
+    m = re.search(r"nproma\s*=\s*(\d+).*?hbm_bytes_per_dispatch\s*=\s*([0-9.eE+]+)", txt, re.S)
+    if not m or int(m.group(1)) != nproma:
+        return None
+    return float(m.group(2))
+
+
+def cpu_baseline(params, sample_cols, nthreads):
+    import numpy as np
+    import torch
+
+    import oracle
+    from openifs_amd import cloudsc as C
+
+    nproma = 32
+    f = C.BlockedFields(sample_cols, nproma, KLEV, "sp", "cuda", names=set(C.INPUT_NAMES))
+    C.synth_device(f, params, 1234)
+    arr = {n: f[n].cpu().numpy() for n in C.INPUT_NAMES}
+    del f
+    torch.cuda.empty_cache()
+    for name, kind, role in C.BLOCKED_ARRAYS:
+        if role == "o":
+            arr[name] = np.empty(C.blocked_shape(kind, nproma, KLEV, (sample_cols + nproma - 1) // nproma),
+                                 dtype=np.float32)
+    plude0 = arr["plude"].copy()
+    small = 4096
+    warm = {k: (v[: small // nproma].copy()) for k, v in arr.items()}
+    oracle.cloudsc_blocked("sp", params, small, nproma, KLEV, 900.0, warm, nthreads=nthreads)
+    t0 = time.perf_counter()
+    oracle.cloudsc_blocked("sp", params, sample_cols, nproma, KLEV, 900.0, arr, nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    arr["plude"][...] = plude0
+    return sample_cols / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nproma", type=int, default=128)
+    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (CLOUDSC is column-local; --gpus {args.gpus} "
+              "is not supported). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from openifs_amd import cloudsc as C
+
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+    params = C.default_params(900.0, KLEV)
+    f = C.BlockedFields(NGPTOT, args.nproma, KLEV, "sp", dev)
+    C.synth_device(f, params, 20261202)
+    plude0 = f["plude"].clone()
+
+    # accuracy on a sample, against the FP64 CPU oracle, from the inputs the benchmark uses
+    import oracle
+
+    nproma = args.nproma
+    nb = 2048 // nproma
+    inp = {n: f[n][:nb].cpu().numpy() for n in C.INPUT_NAMES}
+    C.cloudsc_blocked(f, params, 900.0)
+    torch.cuda.synchronize()
+    out = {n: f[n][:nb].cpu().numpy() for n in C.OUTPUT_NAMES}
+    ref = {k: (v.astype(np.float64) if v.dtype.kind == "f" else v.copy()) for k, v in inp.items()}
+    for name, kind, role in C.BLOCKED_ARRAYS:
+        if role == "o":
+            ref[name] = np.full(C.blocked_shape(kind, nproma, KLEV, nb), np.nan)
+    oracle.cloudsc_blocked("dp", params, nb * nproma, nproma, KLEV, 900.0, ref, nthreads=args.cpu_threads)
+
+    def rr(a, r):
+        a = a.astype(np.float64)
+        m = ~np.isnan(r)
+        return float(np.sqrt(np.sum((a[m] - r[m]) ** 2) / np.sum(r[m] ** 2)))
+
+    err = {n: rr(out[n], ref[n]) for n in ("ptendency_loc_t", "ptendency_loc_q", "ptendency_loc_a", "pfplsl",
+                                           "pfplsn")}
+    for s, nm in enumerate(("l", "i", "r", "s")):
+        err[f"ptendency_loc_cld_{nm}"] = rr(out["ptendency_loc_cld"][:, s], ref["ptendency_loc_cld"][:, s])
+    sp = out["pfplsl"][:, -1].astype(np.float64) + out["pfplsn"][:, -1]
+    sr = ref["pfplsl"][:, -1] + ref["pfplsn"][:, -1]
+    err["surface_precip"] = float(np.sqrt(np.sum((sp - sr) ** 2) / np.sum(sr ** 2)))
+
+    stream = torch.cuda.current_stream()
+
+    def step():
+        f["plude"].copy_(plude0, non_blocking=True)
+        C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        f["plude"].copy_(plude0, non_blocking=True)
+        ev[i][0].record(stream)
+        C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    ms_per_step = wall * 1e3 / args.steps
+    value = NGPTOT / (ms_per_step / 1e3)
+
+    bpc = bytes_per_column(KLEV)
+    alg_bytes = bpc * NGPTOT
+    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
+    traffic = read_traffic(args.nproma)
+
+    cpu = None
+    if not args.no_cpu:
+        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        del f, plude0
+        torch.cuda.empty_cache()
+        cps, cdt = cpu_baseline(params, args.cpu_sample, nthreads)
+        cpu = {"value": cps, "unit": "columns/s", "cores": nthreads, "kind": "port",
+               "sample": f"{args.cpu_sample} of the {NGPTOT} columns, FP32 oracle (C++/OpenMP restatement of "
+                         f"CLOUDSC), NPROMA=32 blocks, SCHEDULE(DYNAMIC,1), {cdt:.1f} s"}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded atmospheres, BASELINE configuration distributions)",
+        "config": {"workload": f"plain CLOUDSC FP32, NLEV={KLEV}, {NGPTOT} columns (TCo399-sized), "
+                               f"NPROMA={args.nproma}, dt=900 s", "columns": NGPTOT, "nlev": KLEV,
+                   "nproma": args.nproma, "parallelism": "single GPU"},
+        "kernel_ms": kern_ms,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_column": bpc},
+        "err_vs_fp64": err,
+        "hybrid": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

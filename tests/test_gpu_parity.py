@@ -1,0 +1,179 @@
+"""GPU parity: the gfx950 kernels against the CPU oracle on identical inputs.
+
+FP64: bit-level agreement is expected (differences only from FMA contraction
+and the device's libm): per-field relative RMS <= 1e-12 and an identical
+write set.  FP32 and mixed: CLOUDSC's near-zero thresholds (ZEPSEC = 1e-14
+against FP32 cancellation residues of ~1e-12, the supersaturation switch)
+make FP32 results depend chaotically on rounding, so the FP32 oracle itself
+is ~1e-2 (relative RMS) away from FP64.  The criterion is that the GPU is as
+accurate as the FP32 oracle: its relative RMS against the FP64 oracle must
+not exceed twice the FP32 oracle's, field by field.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from openifs_amd import cloudsc as C
+from tests.helpers import column_budgets, columns, oracle_blocked, relrms
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ["ptendency_loc_t", "ptendency_loc_q", "ptendency_loc_a", "ptendency_loc_cld", "pcovptot", "plude",
+          "prainfrac_toprfz", "pfsqlf", "pfsqif", "pfcqnng", "pfcqlng", "pfsqrf", "pfsqsf", "pfcqrng", "pfcqsng",
+          "pfsqltur", "pfsqitur", "pfplsl", "pfplsn", "pfhpsl", "pfhpsn"]
+
+
+def gpu_run(params, prec, ngptot, nproma, klev, seed, ptsphy=900.0):
+    f = C.BlockedFields(ngptot, nproma, klev, prec, "cuda", fill=float("nan"))
+    C.synth_device(f, params, seed)
+    inp = {n: f[n].cpu().numpy().copy() for n in C.INPUT_NAMES}
+    C.cloudsc_blocked(f, params, ptsphy)
+    torch.cuda.synchronize()
+    out = {n: f[n].cpu().numpy() for n in C.OUTPUT_NAMES}
+    del f
+    return inp, out
+
+
+def assert_same_write_set(a, b, name):
+    np.testing.assert_array_equal(np.isnan(a), np.isnan(b), err_msg=f"{name}: write sets differ")
+
+
+@pytest.mark.parametrize("ngptot,nproma,klev,seed", [(2048, 32, 137, 11), (1000, 64, 137, 12), (777, 128, 91, 13)])
+def test_fp64_matches_oracle(ngptot, nproma, klev, seed, params137, params91):
+    p = params137 if klev == 137 else params91
+    inp, out = gpu_run(p, "dp", ngptot, nproma, klev, seed)
+    ref = oracle_blocked("dp", p, inp, ngptot, nproma, klev, nthreads=16)
+    for n in FIELDS:
+        g = columns(out[n], nproma)[:ngptot]
+        r = columns(ref[n], nproma)[:ngptot]
+        assert_same_write_set(g, r, n)
+        assert relrms(g, r) <= 1e-12, n
+    # padding columns of the last block are untouched except PRAINFRAC_TOPRFZ (zeroed for all KLON)
+    if ngptot % nproma:
+        pad = columns(out["ptendency_loc_t"], nproma)[ngptot:]
+        assert np.all(np.isnan(pad))
+        assert np.all(columns(out["prainfrac_toprfz"], nproma)[ngptot:] == 0.0)
+
+
+def test_fp64_conservation_and_plude(params137):
+    inp, out = gpu_run(params137, "dp", 4096, 64, 137, 21)
+    (wl, wr, ws), (hl, hr, hs) = column_budgets(params137, out, inp, 64, 4096)
+    assert np.max(np.abs(wl - wr) / ws) < 1e-10
+    assert np.max(np.abs(hl - hr) / hs) < 1e-12
+
+
+def _block_arrays(inp, out_fill, b, nproma, klev, prec):
+    arrs = C.allocate_block(nproma, klev, prec)
+    for n, kind, role in C.BLOCKED_ARRAYS:
+        if role != "o":
+            arrs[n][...] = inp[n][b].T
+    return arrs
+
+
+@pytest.mark.parametrize("prec", ["dp", "sp", "mixed"])
+def test_per_block_drop_in_equals_blocked(prec, params137):
+    """cloudsc_*_c (host arrays, one block) reproduces the blocked launch bit for bit."""
+    nproma, klev = 32, 137
+    inp, out = gpu_run(params137, prec, 4 * nproma, nproma, klev, 31)
+    for b in (0, 3):
+        arrs = _block_arrays(inp, np.nan, b, nproma, klev, "dp" if prec == "dp" else "sp")
+        C.CLOUDSC(params137, 1, nproma, nproma, klev, 900.0, arrs, precision=prec)
+        for n, kind, role in C.BLOCKED_ARRAYS:
+            if role != "i":
+                np.testing.assert_array_equal(arrs[n], out[n][b].T, err_msg=f"{prec} block {b} {n}")
+
+
+def test_per_block_kidia_window_fp64(params137):
+    """KIDIA > 1: columns outside KIDIA:KFDIA stay untouched (PRAINFRAC_TOPRFZ is zeroed for all),
+    and the supersaturation check reads the pressure section from KIDIA as the reference does."""
+    import oracle
+
+    nproma, klev = 32, 137
+    inp, _ = gpu_run(params137, "dp", nproma, nproma, klev, 41)
+    kidia, kfdia = 3, 27
+    a_gpu = _block_arrays(inp, np.nan, 0, nproma, klev, "dp")
+    a_orc = _block_arrays(inp, np.nan, 0, nproma, klev, "dp")
+    C.CLOUDSC(params137, kidia, kfdia, nproma, klev, 900.0, a_gpu, precision="dp")
+    oracle.cloudsc_block("dp", params137, kidia, kfdia, nproma, klev, 900.0, a_orc)
+    for n, kind, role in C.BLOCKED_ARRAYS:
+        if role == "i":
+            continue
+        g, r = a_gpu[n], a_orc[n]
+        assert_same_write_set(g, r, n)
+        assert relrms(g, r) <= 1e-12, n
+    outside = np.r_[0:kidia - 1, kfdia:nproma]
+    assert np.all(np.isnan(a_gpu["ptendency_loc_t"][outside]))
+    np.testing.assert_array_equal(a_gpu["plude"][outside], inp["plude"][0].T[outside])
+    assert np.all(a_gpu["prainfrac_toprfz"] == a_gpu["prainfrac_toprfz"])  # no NaN anywhere
+    assert np.all(a_gpu["prainfrac_toprfz"][outside] == 0.0)
+
+
+@pytest.mark.parametrize("prec", ["sp", "mixed"])
+def test_fp32_as_accurate_as_fp32_oracle(prec, params137):
+    ngptot, nproma, klev = 8192, 64, 137
+    inp, out = gpu_run(params137, prec, ngptot, nproma, klev, 51)
+    ref64 = oracle_blocked("dp", params137, inp, ngptot, nproma, klev, nthreads=16)
+    ref32 = oracle_blocked(prec, params137, inp, ngptot, nproma, klev, nthreads=16)
+    report = []
+    for n in FIELDS:
+        g, r64, r32 = (columns(x[n], nproma) for x in (out, ref64, ref32))
+        assert_same_write_set(g, r64, n)
+        e_gpu, e_cpu = relrms(g, r64), relrms(r32, r64)
+        report.append((n, e_gpu, e_cpu))
+        assert e_gpu <= 2.0 * e_cpu + 1e-6, f"{n}: GPU {e_gpu:.3e} vs FP32 oracle {e_cpu:.3e}"
+    print("\n".join(f"{n:20s} gpu {a:.3e}  fp32-oracle {b:.3e}" for n, a, b in report))
+
+
+def _budgets_torch(params, f, k0):
+    """Column water/enthalpy budgets of a device-resident FP32 result, in FP64 on the GPU."""
+    p = params
+    d = lambda n: f[n].double()
+    dp = (d("paph")[:, 1:, :] - d("paph")[:, :-1, :])[:, k0:, :]
+    tq, tt = d("ptendency_loc_q")[:, k0:], d("ptendency_loc_t")[:, k0:]
+    tc = d("ptendency_loc_cld")[:, :4, k0:]
+    lev = torch.zeros(f.klev, dtype=torch.bool, device=dp.device)
+    lev[k0:f.klev - 1] = True
+    detr = (d("plude") > 0) & lev[None, :, None]
+    lde, snde = d("pludeli"), d("psnde")
+    src_li = torch.where(detr, lde[:, 0] + lde[:, 1], 0.0).sum(1)
+    src_l, src_i = torch.where(detr, lde[:, 0], 0.0).sum(1), torch.where(detr, lde[:, 1], 0.0).sum(1)
+    src_r = torch.where(lev[None, :, None], snde[:, 0], 0.0).sum(1)
+    src_s = torch.where(lev[None, :, None], snde[:, 1], 0.0).sum(1)
+    pl, pn = d("pfplsl")[:, -1], d("pfplsn")[:, -1]
+    wl = ((tq + tc.sum(1)) * dp).sum(1) / p.rg
+    wr = src_li + src_r + src_s - pl - pn
+    ws = ((tq * dp).abs().sum(1) + (tc * dp[:, None]).abs().sum((1, 2))) / p.rg + wr.abs() + pl + pn
+    hl = ((p.rcpd * tt - p.rlvtt * (tc[:, 0] + tc[:, 2]) - p.rlstt * (tc[:, 1] + tc[:, 3])) * dp).sum(1) / p.rg
+    hr = -p.rlvtt * (src_l + src_r) - p.rlstt * (src_i + src_s) + p.rlvtt * pl + p.rlstt * pn
+    hs = ((p.rcpd * tt * dp).abs().sum(1) + p.rlstt * (tc * dp[:, None]).abs().sum((1, 2))) / p.rg + hr.abs()
+    return (wl - wr).abs() / ws, (hl - hr).abs() / hs
+
+
+@pytest.mark.parametrize("prec", ["sp", "mixed"])
+def test_full_size_fp32_properties(prec, params137):
+    """655,360 columns (TCo399-sized), the benchmark workload: every written output is
+    finite, the write set is the reference's, and the column budgets close to FP32 rounding."""
+    ngptot, nproma, klev = 655360, 128, 137
+    f = C.BlockedFields(ngptot, nproma, klev, prec, "cuda", fill=float("nan"))
+    C.synth_device(f, params137, 61)
+    plude_in = f["plude"].clone()
+    C.cloudsc_blocked(f, params137, 900.0)
+    torch.cuda.synchronize()
+    k0 = params137.ncldtop - 1
+    for n in C.OUTPUT_NAMES:
+        x = f[n]
+        if n == "ptendency_loc_cld":
+            assert torch.isfinite(x[:, :4]).all() and torch.isnan(x[:, 4]).all()
+        elif n == "pcovptot":
+            assert torch.isfinite(x[:, k0:]).all() and torch.isnan(x[:, :k0]).all()
+        else:
+            assert torch.isfinite(x).all(), n
+    assert torch.equal(f["plude"][:, :k0], plude_in[:, :k0]) and torch.equal(f["plude"][:, -1], plude_in[:, -1])
+    assert torch.equal(f["pfhpsl"], -params137.rlvtt * f["pfplsl"]) or \
+        torch.allclose(f["pfhpsl"], -params137.rlvtt * f["pfplsl"], rtol=1e-6, atol=0)
+    ew, eh = _budgets_torch(params137, f, k0)
+    assert float(ew.median()) < 1e-5 and float(eh.median()) < 1e-5
+    assert float(ew.max()) < 1e-2 and float(eh.max()) < 1e-2
