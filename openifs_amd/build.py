@@ -30,6 +30,8 @@ HIP_FLAGS = [
     # thresholds FP32 results fall on in almost every column, while unfused arithmetic keeps
     # the FP32 kernel as close to FP64 as the FP32 CPU oracle is.
     "-ffp-contract=off",
+    # FP32 exp/pow on v_exp_f32/v_log_f32 (FP64 keeps the accurate library functions)
+    "-DCLOUDSC_FAST_TRANSCENDENTALS",
     # packed FP32 (v_pk_*_f32) is disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
@@ -68,9 +70,31 @@ def build_oracle():
     return os.path.join(ROOT, "oracle", "liboracle_cloudsc.so")
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN_MOD = os.path.join(HERE, "fortran", "cloudsc_hip_mod.F90")
+FORTRAN_DRIVER_SRC = os.path.join(ROOT, "tests", "fortran", "cloudsc_fortran_driver.F90")
+FORTRAN_DRIVER = os.path.join(ROOT, "tests", "fortran", "cloudsc_fortran_driver")
+
+
+def build_fortran():
+    """The ISO_C_BINDING module and the Fortran host driver used by the tests (needs flang)."""
+    if not os.path.exists(FLANG):
+        return None
+    fdir = os.path.join(OBJDIR, "fortran")
+    os.makedirs(fdir, exist_ok=True)
+    mod_o = os.path.join(fdir, "cloudsc_hip_mod.o")
+    drv_o = os.path.join(fdir, "cloudsc_fortran_driver.o")
+    _run([FLANG, "-O1", "-c", FORTRAN_MOD, "-o", mod_o, "-module-dir", fdir])
+    _run([FLANG, "-O1", "-c", FORTRAN_DRIVER_SRC, "-o", drv_o, "-module-dir", fdir, "-I", fdir])
+    _run([FLANG, "-o", FORTRAN_DRIVER, drv_o, mod_o, "-L", HERE, "-lcloudsc_hip",
+          "-Wl,-rpath,$ORIGIN/../../openifs_amd"])
+    return FORTRAN_DRIVER
+
+
 def build_all(verbose=False):
     lib = build_hip(verbose)
     orc = build_oracle()
+    build_fortran()
     return lib, orc
 
 

@@ -25,6 +25,7 @@
 #include <limits>
 
 #include "../../include/cloudsc_hip.h"
+#include "cloudsc_defaults.h"
 
 namespace cloudsc_gpu {
 
@@ -47,6 +48,46 @@ struct KConst {
   T ptsphy, zqtmst, zrdcp, zrg_r, zrldcp, zepsilon, rdtg_r;
   T c_aut, c_acc;  // RCL_INHOMOGAUT*PTSPHY*RCL_KKAAU, RCL_INHOMOGACC*PTSPHY*RCL_KKAAC
   int32_t ncldtop, lmfdsnow;
+};
+
+// The default configuration (cloudsc_defaults.h) as compile-time constants: the kernel body
+// reads the same member names as in KConst, but every configuration constant becomes an
+// instruction literal instead of a register.  Only the values that depend on the time step or
+// on the vertical grid are run-time members.
+template <typename T>
+struct KConstDefault {
+#define CLOUDSC_DEF(name) static constexpr T name = T(cloudsc_defaults::name);
+  CLOUDSC_DEF(rg) CLOUDSC_DEF(rd) CLOUDSC_DEF(retv) CLOUDSC_DEF(rtt) CLOUDSC_DEF(rlvtt) CLOUDSC_DEF(rlstt)
+  CLOUDSC_DEF(rpi) CLOUDSC_DEF(r2es) CLOUDSC_DEF(r3les) CLOUDSC_DEF(r3ies) CLOUDSC_DEF(r4les) CLOUDSC_DEF(r4ies)
+  CLOUDSC_DEF(r5les) CLOUDSC_DEF(r5ies) CLOUDSC_DEF(r5alvcp) CLOUDSC_DEF(r5alscp) CLOUDSC_DEF(ralvdcp)
+  CLOUDSC_DEF(ralsdcp) CLOUDSC_DEF(rtwat) CLOUDSC_DEF(rtice) CLOUDSC_DEF(rtwat_rtice_r) CLOUDSC_DEF(rkoop1)
+  CLOUDSC_DEF(rkoop2) CLOUDSC_DEF(rv) CLOUDSC_DEF(ramid) CLOUDSC_DEF(ramin) CLOUDSC_DEF(rlmin) CLOUDSC_DEF(rcldiff)
+  CLOUDSC_DEF(rcldiff_convi) CLOUDSC_DEF(rkconv) CLOUDSC_DEF(rsnowlin1) CLOUDSC_DEF(rsnowlin2)
+  CLOUDSC_DEF(rlcritsnow) CLOUDSC_DEF(rvice) CLOUDSC_DEF(rvrain) CLOUDSC_DEF(rvsnow) CLOUDSC_DEF(rxfallspeedrain)
+  CLOUDSC_DEF(rcldtopcf) CLOUDSC_DEF(rdepliqrefrate) CLOUDSC_DEF(rdepliqrefdepth) CLOUDSC_DEF(rcl_overlapliqice)
+  CLOUDSC_DEF(rcl_effrime) CLOUDSC_DEF(riceinit) CLOUDSC_DEF(rthomo) CLOUDSC_DEF(rprecrhmax) CLOUDSC_DEF(rpecons)
+  CLOUDSC_DEF(rvrfactor) CLOUDSC_DEF(rcovpmin) CLOUDSC_DEF(rtaumel) CLOUDSC_DEF(rdenswat) CLOUDSC_DEF(rdensref)
+  CLOUDSC_DEF(rcl_dr) CLOUDSC_DEF(rcl_x1r) CLOUDSC_DEF(rcl_x2r) CLOUDSC_DEF(rcl_x1s) CLOUDSC_DEF(rcl_lam1r)
+  CLOUDSC_DEF(rcl_lam2r) CLOUDSC_DEF(rcl_lam1s) CLOUDSC_DEF(rcl_lam2s) CLOUDSC_DEF(rcl_cdenom1)
+  CLOUDSC_DEF(rcl_cdenom2) CLOUDSC_DEF(rcl_cdenom3) CLOUDSC_DEF(rcl_const1r) CLOUDSC_DEF(rcl_const2r)
+  CLOUDSC_DEF(rcl_const3r) CLOUDSC_DEF(rcl_const4r) CLOUDSC_DEF(rcl_const5r) CLOUDSC_DEF(rcl_const6r)
+  CLOUDSC_DEF(rcl_const7r) CLOUDSC_DEF(rcl_kkaac) CLOUDSC_DEF(rcl_kkbac) CLOUDSC_DEF(rcl_kkbauq)
+  CLOUDSC_DEF(rcl_kkbaun) CLOUDSC_DEF(rcl_kk_cloud_num_sea) CLOUDSC_DEF(rcl_kk_cloud_num_land)
+  CLOUDSC_DEF(rcl_fzrab) CLOUDSC_DEF(rcl_apb1) CLOUDSC_DEF(rcl_apb2) CLOUDSC_DEF(rcl_apb3) CLOUDSC_DEF(rcl_const1s)
+  CLOUDSC_DEF(rcl_const2s) CLOUDSC_DEF(rcl_const3s) CLOUDSC_DEF(rcl_const4s) CLOUDSC_DEF(rcl_const5s)
+  CLOUDSC_DEF(rcl_const6s) CLOUDSC_DEF(rcl_const7s) CLOUDSC_DEF(rcl_const8s) CLOUDSC_DEF(rmfadvw)
+  CLOUDSC_DEF(rmfadvwdd)
+#undef CLOUDSC_DEF
+  // derived as in KConst (cloudsc.F90:693, 798-804), in T arithmetic
+  static constexpr T rv_rd = rv / rd;
+  static constexpr T zrdcp = rd / T(cloudsc_defaults::rcpd);
+  static constexpr T zrg_r = T(1) / rg;
+  static constexpr T zrldcp = T(1) / (ralsdcp - ralvdcp);
+  static constexpr T zepsilon = T(100) * std::numeric_limits<T>::epsilon();
+  static constexpr int32_t lmfdsnow = cloudsc_defaults::lmfdsnow;
+  // run-time values
+  T ptsphy, zqtmst, rdtg_r, c_aut, c_acc, rssicefactor;
+  int32_t ncldtop;
 };
 
 // Device pointers of one launch, blocked layout.

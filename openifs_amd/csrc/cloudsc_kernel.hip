@@ -7,10 +7,26 @@
 
 namespace cloudsc_gpu {
 
+#ifdef CLOUDSC_FAST_TRANSCENDENTALS
+// FP32 exp and pow on the hardware base-2 instructions (v_exp_f32, v_log_f32):
+// exp(x) = 2^(x log2 e), pow(x,y) = 2^(y log2 x).  Relative error ~ (1 + |y log2 x|) ulp.
+__device__ __forceinline__ float m_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+__device__ __forceinline__ float m_pow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+#else
 __device__ __forceinline__ float m_exp(float x) { return expf(x); }
-__device__ __forceinline__ double m_exp(double x) { return exp(x); }
 __device__ __forceinline__ float m_pow(float x, float y) { return powf(x, y); }
+#endif
+__device__ __forceinline__ double m_exp(double x) { return exp(x); }
 __device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
+#ifdef CLOUDSC_FAST_TRANSCENDENTALS
+// Division inside microphysical rate expressions (continuous in their inputs): reciprocal
+// (v_rcp_f32, 1 ulp) times numerator.  State arithmetic that feeds thresholds (saturation
+// values, the supersaturation check, the solver, tendencies) keeps the IEEE quotient.
+__device__ __forceinline__ float m_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+#else
+__device__ __forceinline__ float m_div(float a, float b) { return a / b; }
+#endif
+__device__ __forceinline__ double m_div(double a, double b) { return a / b; }
 __device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
 template <typename T>
@@ -25,8 +41,8 @@ template <typename T>
 struct Sat {
   T alfa, el, ei;  // FOEALFA, EXP(R3LES*(T-RTT)/(T-R4LES)), EXP(R3IES*(T-RTT)/(T-R4IES))
 };
-template <typename T>
-__device__ __forceinline__ Sat<T> saturation(const KConst<T>& c, T t) {
+template <typename T, typename KCT>
+__device__ __forceinline__ Sat<T> saturation(const KCT& c, T t) {
   Sat<T> s;
   T x = (m_max(c.rtice, m_min(c.rtwat, t)) - c.rtice) * c.rtwat_rtice_r;
   s.alfa = m_min(T(1), x * x);
@@ -36,15 +52,15 @@ __device__ __forceinline__ Sat<T> saturation(const KConst<T>& c, T t) {
 }
 
 // CLOUD_SUPERSATCHECK for one point, KFTLIQICE = 1 (cloud_supersatcheck.F90:111-226).
-template <typename T>
-__device__ __forceinline__ void supersatcheck(const KConst<T>& c, T pt, T pq, T pa, T pp, T& t_adj, T& q_adj,
+template <typename T, typename KCT>
+__device__ __forceinline__ void supersatcheck(const KCT& c, T pt, T pq, T pa, T pp, T& t_adj, T& q_adj,
                                               T& a_adj, T& l_adj, T& i_adj) {
   const T zepsec = T(1.e-14);
   T zda = T(0);
   T zt = m_max(pt, T(160));
   T za = m_max(m_min(pa, T(1)), T(0));
   const T zp_r = T(1) / pp;
-  const Sat<T> s = saturation<T>(c, zt);
+  const Sat<T> s = saturation<T, KCT>(c, zt);
   const T foeeliq = c.r2es * s.el;
   const T zfoeeliq = m_min(foeeliq * zp_r, T(0.5));
   // FOEEW: water above RTT, ice below (FOEDELTA = MAX(0,SIGN(1,T-RTT)))
@@ -85,35 +101,51 @@ __device__ __forceinline__ T trunc_scale(T x, int jo, int a, int b, T zz) {
   return hit ? x * zz : x;
 }
 
-template <typename R, typename RD, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c, const KConst<RD> cd,
-                                                                const KPtrs<R> p, const KDims d, int ssc_off) {
-  const int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  const int64_t ncols = (int64_t)d.ngpblks * d.nproma;
-  if (g >= ncols) return;
-  const int64_t b = g / d.nproma;
-  const int jl = (int)(g - b * d.nproma);  // 0-based JL-1
-  const int nproma = d.nproma, klev = d.klev;
-  // PRAINFRAC_TOPRFZ is zeroed for all KLON columns of the block (:1010)
-  p.prainfrac_toprfz[b * nproma + jl] = R(0);
-  const int kfdia = (int)min((int64_t)d.kfdia, (int64_t)d.ngptot - b * nproma);
-  if (jl < d.kidia - 1 || jl >= kfdia) return;
+// Loads and stores at a 32-bit byte offset from an array base: the form the hardware
+// addresses directly (64-bit base in SGPRs, 32-bit offset in one VGPR).  The launcher
+// splits the grid so that every array of one launch spans less than 4 GiB.
+template <typename T>
+__device__ __forceinline__ T LD(const T* base, uint32_t off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+}
+template <typename T>
+__device__ __forceinline__ void ST(T* base, uint32_t off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+}
 
-  const int64_t s2 = (int64_t)nproma * klev;
-  const int64_t b2 = b * s2 + jl;                                  // (NPROMA,KLEV) level 0
-  const int64_t bh = b * (int64_t)nproma * (klev + 1) + jl;        // (NPROMA,KLEV+1) level 0
-  const int64_t b5 = b * s2 * 5 + jl, b4 = b * s2 * 4 + jl, b2x = b * s2 * 2 + jl;
-  const int64_t b1 = b * nproma + jl;
+// KC / KCD: constants of the working precision and of the FP64 islands, either the run-time
+// structure KConst or the compile-time default configuration KConstDefault (same member names).
+template <typename R, typename RD, typename KC, typename KCD, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p,
+                                                                const KDims d, int ssc_off) {
+  const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t nproma = (uint32_t)d.nproma, klev = (uint32_t)d.klev;
+  if (g >= (uint32_t)d.ngpblks * nproma) return;
+  const uint32_t b = g / nproma;
+  const uint32_t jl = g - b * nproma;  // 0-based JL-1
+  constexpr uint32_t E = sizeof(R);
+  const uint32_t o1 = (b * nproma + jl) * E, o1i = (b * nproma + jl) * 4u;
+  // PRAINFRAC_TOPRFZ is zeroed for all KLON columns of the block (:1010)
+  ST(p.prainfrac_toprfz, o1, R(0));
+  const int kfdia = min(d.kfdia, d.ngptot - (int)(b * nproma));
+  if ((int)jl < d.kidia - 1 || (int)jl >= kfdia) return;
+
+  const uint32_t s2 = nproma * klev;
+  const uint32_t lstep = nproma * E;                  // one level
+  const uint32_t sstep = s2 * E;                      // one species of a (NPROMA,KLEV,n) array
+  const uint32_t b2 = (b * s2 + jl) * E;              // (NPROMA,KLEV) level 0
+  const uint32_t bh = (b * (s2 + nproma) + jl) * E;   // (NPROMA,KLEV+1) level 0
+  const uint32_t b5 = (b * s2 * 5 + jl) * E, b4 = (b * s2 * 4 + jl) * E, b2x = (b * s2 * 2 + jl) * E;
 
   const R ptsphy = c.ptsphy, zqtmst = c.zqtmst, zepsec = R(1.e-14);
   const R rlmin = c.rlmin;
 
   // Per-column inputs
-  const R plsm = p.plsm[b1];
-  const int32_t ktype = p.ktype[b1];
-  const bool ldcum = p.ldcum[b1] != 0;
-  const R peis = p.peis[b1];
-  const R paph_surf = p.paph[bh + (int64_t)klev * nproma];
+  const R plsm = LD(p.plsm, o1);
+  const int32_t ktype = LD(p.ktype, o1i);
+  const bool ldcum = LD(p.ldcum, o1i) != 0;
+  const R peis = LD(p.peis, o1);
+  const R paph_surf = LD(p.paph, bh + klev * lstep);
   const R zadvw = (ktype == 1 && c.rmfadvw > R(0)) ? R(1) - c.rmfadvw : R(1);                  // :696-703
   const R zadvwd = (ktype == 1 && c.rmfadvw > R(0)) ? R(1) - c.rmfadvw * c.rmfadvwdd : R(1);
   const R zconst = plsm > R(0.5) ? c.rcl_kk_cloud_num_land : c.rcl_kk_cloud_num_sea;          // :2063-2069
@@ -129,49 +161,51 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
   R za_prev = R(0), qr_prev = R(0), qs_prev = R(0), t_prev = R(0), pap_prev = R(0);
   // DDH flux running sums at the upper half level (:4231-4242)
   R sqlf = R(0), sqif = R(0), cqlng = R(0), cqnng = R(0), sqltur = R(0), sqitur = R(0);
-  p.pfsqlf[bh] = R(0);
-  p.pfsqif[bh] = R(0);
-  p.pfsqrf[bh] = R(0);
-  p.pfsqsf[bh] = R(0);
-  p.pfcqlng[bh] = R(0);
-  p.pfcqnng[bh] = R(0);
-  p.pfcqrng[bh] = R(0);
-  p.pfcqsng[bh] = R(0);
-  p.pfsqltur[bh] = R(0);
-  p.pfsqitur[bh] = R(0);
+  ST(p.pfsqlf, bh, R(0));
+  ST(p.pfsqif, bh, R(0));
+  ST(p.pfsqrf, bh, R(0));
+  ST(p.pfsqsf, bh, R(0));
+  ST(p.pfcqlng, bh, R(0));
+  ST(p.pfcqnng, bh, R(0));
+  ST(p.pfcqrng, bh, R(0));
+  ST(p.pfcqsng, bh, R(0));
+  ST(p.pfsqltur, bh, R(0));
+  ST(p.pfsqitur, bh, R(0));
 
-  R paph_k = p.paph[bh];
-  R pmfu_k = p.pmfu[b2], pmfd_k = p.pmfd[b2];
+  R paph_k = LD(p.paph, bh);
+  R pmfu_k = LD(p.pmfu, b2), pmfd_k = LD(p.pmfd, b2);
 
-  for (int k = 0; k < klev; ++k) {
-    const int64_t i2 = b2 + (int64_t)k * nproma;
-    const int64_t ih1 = bh + (int64_t)(k + 1) * nproma;
-    const int64_t i5 = b5 + (int64_t)k * nproma;
+  for (uint32_t k = 0; k < klev; ++k) {
+    const uint32_t i2 = b2 + k * lstep;        // level k of a (NPROMA,KLEV) array
+    const uint32_t ih0 = bh + k * lstep;       // upper half level of layer k
+    const uint32_t ih1 = ih0 + lstep;          // lower half level
+    const uint32_t i5 = b5 + k * lstep;
+    const uint32_t i4 = b4 + k * lstep;
     const bool has_next = k + 1 < klev;
-    const R pmfu_n = has_next ? p.pmfu[i2 + nproma] : R(0);
-    const R pmfd_n = has_next ? p.pmfd[i2 + nproma] : R(0);
-    const R plu_n = has_next ? p.plu[i2 + nproma] : R(0);
-    const R paph_k1 = p.paph[ih1];
-    const R pap = p.pap[i2];
-    const R pt = p.pt[i2], pq = p.pq[i2], pa = p.pa[i2];
-    const R tcml_t = p.tcml_t[i2], tcml_q = p.tcml_q[i2], tcml_a = p.tcml_a[i2];
-    const R pvfl = p.pvfl[i2], pvfi = p.pvfi[i2];
-    const R pludeli1 = p.pludeli[b4 + (int64_t)k * nproma];
-    const R pludeli2 = p.pludeli[b4 + (int64_t)k * nproma + s2];
+    const R pmfu_n = has_next ? LD(p.pmfu, i2 + lstep) : R(0);
+    const R pmfd_n = has_next ? LD(p.pmfd, i2 + lstep) : R(0);
+    const R plu_n = has_next ? LD(p.plu, i2 + lstep) : R(0);
+    const R paph_k1 = LD(p.paph, ih1);
+    const R pap = LD(p.pap, i2);
+    const R pt = LD(p.pt, i2), pq = LD(p.pq, i2), pa = LD(p.pa, i2);
+    const R tcml_t = LD(p.tcml_t, i2), tcml_q = LD(p.tcml_q, i2), tcml_a = LD(p.tcml_a, i2);
+    const R pvfl = LD(p.pvfl, i2), pvfi = LD(p.pvfi, i2);
+    const R pludeli1 = LD(p.pludeli, i4);
+    const R pludeli2 = LD(p.pludeli, i4 + sstep);
 
     // ---- 1. initial values (:982-1002) ----
     const R ztp1 = pt + ptsphy * tcml_t;
     R qv = pq + ptsphy * tcml_q;
     R za = pa + ptsphy * tcml_a;
     const R zaorig = za;
-    const R ql0 = p.pclv[i5] + ptsphy * p.tcml_cld[i5];
-    const R qi0 = p.pclv[i5 + s2] + ptsphy * p.tcml_cld[i5 + s2];
-    const R qr0 = p.pclv[i5 + 2 * s2] + ptsphy * p.tcml_cld[i5 + 2 * s2];
-    const R qs0 = p.pclv[i5 + 3 * s2] + ptsphy * p.tcml_cld[i5 + 3 * s2];
+    const R ql0 = LD(p.pclv, i5) + ptsphy * LD(p.tcml_cld, i5);
+    const R qi0 = LD(p.pclv, i5 + sstep) + ptsphy * LD(p.tcml_cld, i5 + sstep);
+    const R qr0 = LD(p.pclv, i5 + 2 * sstep) + ptsphy * LD(p.tcml_cld, i5 + 2 * sstep);
+    const R qs0 = LD(p.pclv, i5 + 3 * sstep) + ptsphy * LD(p.tcml_cld, i5 + 3 * sstep);
     R ql = ql0, qi = qi0, qr = qr0, qs = qs0;
 
     // saturation values (:1087-1115)
-    const Sat<R> sat = saturation<R>(c, ztp1);
+    const Sat<R> sat = saturation<R, KC>(c, ztp1);
     const R foeeliq = c.r2es * sat.el, foeeice = c.r2es * sat.ei;
     const R zfoeewm = m_min(c.r2es * (sat.alfa * sat.el + (R(1) - sat.alfa) * sat.ei) / pap, R(0.5));
     const R zqsmix = zfoeewm / (R(1) - c.retv * zfoeewm);
@@ -227,15 +261,15 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
     // PFPLSL/PFPLSN/PFHPSL/PFHPSN at the upper half level (:4218-4224,4297-4302)
     {
       const R fl = pf_r + R(0), fn = pf_s + pf_i;
-      p.pfplsl[bh + (int64_t)k * nproma] = fl;
-      p.pfplsn[bh + (int64_t)k * nproma] = fn;
-      p.pfhpsl[bh + (int64_t)k * nproma] = -c.rlvtt * fl;
-      p.pfhpsn[bh + (int64_t)k * nproma] = -c.rlstt * fn;
+      ST(p.pfplsl, ih0, fl);
+      ST(p.pfplsn, ih0, fn);
+      ST(p.pfhpsl, ih0, -c.rlvtt * fl);
+      ST(p.pfhpsn, ih0, -c.rlstt * fn);
     }
 
-    if (k >= ncldtop0) {
-      const bool above_top = k > ncldtop0;  // JK > NCLDTOP
-      const R plude_in = p.plude[i2];
+    if ((int)k >= ncldtop0) {
+      const bool above_top = (int)k > ncldtop0;  // JK > NCLDTOP
+      const R plude_in = LD(p.plude, i2);
       // derived variables (:1281-1324)
       const R zdp = paph_k1 - paph_k;
       const R zgdp = c.rg / zdp;
@@ -292,12 +326,12 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
           plude = R(0);
         }
         if (c.lmfdsnow) {
-          csrc_r = p.psnde[b2x + (int64_t)k * nproma] * zdtgdp;
-          csrc_s = p.psnde[b2x + (int64_t)k * nproma + s2] * zdtgdp;
+          csrc_r = LD(p.psnde, b2x + k * lstep) * zdtgdp;
+          csrc_s = LD(p.psnde, b2x + k * lstep + sstep) * zdtgdp;
           dr = dr + csrc_r;
           ds = ds + csrc_s;
         }
-        p.plude[i2] = plude;
+        ST(p.plude, i2, plude);
       }
 
       // 3.3 subsidence from the level above (:1419-1492)
@@ -349,12 +383,12 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
           zldifdt = R(2.0) * c.rcldiff_convi * zldifdt;
         }
         if (zli > zepsec) {
-          const R zqe = (qv - za * zqsmix) / m_max(zepsec, R(1) - za);
+          const R zqe = m_div(qv - za * zqsmix, m_max(zepsec, R(1) - za));
           const R ze = m_max(zqsmix - zqe, R(0));
           R zleros = R(0.333) * za * (R(1) - za) * zldifdt * ze;
           zleros = m_min(zleros, zevaplimmix);
           zleros = m_min(zleros, zli);
-          const R zaeros = R(0.5) * zleros / zlicld;
+          const R zaeros = m_div(R(0.5) * zleros, zlicld);
           zsolac = zsolac - zaeros;
           vl = vl + zliqfrac * zleros;
           vi = vi + zicefrac * zleros;
@@ -362,9 +396,9 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
       }
 
       // 4.1 sedimentation (:1783-1877)
-      const R zfallcorr = m_pow(c.rdensref / zrho, R(0.4));
+      const R zfallcorr = m_pow(m_div(c.rdensref, zrho), R(0.4));
       const R ztemp_r = R(0.5) * (qr + qxnm1_r);
-      const R zrlambda = m_pow(zrho * ztemp_r / c.rcl_lam1r, c.rcl_lam2r);
+      const R zrlambda = m_pow(m_div(zrho * ztemp_r, c.rcl_lam1r), c.rcl_lam2r);
       const R vfall_r = c.rxfallspeedrain * zfallcorr * c.rcl_const7r * m_pow(zrlambda, c.rcl_dr);
       const R vfall_i = c.rvice, vfall_s = c.rvsnow;
       if (above_top) {
@@ -392,9 +426,9 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
                            (R(1) - m_min(za_prev, R(1) - R(1.E-06))));
         zcovptot = m_max(zcovptot, c.rcovpmin);
         zcovpclr = m_max(R(0), zcovptot - za);
-        zraincld = R(0.5) * (qr + qr_prev) / zcovptot;
-        zraincldm1 = R(0.5) * (qr + qxnm1_r) / zcovptot;
-        zsnowcld = R(0.5) * (qs + qs_prev) / zcovptot;
+        zraincld = m_div(R(0.5) * (qr + qr_prev), zcovptot);
+        zraincldm1 = m_div(R(0.5) * (qr + qxnm1_r), zcovptot);
+        zsnowcld = m_div(R(0.5) * (qs + qs_prev), zcovptot);
         zcovpmax = m_max(zcovptot, zcovpmax);
       } else {
         zcovptot = R(0);
@@ -413,12 +447,12 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
 
       // 4.4 rain evaporation (:2271-2378)
       {
-        R zzrh = c.rprecrhmax + (R(1) - c.rprecrhmax) * zcovpmax / m_max(zepsec, R(1) - za);
+        R zzrh = c.rprecrhmax + m_div((R(1) - c.rprecrhmax) * zcovpmax, m_max(zepsec, R(1) - za));
         zzrh = m_min(m_max(zzrh, c.rprecrhmax), R(1));
         R zrhc = c.ramid;
-        const R zsigk = pap / paph_surf;
+        const R zsigk = m_div(pap, paph_surf);
         if (zsigk > R(0.8)) {
-          const R s = (zsigk - R(0.8)) / R(0.2);
+          const R s = m_div(zsigk - R(0.8), R(0.2));
           zrhc = c.ramid + (R(1) - c.ramid) * (s * s);
         }
         zzrh = m_min(zrhc, zzrh);
@@ -426,18 +460,18 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
         const R zqe = m_max(R(0), m_min(qv, zqsliq));
         if (zcovpclr > zepsec && zraincldm1 > zepsec && zqe < zzrh * zqsliq) {
           const R zpreclr = zraincldm1;
-          const R zfallcorr2 = m_pow(c.rdensref / zrho, R(0.4f));
+          const R zfallcorr2 = m_pow(m_div(c.rdensref, zrho), R(0.4f));
           const R zesatliq = c.rv_rd * foeeliq;
-          const R zlambda = m_pow(c.rcl_lam1r / (zrho * zpreclr), c.rcl_lam2r);
+          const R zlambda = m_pow(m_div(c.rcl_lam1r, zrho * zpreclr), c.rcl_lam2r);
           const R t = ztp1;
           const R zevap_denom = c.rcl_cdenom1 * zesatliq - c.rcl_cdenom2 * t * zesatliq + c.rcl_cdenom3 * (t * t * t) * pap;
-          const R zcorr2 = m_pow(t / R(273.), R(1.5)) * R(393.) / (t + R(120.));
+          const R zcorr2 = m_div(m_pow(m_div(t, R(273.)), R(1.5)) * R(393.), t + R(120.));
           const R zsubsat = m_max(zzrh * zqsliq - zqe, R(0));
-          const R zbeta = (R(0.15) / zqsliq) * (t * t) * zesatliq * c.rcl_const1r * (zcorr2 / zevap_denom) *
-                          (R(0.78) / m_pow(zlambda, c.rcl_const4r) +
-                           c.rcl_const2r * m_sqrt(zrho * zfallcorr2) / (m_sqrt(zcorr2) * m_pow(zlambda, c.rcl_const3r)));
+          const R zbeta = m_div(R(0.15), zqsliq) * (t * t) * zesatliq * c.rcl_const1r * m_div(zcorr2, zevap_denom) *
+                          (m_div(R(0.78), m_pow(zlambda, c.rcl_const4r)) +
+                           m_div(c.rcl_const2r * m_sqrt(zrho * zfallcorr2), m_sqrt(zcorr2) * m_pow(zlambda, c.rcl_const3r)));
           const R zdenom = R(1) + zbeta * ptsphy * zcorqsliq;
-          const R zdpevap = zcovpclr * zbeta * ptsphy * zsubsat / zdenom;
+          const R zdpevap = m_div(zcovpclr * zbeta * ptsphy * zsubsat, zdenom);
           const R zevap = m_min(zdpevap, qxfg_r);
           vr = vr + zevap;
           qxfg_r = qxfg_r - zevap;
@@ -450,21 +484,21 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
         if (za_prev < c.rcldtopcf && za >= c.rcldtopcf) {
           zcldtopdist = R(0);
         } else {
-          zcldtopdist = zcldtopdist + zdp / (zrho * c.rg);
+          zcldtopdist = zcldtopdist + m_div(zdp, zrho * c.rg);
         }
         if (ztp1 < c.rtt - R(5.) && qxfg_l > rlmin) {
-          const R zsupersatice = (zqsliq - zqsice) / zqsice;
+          const R zsupersatice = m_div(zqsliq - zqsice, zqsice);
           const R zicenuclei = R(1000.0) * m_exp(R(12.96) * zsupersatice - R(0.639));
           const R t = ztp1;
-          const R zadd = c.rlstt * (c.rlstt / (c.rv * t) - R(1)) / (R(2.4E-2) * t);
-          const R zbdd = c.rv * t * pap / (R(2.21) * zvpice);
-          const R zcvds = R(7.8) * m_pow(zicenuclei / zrho, R(0.666)) * zsupersatice / (R(8.87) * (zadd + zbdd));
-          const R zice0 = m_max(zicecld, zicenuclei * c.riceinit / zrho);
+          const R zadd = m_div(c.rlstt * (m_div(c.rlstt, c.rv * t) - R(1)), R(2.4E-2) * t);
+          const R zbdd = m_div(c.rv * t * pap, R(2.21) * zvpice);
+          const R zcvds = m_div(R(7.8) * m_pow(m_div(zicenuclei, zrho), R(0.666)) * zsupersatice, R(8.87) * (zadd + zbdd));
+          const R zice0 = m_max(zicecld, m_div(zicenuclei * c.riceinit, zrho));
           const R zinew = m_pow(R(0.666) * zcvds * ptsphy + m_pow(zice0, R(0.666)), R(1.5));
           R zdepos = m_max(c.rcl_overlapliqice * za * (zinew - zice0), R(0));
           zdepos = m_min(zdepos, qxfg_l);
-          const R zinfactor = m_min(zicenuclei / R(15000.), R(1));
-          zdepos = zdepos * m_min(zinfactor + (R(1) - zinfactor) * (c.rdepliqrefrate + zcldtopdist / c.rdepliqrefdepth), R(1));
+          const R zinfactor = m_min(m_div(zicenuclei, R(15000.)), R(1));
+          zdepos = zdepos * m_min(zinfactor + (R(1) - zinfactor) * (c.rdepliqrefrate + m_div(zcldtopdist, c.rdepliqrefdepth)), R(1));
           il = il + zdepos;
           qxfg_i = qxfg_i + zdepos;
           qxfg_l = qxfg_l - zdepos;
@@ -473,16 +507,16 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
 
       // 4.7 deposition onto snow, IDEPSNOW = 1 (:2700-2745)
       {
-        const R zsupersatice = m_max((qv - zqsmix) / zqsmix, R(0));
+        const R zsupersatice = m_max(m_div(qv - zqsmix, zqsmix), R(0));
         if (ztp1 < c.rtt && zsnowcld > zepsec && zsupersatice > R(0)) {
           const R t = ztp1;
           const R zaplusb = c.rcl_apb1 * zvpice - c.rcl_apb2 * zvpice * t + pap * c.rcl_apb3 * (t * t * t);
-          const R zcorrfac = m_sqrt(R(1) / zrho);
-          const R zcorrfac2 = m_pow(t / R(273.0), R(1.5)) * (R(393.0) / (t + R(120.0)));
+          const R zcorrfac = m_sqrt(m_div(R(1), zrho));
+          const R zcorrfac2 = m_pow(m_div(t, R(273.0)), R(1.5)) * m_div(R(393.0), t + R(120.0));
           const R zpr02 = zrho * zsnowcld * c.rcl_const1s;
-          const R zterm1 = zsupersatice * (t * t) * zvpice * zcorrfac2 * c.rcl_const2s / (zrho * zaplusb);
+          const R zterm1 = m_div(zsupersatice * (t * t) * zvpice * zcorrfac2 * c.rcl_const2s, zrho * zaplusb);
           const R zterm2 = R(0.65) * c.rcl_const6s * m_pow(zpr02, c.rcl_const4s) +
-                           c.rcl_const3s * m_sqrt(zcorrfac) * m_sqrt(zrho) * m_pow(zpr02, c.rcl_const5s) / m_sqrt(zcorrfac2);
+                           m_div(c.rcl_const3s * m_sqrt(zcorrfac) * m_sqrt(zrho) * m_pow(zpr02, c.rcl_const5s), m_sqrt(zcorrfac2));
           R zpsdep = m_max(zcovptot * zterm1 * zterm2 * ptsphy, R(0));
           zpsdep = m_min(zpsdep, qv - zqsmix) / zcorqsmix;
           vs = vs - zpsdep;  // ZSOLQA(S,V) += ZPSDEP, ZSOLQA(V,S) -= ZPSDEP
@@ -496,7 +530,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
       // 4.8 ice -> snow autoconversion (:2769-2800)
       if (ztp1 <= c.rtt && zicecld > zepsec) {
         const R zzco = ptsphy * c.rsnowlin1 * m_exp(c.rsnowlin2 * (ztp1 - c.rtt));
-        const R r = zicecld / c.rlcritsnow;
+        const R r = m_div(zicecld, c.rlcritsnow);
         const R ztemp = R(-1.) * (r * r);
         const R zsnowaut = (ztemp < R(50.0)) ? zzco * (R(1) - m_exp(ztemp)) : zzco;
         qbsi = qbsi + zsnowaut;
@@ -509,16 +543,16 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
         if (zicetot > zepsec && ztp1 > c.rtt) {
           const R zsubsat = m_max(zqsice - qv, R(0));
           const R ztdmtw0 = ztp1 - c.rtt - zsubsat * (R(1329.31) + R(0.0074615) * (pap - R(0.85E5)) - R(40.637) * (ztp1 - R(275.0)));
-          const R zcons1 = m_abs(ptsphy * (R(1) + R(0.5) * ztdmtw0) / c.rtaumel);
+          const R zcons1 = m_abs(m_div(ptsphy * (R(1) + R(0.5) * ztdmtw0), c.rtaumel));
           zmeltmax = m_max(ztdmtw0 * zcons1 * c.zrldcp, R(0));
         }
         if (zmeltmax > zepsec && zicetot > zepsec) {
-          R zalfa = qxfg_i / zicetot;
+          R zalfa = m_div(qxfg_i, zicetot);
           R zmelt = m_min(qxfg_i, zalfa * zmeltmax);
           qxfg_i = qxfg_i - zmelt;
           qxfg_r = qxfg_r + zmelt;
           ri = ri + zmelt;  // ZSOLQA(R,I) += melt
-          zalfa = qxfg_s / zicetot;
+          zalfa = m_div(qxfg_s, zicetot);
           zmelt = m_min(qxfg_s, zalfa * zmeltmax);
           qxfg_s = qxfg_s - zmelt;
           qxfg_r = qxfg_r + zmelt;
@@ -536,12 +570,12 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
         if (ztp1 < c.rtt) {
           R zfrzmax;
           if (llrainliq) {
-            const R zlambda = m_pow(c.rcl_lam1r / (zrho * qr), c.rcl_lam2r);
+            const R zlambda = m_pow(m_div(c.rcl_lam1r, zrho * qr), c.rcl_lam2r);
             const R ztemp = m_min(c.rcl_fzrab * (ztp1 - c.rtt), R(50.));
-            const R zfrz = ptsphy * (c.rcl_const5r / zrho) * (m_exp(ztemp) - R(1.)) * m_pow(zlambda, c.rcl_const6r);
+            const R zfrz = ptsphy * m_div(c.rcl_const5r, zrho) * (m_exp(ztemp) - R(1.)) * m_pow(zlambda, c.rcl_const6r);
             zfrzmax = m_max(zfrz, R(0));
           } else {
-            const R zcons1 = m_abs(ptsphy * (R(1) + R(0.5) * (c.rtt - ztp1)) / c.rtaumel);
+            const R zcons1 = m_abs(m_div(ptsphy * (R(1) + R(0.5) * (c.rtt - ztp1)), c.rtaumel));
             zfrzmax = m_max((c.rtt - ztp1) * zcons1 * c.zrldcp, R(0));
           }
           if (zfrzmax > zepsec) {
@@ -563,7 +597,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
       // 4.9 riming of snow (:2975-3017)
       if (ztp1 <= c.rtt && zliqcld > zepsec) {
         if (zsnowcld > zepsec && zcovptot > R(0.01)) {
-          const R zfallcorr3 = m_pow(c.rdensref / zrho, R(0.4f));
+          const R zfallcorr3 = m_pow(m_div(c.rdensref, zrho), R(0.4f));
           R zsnowrime = c.rcl_effrime * zcovptot * ptsphy * c.rcl_const7s * zfallcorr3 *
                         m_pow(zrho * zsnowcld * c.rcl_const1s, c.rcl_const8s);
           zsnowrime = m_min(zsnowrime, R(1));
@@ -573,23 +607,23 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
 
       // PSD slopes, PSACR and PIACR (:3029-3198)
       if (ztp1 <= c.rtt && prainfrac < zepsec && zraincld > zepsec) {
-        const R zlamr = m_pow(c.rcl_lam1r / (zrho * zraincld), c.rcl_lam2r);
+        const R zlamr = m_pow(m_div(c.rcl_lam1r, zrho * zraincld), c.rcl_lam2r);
         const R zn0r = c.rcl_x1r * m_pow(zlamr, c.rcl_x2r);
         const R lr3 = zlamr * zlamr * zlamr;
         if (zsnowcld > zepsec) {
-          const R zlams = m_pow(c.rcl_lam1s / (zrho * zsnowcld), c.rcl_lam2s);
+          const R zlams = m_pow(m_div(c.rcl_lam1s, zrho * zsnowcld), c.rcl_lam2s);
           const R zfalldiff = m_max(m_abs(vfall_r - vfall_s), (vfall_r + vfall_s) / R(8.));
-          const R zfac = c.rpi * c.rpi * c.rdenswat * R(1.0) * zrho * zn0r * c.rcl_x1s / (zlams * lr3);
-          const R zrat = zlamr / zlams;
-          const R zpsacr = zcovptot * ptsphy * zfac * zfalldiff * ((R(0.5) * zrat + R(2.)) * zrat + R(5.)) / lr3;
+          const R zfac = m_div(c.rpi * c.rpi * c.rdenswat * R(1.0) * zrho * zn0r * c.rcl_x1s, zlams * lr3);
+          const R zrat = m_div(zlamr, zlams);
+          const R zpsacr = m_div(zcovptot * ptsphy * zfac * zfalldiff * ((R(0.5) * zrat + R(2.)) * zrat + R(5.)), lr3);
           rs = rs - zpsacr;  // ZSOLQA(S,R) += psacr
         }
         if (zicecld > zepsec) {
-          const R zlami = m_pow(c.rcl_lam1s / (zrho * zicecld), c.rcl_lam2s);
+          const R zlami = m_pow(m_div(c.rcl_lam1s, zrho * zicecld), c.rcl_lam2s);
           const R zfalldiff = m_max(m_abs(vfall_r - vfall_i), (vfall_r + vfall_i) / R(8.));
-          const R zfac = c.rpi * c.rpi * c.rdenswat * R(0.1) * zrho * zn0r * c.rcl_x1s / (zlami * lr3);
-          const R zrat = zlamr / zlami;
-          const R zpiacr = zcovptot * ptsphy * zfac * zfalldiff * ((R(0.5) * zrat + R(2.)) * zrat + R(5.)) / lr3;
+          const R zfac = m_div(c.rpi * c.rpi * c.rdenswat * R(0.1) * zrho * zn0r * c.rcl_x1s, zlami * lr3);
+          const R zrat = m_div(zlamr, zlami);
+          const R zpiacr = m_div(zcovptot * ptsphy * zfac * zfalldiff * ((R(0.5) * zrat + R(2.)) * zrat + R(5.)), lr3);
           ri = ri - zpiacr;  // ZSOLQA(I,R) += piacr
         }
       }
@@ -598,20 +632,20 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
       {
         R zzrh = c.rprecrhmax + (R(1) - c.rprecrhmax) * zcovpmax / m_max(zepsec, R(1) - za);
         zzrh = m_min(m_max(zzrh, c.rprecrhmax), R(1));
-        R zqe = (qv - za * zqsice) / m_max(zepsec, R(1) - za);
+        R zqe = m_div(qv - za * zqsice, m_max(zepsec, R(1) - za));
         zqe = m_max(R(0), m_min(zqe, zqsice));
         if (zcovpclr > zepsec && qxfg_s > zepsec && zqe < zzrh * zqsice) {
           const R dd = zcovptot * zdtgdp;
           const R den = m_max(m_abs(dd), c.zepsilon);
-          const R zpreclr = qxfg_s * zcovpclr / (dd < R(0) ? -den : den);
-          const R zbeta1 = m_sqrt(pap / paph_surf) / c.rvrfactor * zpreclr / m_max(zcovpclr, zepsec);
+          const R zpreclr = m_div(qxfg_s * zcovpclr, dd < R(0) ? -den : den);
+          const R zbeta1 = m_div(m_div(m_sqrt(m_div(pap, paph_surf)), c.rvrfactor) * zpreclr, m_max(zcovpclr, zepsec));
           const R zbeta = c.rg * c.rpecons * m_pow(zbeta1, R(0.5777));
           const R zdenom = R(1) + zbeta * ptsphy * zcorqsice;
-          const R zdpr = zcovpclr * zbeta * (zqsice - zqe) / zdenom * zdp * c.zrg_r;
+          const R zdpr = m_div(zcovpclr * zbeta * (zqsice - zqe), zdenom) * zdp * c.zrg_r;
           const R zdpevap = zdpr * zdtgdp;
           const R zevap = m_min(zdpevap, qxfg_s);
           vs = vs + zevap;
-          zcovptot = m_max(c.rcovpmin, zcovptot - m_max(R(0), (zcovptot - za) * zevap / qxfg_s));
+          zcovptot = m_max(c.rcovpmin, zcovptot - m_max(R(0), m_div((zcovptot - za) * zevap, qxfg_s)));
           qxfg_s = qxfg_s - zevap;
         }
       }
@@ -760,9 +794,9 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
         const RD zq_upd = RD(pq) + RD(ptsphy) * (RD(tq) + RD(tcml_q));
         const RD zt_upd = RD(pt) + RD(ptsphy) * (RD(tt) + RD(tcml_t));
         const RD za_upd = RD(pa) + RD(ptsphy) * (RD(ta) + RD(tcml_a));
-        const R pp = ssc_off ? p.pap[i2 + ssc_off] : pap;
+        const R pp = ssc_off ? LD(p.pap, i2 + (uint32_t)ssc_off * E) : pap;
         RD t_adj, q_adj, a_adj, l_adj, i_adj;
-        supersatcheck<RD>(cd, zt_upd, zq_upd, za_upd, RD(pp), t_adj, q_adj, a_adj, l_adj, i_adj);
+        supersatcheck<RD, KCD>(cd, zt_upd, zq_upd, za_upd, RD(pp), t_adj, q_adj, a_adj, l_adj, i_adj);
         tq = tq + R(q_adj) * zqtmst;
         tt = tt + R(t_adj) * zqtmst;
         ta = ta + R(a_adj) * zqtmst;
@@ -784,17 +818,17 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
           tt = tt - c.ralsdcp * zi * zqtmst;
         }
       }
-      p.pcovptot[i2] = zcovptot;  // (:3814-3816)
+      ST(p.pcovptot, i2, zcovptot);  // (:3814-3816)
     }
 
     // store tendencies of this level (levels above NCLDTOP keep the tidy-up terms only)
-    p.tloc_t[i2] = tt;
-    p.tloc_q[i2] = tq;
-    p.tloc_a[i2] = ta;
-    p.tloc_cld[i5] = tcld_l;
-    p.tloc_cld[i5 + s2] = tcld_i;
-    p.tloc_cld[i5 + 2 * s2] = tcld_r;
-    p.tloc_cld[i5 + 3 * s2] = tcld_s;
+    ST(p.tloc_t, i2, tt);
+    ST(p.tloc_q, i2, tq);
+    ST(p.tloc_a, i2, ta);
+    ST(p.tloc_cld, i5, tcld_l);
+    ST(p.tloc_cld, i5 + sstep, tcld_i);
+    ST(p.tloc_cld, i5 + 2 * sstep, tcld_r);
+    ST(p.tloc_cld, i5 + 3 * sstep, tcld_s);
 
     // DDH fluxes at the lower half level (:4244-4291)
     {
@@ -809,16 +843,16 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
       sqif = sqif + (qxn_i - qi0 + pvfi * ptsphy) * zgdph_r + pludeli2;
       cqnng = cqnng + zlneg_i * zgdph_r;
       sqitur = sqitur + pvfi * ptsphy * zgdph_r;
-      p.pfsqlf[ih1] = sqlf;
-      p.pfsqif[ih1] = sqif;
-      p.pfsqrf[ih1] = sqrf;
-      p.pfsqsf[ih1] = sqsf;
-      p.pfcqlng[ih1] = cqlng;
-      p.pfcqnng[ih1] = cqnng;
-      p.pfcqrng[ih1] = cqrng;
-      p.pfcqsng[ih1] = cqsng;
-      p.pfsqltur[ih1] = sqltur;
-      p.pfsqitur[ih1] = sqitur;
+      ST(p.pfsqlf, ih1, sqlf);
+      ST(p.pfsqif, ih1, sqif);
+      ST(p.pfsqrf, ih1, sqrf);
+      ST(p.pfsqsf, ih1, sqsf);
+      ST(p.pfcqlng, ih1, cqlng);
+      ST(p.pfcqnng, ih1, cqnng);
+      ST(p.pfcqrng, ih1, cqrng);
+      ST(p.pfcqsng, ih1, cqsng);
+      ST(p.pfsqltur, ih1, sqltur);
+      ST(p.pfsqitur, ih1, sqitur);
     }
 
     // carry to the next level
@@ -833,14 +867,14 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KConst<R> c
   }
   // precipitation fluxes at the surface half level
   {
-    const int64_t ih = bh + (int64_t)klev * nproma;
+    const uint32_t ih = bh + klev * lstep;
     const R fl = pf_r + R(0), fn = pf_s + pf_i;
-    p.pfplsl[ih] = fl;
-    p.pfplsn[ih] = fn;
-    p.pfhpsl[ih] = -c.rlvtt * fl;
-    p.pfhpsn[ih] = -c.rlstt * fn;
+    ST(p.pfplsl, ih, fl);
+    ST(p.pfplsn, ih, fn);
+    ST(p.pfhpsl, ih, -c.rlvtt * fl);
+    ST(p.pfhpsn, ih, -c.rlstt * fn);
   }
-  p.prainfrac_toprfz[b1] = prainfrac;
+  ST(p.prainfrac_toprfz, o1, prainfrac);
 }
 
 // ---------------------------------------------------------------------------
@@ -893,17 +927,97 @@ KConst<T> make_kconst(const cloudsc_params_t& q, double ptsphy_in) {
   return c;
 }
 
+template <typename T>
+KConstDefault<T> make_kconst_default(const cloudsc_params_t& q, double ptsphy_in) {
+  KConstDefault<T> c{};
+  const T ptsphy = T(ptsphy_in);
+  c.ptsphy = ptsphy;
+  c.zqtmst = T(1) / ptsphy;
+  c.rdtg_r = T(1) / (ptsphy * KConstDefault<T>::rg);
+  c.c_aut = T(cloudsc_defaults::rcl_inhomogaut) * ptsphy * T(cloudsc_defaults::rcl_kkaau);
+  c.c_acc = T(cloudsc_defaults::rcl_inhomogacc) * ptsphy * T(cloudsc_defaults::rcl_kkaac);
+  c.rssicefactor = T(q.rssicefactor);
+  c.ncldtop = q.ncldtop;
+  return c;
+}
+
+// True when every configuration value except RSSICEFACTOR (time step) and NCLDTOP (levels)
+// equals the default configuration the specialised kernels are compiled with.
+bool is_default_config(const cloudsc_params_t& q) {
+  using namespace cloudsc_defaults;
+#define CLOUDSC_SAME(name) (q.name == name) &&
+  return CLOUDSC_SAME(rg) CLOUDSC_SAME(rd) CLOUDSC_SAME(rv) CLOUDSC_SAME(rcpd) CLOUDSC_SAME(retv) CLOUDSC_SAME(rtt)
+      CLOUDSC_SAME(rlvtt) CLOUDSC_SAME(rlstt) CLOUDSC_SAME(rpi) CLOUDSC_SAME(r2es) CLOUDSC_SAME(r3les)
+      CLOUDSC_SAME(r3ies) CLOUDSC_SAME(r4les) CLOUDSC_SAME(r4ies) CLOUDSC_SAME(r5les) CLOUDSC_SAME(r5ies)
+      CLOUDSC_SAME(r5alvcp) CLOUDSC_SAME(r5alscp) CLOUDSC_SAME(ralvdcp) CLOUDSC_SAME(ralsdcp) CLOUDSC_SAME(rtwat)
+      CLOUDSC_SAME(rtice) CLOUDSC_SAME(rtwat_rtice_r) CLOUDSC_SAME(rkoop1) CLOUDSC_SAME(rkoop2) CLOUDSC_SAME(ramid)
+      CLOUDSC_SAME(ramin) CLOUDSC_SAME(rlmin) CLOUDSC_SAME(rcldiff) CLOUDSC_SAME(rcldiff_convi)
+      CLOUDSC_SAME(rcl_inhomogaut) CLOUDSC_SAME(rcl_inhomogacc) CLOUDSC_SAME(rsnowlin1) CLOUDSC_SAME(rsnowlin2)
+      CLOUDSC_SAME(rlcritsnow) CLOUDSC_SAME(rvice) CLOUDSC_SAME(rvrain) CLOUDSC_SAME(rvsnow)
+      CLOUDSC_SAME(rxfallspeedrain) CLOUDSC_SAME(rcldtopcf) CLOUDSC_SAME(rdepliqrefrate)
+      CLOUDSC_SAME(rdepliqrefdepth) CLOUDSC_SAME(rcl_overlapliqice) CLOUDSC_SAME(rcl_effrime) CLOUDSC_SAME(riceinit)
+      CLOUDSC_SAME(rthomo) CLOUDSC_SAME(rprecrhmax) CLOUDSC_SAME(rpecons) CLOUDSC_SAME(rvrfactor)
+      CLOUDSC_SAME(rcovpmin) CLOUDSC_SAME(rtaumel) CLOUDSC_SAME(rdenswat) CLOUDSC_SAME(rdensref) CLOUDSC_SAME(rcl_dr)
+      CLOUDSC_SAME(rcl_x1r) CLOUDSC_SAME(rcl_x2r) CLOUDSC_SAME(rcl_x1s) CLOUDSC_SAME(rcl_lam1r)
+      CLOUDSC_SAME(rcl_lam2r) CLOUDSC_SAME(rcl_lam1s) CLOUDSC_SAME(rcl_lam2s) CLOUDSC_SAME(rcl_cdenom1)
+      CLOUDSC_SAME(rcl_cdenom2) CLOUDSC_SAME(rcl_cdenom3) CLOUDSC_SAME(rcl_const1r) CLOUDSC_SAME(rcl_const2r)
+      CLOUDSC_SAME(rcl_const3r) CLOUDSC_SAME(rcl_const4r) CLOUDSC_SAME(rcl_const5r) CLOUDSC_SAME(rcl_const6r)
+      CLOUDSC_SAME(rcl_const7r) CLOUDSC_SAME(rcl_kkaac) CLOUDSC_SAME(rcl_kkbac) CLOUDSC_SAME(rcl_kkaau)
+      CLOUDSC_SAME(rcl_kkbauq) CLOUDSC_SAME(rcl_kkbaun) CLOUDSC_SAME(rcl_kk_cloud_num_sea)
+      CLOUDSC_SAME(rcl_kk_cloud_num_land) CLOUDSC_SAME(rcl_fzrab) CLOUDSC_SAME(rcl_apb1) CLOUDSC_SAME(rcl_apb2)
+      CLOUDSC_SAME(rcl_apb3) CLOUDSC_SAME(rcl_const1s) CLOUDSC_SAME(rcl_const2s) CLOUDSC_SAME(rcl_const3s)
+      CLOUDSC_SAME(rcl_const4s) CLOUDSC_SAME(rcl_const5s) CLOUDSC_SAME(rcl_const6s) CLOUDSC_SAME(rcl_const7s)
+      CLOUDSC_SAME(rcl_const8s) CLOUDSC_SAME(rmfadvw) CLOUDSC_SAME(rmfadvwdd) (q.lmfdsnow == lmfdsnow);
+#undef CLOUDSC_SAME
+}
+
+template <typename R>
+KPtrs<R> offset_ptrs(const KPtrs<R>& p, int64_t b0, int64_t nproma, int64_t klev) {
+  const int64_t s1 = nproma, s2 = nproma * klev, sh = nproma * (klev + 1);
+  KPtrs<R> q = p;
+  q.pt += b0 * s2; q.pq += b0 * s2; q.tcml_t += b0 * s2; q.tcml_q += b0 * s2; q.tcml_a += b0 * s2;
+  q.tcml_cld += b0 * s2 * 5; q.tloc_t += b0 * s2; q.tloc_q += b0 * s2; q.tloc_a += b0 * s2;
+  q.tloc_cld += b0 * s2 * 5; q.pvfl += b0 * s2; q.pvfi += b0 * s2; q.pap += b0 * s2; q.paph += b0 * sh;
+  q.plsm += b0 * s1; q.ldcum += b0 * s1; q.ktype += b0 * s1; q.peis += b0 * s1; q.plu += b0 * s2;
+  q.plude += b0 * s2; q.pludeli += b0 * s2 * 4; q.psnde += b0 * s2 * 2; q.pmfu += b0 * s2; q.pmfd += b0 * s2;
+  q.pa += b0 * s2; q.pclv += b0 * s2 * 5; q.pcovptot += b0 * s2; q.prainfrac_toprfz += b0 * s1;
+  R** h[] = {&q.pfsqlf, &q.pfsqif, &q.pfcqnng, &q.pfcqlng, &q.pfsqrf, &q.pfsqsf, &q.pfcqrng,
+             &q.pfcqsng, &q.pfsqltur, &q.pfsqitur, &q.pfplsl, &q.pfplsn, &q.pfhpsl, &q.pfhpsn};
+  for (R** x : h) *x += b0 * sh;
+  return q;
+}
+
 template <typename R, typename RD>
 hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
                           hipStream_t stream) {
   constexpr int BLOCK = 256;
-  const KConst<R> c = make_kconst<R>(q, ptsphy);
-  const KConst<RD> cd = make_kconst<RD>(q, ptsphy);
-  const int64_t ncols = (int64_t)d.ngpblks * d.nproma;
-  const int64_t nblk = (ncols + BLOCK - 1) / BLOCK;
-  hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, BLOCK>), dim3((unsigned)nblk), dim3(BLOCK), 0, stream, c, cd, p, d,
-                     ssc_off);
-  return hipGetLastError();
+  const bool def = is_default_config(q);
+  // blocks per launch: the largest array (NPROMA,KLEV,5) must stay below 4 GiB for 32-bit offsets
+  const int64_t per_block = (int64_t)d.nproma * d.klev * 5 * (int64_t)sizeof(R);
+  const int64_t max_blocks = std::max<int64_t>(1, ((int64_t)1 << 32) / per_block - 1);
+  for (int64_t b0 = 0; b0 < d.ngpblks; b0 += max_blocks) {
+    const int64_t nb = std::min<int64_t>(max_blocks, d.ngpblks - b0);
+    KDims dc = d;
+    dc.ngpblks = (int32_t)nb;
+    dc.ngptot = (int32_t)(d.ngptot - b0 * d.nproma);
+    const KPtrs<R> pc = offset_ptrs(p, b0, d.nproma, d.klev);
+    const int64_t ncols = nb * d.nproma;
+    const unsigned grid = (unsigned)((ncols + BLOCK - 1) / BLOCK);
+    if (def) {
+      const KConstDefault<R> c = make_kconst_default<R>(q, ptsphy);
+      const KConstDefault<RD> cd = make_kconst_default<RD>(q, ptsphy);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KConstDefault<R>, KConstDefault<RD>, BLOCK>), dim3(grid),
+                         dim3(BLOCK), 0, stream, c, cd, pc, dc, ssc_off);
+    } else {
+      const KConst<R> c = make_kconst<R>(q, ptsphy);
+      const KConst<RD> cd = make_kconst<RD>(q, ptsphy);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KConst<R>, KConst<RD>, BLOCK>), dim3(grid), dim3(BLOCK), 0,
+                         stream, c, cd, pc, dc, ssc_off);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 template hipError_t launch_cloudsc<double, double>(const cloudsc_params_t&, double, const KPtrs<double>&,
