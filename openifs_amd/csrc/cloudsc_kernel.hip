@@ -27,12 +27,18 @@ __device__ __forceinline__ float m_div(float a, float b) { return a * __builtin_
 __device__ __forceinline__ float m_div(float a, float b) { return a / b; }
 #endif
 __device__ __forceinline__ double m_div(double a, double b) { return a / b; }
+#ifdef CLOUDSC_FAST_TRANSCENDENTALS
+// square roots occur only inside rate expressions: v_sqrt_f32 (1 ulp)
+__device__ __forceinline__ float m_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+#else
 __device__ __forceinline__ float m_sqrt(float x) { return sqrtf(x); }
+#endif
 __device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
-template <typename T>
-__device__ __forceinline__ T m_max(T a, T b) { return a < b ? b : a; }
-template <typename T>
-__device__ __forceinline__ T m_min(T a, T b) { return b < a ? b : a; }
+// MAX/MIN on v_max/v_min (identical to the Fortran intrinsics for non-NaN arguments)
+__device__ __forceinline__ float m_max(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ float m_min(float a, float b) { return __builtin_fminf(a, b); }
+__device__ __forceinline__ double m_max(double a, double b) { return __builtin_fmax(a, b); }
+__device__ __forceinline__ double m_min(double a, double b) { return __builtin_fmin(a, b); }
 template <typename T>
 __device__ __forceinline__ T m_abs(T a) { return a < T(0) ? -a : a; }
 
@@ -46,8 +52,8 @@ __device__ __forceinline__ Sat<T> saturation(const KCT& c, T t) {
   Sat<T> s;
   T x = (m_max(c.rtice, m_min(c.rtwat, t)) - c.rtice) * c.rtwat_rtice_r;
   s.alfa = m_min(T(1), x * x);
-  s.el = m_exp(c.r3les * (t - c.rtt) / (t - c.r4les));
-  s.ei = m_exp(c.r3ies * (t - c.rtt) / (t - c.r4ies));
+  s.el = m_exp(m_div(c.r3les * (t - c.rtt), t - c.r4les));
+  s.ei = m_exp(m_div(c.r3ies * (t - c.rtt), t - c.r4ies));
   return s;
 }
 
@@ -72,7 +78,7 @@ __device__ __forceinline__ void supersatcheck(const KCT& c, T pt, T pq, T pa, T 
   const T zcor = T(1) / (T(1) - c.retv * zfoeemix);
   const T zqsmix = zfoeemix * zcor;
   const T dl = zt - c.r4les, di = zt - c.r4ies;
-  const T foedem = s.alfa * c.r5alvcp * (T(1) / (dl * dl)) + (T(1) - s.alfa) * c.r5alscp * (T(1) / (di * di));
+  const T foedem = s.alfa * c.r5alvcp * m_div(T(1), dl * dl) + (T(1) - s.alfa) * c.r5alscp * m_div(T(1), di * di);
   const T zcorqsmix = T(1) + zqsmix * zcor * foedem;
   zqsliqk = T(0.5) * (zqsliqk + zqsmix);
   T zqslim = za * zqsmix + (T(1) - za) * zqsliqk;
@@ -672,16 +678,19 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
         rsum[2] = (((R(0) - ri) - dr) - rs) + vr; // row R: ri, dr, rs, (R,V)=-vr
         rsum[3] = ((R(0) + rs) - ds) + vs;        // row S: (S,R)=-rs, ds, (S,V)=-vs
         rsum[4] = (((R(0) - vl) - vi) - vr) - vs; // row V: vl, vi, vr, vs
-        R zratio[5];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          const R zmax = m_max(qxs[m], zepsec);
-          zratio[m] = zmax / m_max(rsum[m], zmax);
-        }
+        // ZRATIO = ZMAX/MAX(sinksum,ZMAX) is exactly 1 unless the sinks overshoot; with no
+        // overshoot anywhere every scale factor below is 1 and the step is a no-op.
+        R zmax[5];
         bool any_trunc = false;
 #pragma unroll
-        for (int m = 0; m < 5; ++m) any_trunc = any_trunc || (zratio[m] < R(1));
+        for (int m = 0; m < 5; ++m) {
+          zmax[m] = m_max(qxs[m], zepsec);
+          any_trunc = any_trunc || (rsum[m] > zmax[m]);
+        }
         if (any_trunc) {
+          R zratio[5];
+#pragma unroll
+          for (int m = 0; m < 5; ++m) zratio[m] = rsum[m] > zmax[m] ? m_div(zmax[m], rsum[m]) : R(1);
           // stable ascending order of zratio (insertion sort of :3479-3495)
           int rank[5];
 #pragma unroll
@@ -697,37 +706,37 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
 #pragma unroll
             for (int m = 0; m < 5; ++m) jo = (rank[m] == s) ? m : jo;
             R ssum;
-            R zmm;
             if (jo == 0) {
               ssum = ((R(0) - dl) + il) + vl;
-              zmm = m_max(ql, zepsec);
             } else if (jo == 1) {
               ssum = (((R(0) - il) - di) + ri) + vi;
-              zmm = m_max(qi, zepsec);
             } else if (jo == 2) {
               ssum = (((R(0) - ri) - dr) - rs) + vr;
-              zmm = m_max(qr, zepsec);
             } else if (jo == 3) {
               ssum = ((R(0) + rs) - ds) + vs;
-              zmm = m_max(qs, zepsec);
             } else {
               ssum = (((R(0) - vl) - vi) - vr) - vs;
-              zmm = m_max(qv, zepsec);
             }
-            const R zz = zmm / m_max(ssum, zmm);
-            // off-diagonal pairs (a,b) with stored x = ZSOLQA(a,b)
-            vl = trunc_scale(vl, jo, 4, 0, zz);
-            vi = trunc_scale(vi, jo, 4, 1, zz);
-            vr = trunc_scale(vr, jo, 4, 2, zz);
-            vs = trunc_scale(vs, jo, 4, 3, zz);
-            il = trunc_scale(il, jo, 1, 0, zz);
-            ri = trunc_scale(ri, jo, 2, 1, zz);
-            rs = trunc_scale(rs, jo, 2, 3, zz);
-            // a negative diagonal entry is scaled twice (:3531-3532)
-            if (jo == 0 && dl < R(0)) dl = (dl * zz) * zz;
-            if (jo == 1 && di < R(0)) di = (di * zz) * zz;
-            if (jo == 2 && dr < R(0)) dr = (dr * zz) * zz;
-            if (jo == 3 && ds < R(0)) ds = (ds * zz) * zz;
+            R zmm = zmax[0];
+#pragma unroll
+            for (int m = 1; m < 5; ++m) zmm = (jo == m) ? zmax[m] : zmm;
+            // ZZRATIO = ZMM/MAX(ZSINKSUM,ZMM) < 1 only on overshoot; otherwise the scaling is by 1
+            if (ssum > zmm) {
+              const R zz = zmm / ssum;
+              // off-diagonal pairs (a,b) with stored x = ZSOLQA(a,b)
+              vl = trunc_scale(vl, jo, 4, 0, zz);
+              vi = trunc_scale(vi, jo, 4, 1, zz);
+              vr = trunc_scale(vr, jo, 4, 2, zz);
+              vs = trunc_scale(vs, jo, 4, 3, zz);
+              il = trunc_scale(il, jo, 1, 0, zz);
+              ri = trunc_scale(ri, jo, 2, 1, zz);
+              rs = trunc_scale(rs, jo, 2, 3, zz);
+              // a negative diagonal entry is scaled twice (:3531-3532)
+              if (jo == 0 && dl < R(0)) dl = (dl * zz) * zz;
+              if (jo == 1 && di < R(0)) di = (di * zz) * zz;
+              if (jo == 2 && dr < R(0)) dr = (dr * zz) * zz;
+              if (jo == 3 && ds < R(0)) ds = (ds * zz) * zz;
+            }
           }
         }
       }
