@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/cloudsc_hip.h"
+#include "capi_util.h"
 #include "cloudsc_kernel.h"
 
 namespace cloudsc_gpu {
@@ -24,11 +25,11 @@ hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<
 using cloudsc_gpu::KDims;
 using cloudsc_gpu::KPtrs;
 
-namespace {
+namespace cloudsc_capi {
 
 thread_local std::string g_last_error;
 
-int fail(int code, const char* what, hipError_t e = hipSuccess) {
+int fail(int code, const char* what, hipError_t e) {
   g_last_error = what;
   if (e != hipSuccess) {
     g_last_error += ": ";
@@ -63,6 +64,14 @@ int check_device() {
   if (!ok[dev]) return fail(CLOUDSC_ENODEVICE, "cloudsc: device is not gfx950 (MI355X)");
   return CLOUDSC_OK;
 }
+
+}  // namespace cloudsc_capi
+
+namespace {
+
+using cloudsc_capi::check_device;
+using cloudsc_capi::check_config;
+using cloudsc_capi::fail;
 
 template <typename R, typename RD>
 int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngpblks, R ptsphy, const KPtrs<R>& p,
@@ -261,6 +270,6 @@ int cloudsc_blocked_dp_c(CLOUDSC_BLOCKED_ARGS(double)) { BLOCKED_CALL(double, do
 int cloudsc_blocked_sp_c(CLOUDSC_BLOCKED_ARGS(float)) { BLOCKED_CALL(float, float); }
 int cloudsc_blocked_mixed_c(CLOUDSC_BLOCKED_ARGS(float)) { BLOCKED_CALL(float, double); }
 
-const char* cloudsc_last_error(void) { return g_last_error.c_str(); }
+const char* cloudsc_last_error(void) { return cloudsc_capi::g_last_error.c_str(); }
 
 }  // extern "C"
