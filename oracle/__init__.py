@@ -12,10 +12,13 @@ Parity status of each oracle function (what pins it; see tests/test_oracle_*.py)
                                  saturation after adjustment to first order), no-op cases
   lu5_solve                      pinned: numpy.linalg.solve on diagonally dominant systems
   truncate_sinks                 pinned: brute-force positivity and ordering properties
-  cloudsc (whole column)         pinned only by column water/enthalpy conservation, the
-                                 reference's write set and special cases (dry, clear-sky,
-                                 warm); process-rate formulas of section 4 are
-                                 parity unpinned (no reference run or golden vector exists)
+  cloudsc (whole column)         pinned by column water/enthalpy conservation, the reference's
+                                 write set and special cases (dry, clear-sky, warm, saturated)
+  section-4 processes            pinned: snow sedimentation (implicit upwind flux form), KK2000
+                                 autoconversion, homogeneous freezing (tests/test_oracle_processes.py);
+                                 parity unpinned: rain evaporation, ice and snow deposition, ice->snow
+                                 autoconversion, melting, rain freezing, riming, PSACR/PIACR, snow
+                                 sublimation, erosion, accretion (no reference run or golden vector exists)
 """
 import ctypes
 import os
