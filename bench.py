@@ -82,6 +82,102 @@ def cpu_baseline(params, sample_cols, nthreads, min_seconds):
     return sample_cols * passes / dt, dt, passes
 
 
+def mlp_flops_per_column(din, dout):
+    """Forward and training (forward + backward) FLOPs per column of the padded MLP."""
+    fwd = sum(2 * a * b for a, b in zip(din, dout))
+    bwd = sum(4 * a * b for a, b in zip(din, dout)) - 2 * din[0] * dout[0]  # no input gradient of layer 0
+    return fwd, fwd + bwd
+
+
+def surrogate_bench(params, train_cols, hybrid_cols, rerun_fraction, gate_fraction, infer_batch):
+    """Trains the surrogate (fresh seeded columns, the plain FP32 kernel as teacher) and times the hybrid
+    against the plain kernel on the same `hybrid_cols` columns."""
+    import numpy as np
+    import torch
+
+    from openifs_amd import cloudsc as C
+    from openifs_amd import surrogate as S
+
+    dev = "cuda:0"
+    sur = S.Surrogate(KLEV, hidden=1024, hidden_layers=3, max_batch=infer_batch, seed=7)
+    fwd_flop, train_flop = mlp_flops_per_column(sur.mlp.din, sur.mlp.dout)
+
+    # training: batches of 4096 columns, Adam lr 3e-4
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = S.train_surrogate(sur, params, train_cols, chunk=262144, lr=3e-4, seed0=5000, batch=4096, device=dev)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    trained_cols = len(losses) * 262144
+
+    # hybrid workload: plain reference first (also its time), then the hybrid on the same inputs
+    f = C.BlockedFields(hybrid_cols, 128, KLEV, "sp", dev)
+    C.synth_device(f, params, 31337)
+    plude0 = f["plude"].clone()
+    scores = sur.gate_scores(f, params)
+    thr = float(torch.quantile(scores[: min(hybrid_cols, 1 << 24)].float().cpu(), 1.0 - gate_fraction)) \
+        if gate_fraction > 0 else float("inf")
+    del scores
+    stream = torch.cuda.current_stream()
+
+    def timed(run, reps=3):
+        run()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            f["plude"].copy_(plude0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            r = run()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts)), r
+
+    plain_ms, _ = timed(lambda: C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream))
+    sp_ref = (f["pfplsl"][:, -1, :].double() + f["pfplsn"][:, -1, :].double()).clone()
+    tq_ref = f["ptendency_loc_q"].clone()
+    tt_ref = f["ptendency_loc_t"].clone()
+    net_ms, _ = timed(lambda: sur.predict(f, params))
+    sp_net = f["pfplsl"][:, -1, :].double() + f["pfplsn"][:, -1, :].double()
+    err_net = float(torch.linalg.vector_norm(sp_net - sp_ref) / torch.linalg.vector_norm(sp_ref))
+    hyb_ms, n_plain = timed(lambda: sur.hybrid(f, params, rerun_fraction=rerun_fraction, rerun_seed=1,
+                                                gate_threshold=thr))
+    sp_h = f["pfplsl"][:, -1, :].double() + f["pfplsn"][:, -1, :].double()
+
+    def rel(a, b):
+        return float(torch.linalg.vector_norm((a.double() - b.double()).flatten()) /
+                     torch.linalg.vector_norm(b.double().flatten()))
+
+    res = {
+        "workload": f"surrogate-hybrid CLOUDSC, NLEV={KLEV}, {hybrid_cols} columns, FP32 state, BF16 MLP "
+                    f"1918-1024-1024-1024-822 (padded {sur.n_in}-...-{sur.n_out})",
+        "columns_per_s": hybrid_cols / (hyb_ms / 1e3),
+        "hybrid_ms": hyb_ms,
+        "plain_ms_same_inputs": plain_ms,
+        "plain_columns_per_s_same_inputs": hybrid_cols / (plain_ms / 1e3),
+        "speedup_vs_plain": plain_ms / hyb_ms,
+        "network_only_ms": net_ms,
+        "network_tflops": fwd_flop * hybrid_cols / (net_ms / 1e3) / 1e12,
+        "network_mfma_frac": fwd_flop * hybrid_cols / (net_ms / 1e3) / 2.5e15,
+        "plain_columns": n_plain,
+        "plain_fraction": n_plain / hybrid_cols,
+        "rerun_fraction": rerun_fraction,
+        "gate_threshold": thr,
+        "surface_precip_relrms_vs_plain": rel(sp_h, sp_ref),
+        "surface_precip_relrms_network_only": err_net,
+        "tendency_t_relrms_vs_plain": rel(f["ptendency_loc_t"], tt_ref),
+        "tendency_q_relrms_vs_plain": rel(f["ptendency_loc_q"], tq_ref),
+        "training": {"columns": trained_cols, "seconds": t_train, "columns_per_s": trained_cols / t_train,
+                     "tflops": train_flop * trained_cols / t_train / 1e12, "batch": 4096, "lr": 3e-4,
+                     "final_loss": losses[-1] if losses else None, "first_loss": losses[0] if losses else None},
+    }
+    del f, plude0, sp_ref, tq_ref, tt_ref
+    sur.close()
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +188,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-hybrid", action="store_true")
+    ap.add_argument("--train-cols", type=int, default=20_000_000)
+    ap.add_argument("--hybrid-cols", type=int, default=6_599_680)
+    ap.add_argument("--rerun-fraction", type=float, default=0.01)
+    ap.add_argument("--gate-fraction", type=float, default=0.02)
+    ap.add_argument("--infer-batch", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (CLOUDSC is column-local; --gpus {args.gpus} "
@@ -165,11 +267,16 @@ def main():
     achieved = alg_bytes / (kern_ms / 1e3) / 1e9
     traffic = read_traffic(args.nproma)
 
+    del f, plude0
+    torch.cuda.empty_cache()
+    hybrid = None
+    if not args.no_hybrid:
+        hybrid = surrogate_bench(params, args.train_cols, args.hybrid_cols, args.rerun_fraction, args.gate_fraction,
+                                 args.infer_batch)
+
     cpu = None
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        del f, plude0
-        torch.cuda.empty_cache()
         cps, cdt, passes = cpu_baseline(params, args.cpu_sample, nthreads, args.cpu_seconds)
         cpu = {"value": cps, "unit": "columns/s", "cores": nthreads, "kind": "port",
                "sample": f"{args.cpu_sample} columns of the same synthetic distributions x {passes} passes "
@@ -197,7 +304,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_column": bpc},
         "err_vs_fp64": err,
-        "hybrid": None,
+        "hybrid": hybrid,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

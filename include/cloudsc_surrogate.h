@@ -76,10 +76,10 @@ int cloudsc_surrogate_dims(const cloudsc_surrogate_t *s, int *klev, int *n_in, i
 int cloudsc_surrogate_norm(cloudsc_surrogate_t *s, float *mu_x, float *sd_x, float *mu_y, float *sd_y, int get);
 /* Computes the statistics from a blocked field set holding plain-kernel inputs and outputs (all NGPTOT columns). */
 int cloudsc_surrogate_fit_norm(cloudsc_surrogate_t *s, CLOUDSC_BLOCKED_ARGS(float));
-/* Trains on all columns of a blocked field set holding plain-kernel inputs and outputs, in consecutive
- * batches of max_batch columns (a trailing partial batch is skipped).  Returns the mean training loss over
- * the batches in *loss (host). */
-int cloudsc_surrogate_train(cloudsc_surrogate_t *s, float lr, float *loss, CLOUDSC_BLOCKED_ARGS(float));
+/* Trains on all columns of a blocked field set holding plain-kernel inputs and outputs, one Adam step per
+ * consecutive batch of `batch` columns (a multiple of 128, <= max_batch; a trailing partial batch is skipped).
+ * Returns the mean training loss over the batches in *loss (host). */
+int cloudsc_surrogate_train(cloudsc_surrogate_t *s, float lr, int batch, float *loss, CLOUDSC_BLOCKED_ARGS(float));
 /* Network-only CLOUDSC: writes PTENDENCY_LOC_T/Q/A, PTENDENCY_LOC_CLD(l,i) and PFPLSL/PFPLSN/PFHPSL/PFHPSN
  * from the network; PFPLSL/PFPLSN split the predicted total by the phase of the level above (liquid where
  * T > RTT); PTENDENCY_LOC_CLD(r,s), PCOVPTOT, PRAINFRAC_TOPRFZ and the ten DDH flux profiles are set to zero;

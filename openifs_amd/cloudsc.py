@@ -154,6 +154,10 @@ def lib():
         _lib.cloudsc_last_error.restype = ctypes.c_char_p
         _lib.cloudsc_params_default.argtypes = [ctypes.POINTER(CloudscParams), ctypes.c_double, ctypes.c_int,
                                                 ctypes.c_void_p]
+        # work-group sizes recorded by the autotuner (openifs_amd/tuning/cloudsc_tuned.json), if any
+        from . import autotune
+
+        autotune.apply(arch="gfx950")
     return _lib
 
 

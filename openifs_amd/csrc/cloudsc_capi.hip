@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/cloudsc_hip.h"
+#include "../../include/cloudsc_tune.h"
 #include "capi_util.h"
 #include "cloudsc_kernel.h"
 
@@ -20,6 +21,8 @@ namespace cloudsc_gpu {
 template <typename R, typename RD>
 hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
                           hipStream_t stream);
+int tuned_block(int variant, int klev);
+void set_tuned_block(int variant, int klev, int block);
 }
 
 using cloudsc_gpu::KDims;
@@ -273,3 +276,12 @@ int cloudsc_blocked_mixed_c(CLOUDSC_BLOCKED_ARGS(float)) { BLOCKED_CALL(float, d
 const char* cloudsc_last_error(void) { return cloudsc_capi::g_last_error.c_str(); }
 
 }  // extern "C"
+
+extern "C" int cloudsc_tune_set_block(int variant, int klev, int block) {
+  if (variant < 0 || variant > 2 || klev < 2 || !(block == 0 || block == 64 || block == 128 || block == 256 || block == 512))
+    return cloudsc_capi::fail(CLOUDSC_EDIMS, "cloudsc_tune_set_block: variant 0..2, KLEV >= 2, block 0/64/128/256/512");
+  cloudsc_gpu::set_tuned_block(variant, klev, block);
+  return CLOUDSC_OK;
+}
+
+extern "C" int cloudsc_tune_get_block(int variant, int klev) { return cloudsc_gpu::tuned_block(variant, klev); }

@@ -3,6 +3,10 @@
 // ifs-source/arpifs/phys_ec/cloudsc.F90 unless another file is named.
 #include <hip/hip_runtime.h>
 
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "cloudsc_kernel.h"
 
 namespace cloudsc_gpu {
@@ -995,11 +999,60 @@ KPtrs<R> offset_ptrs(const KPtrs<R>& p, int64_t b0, int64_t nproma, int64_t klev
   return q;
 }
 
+// Work-group size per (arithmetic variant, KLEV), set by the autotuner (cloudsc_tune_set_block); 256 otherwise.
+namespace {
+std::mutex g_tune_mutex;
+std::map<std::pair<int, int>, int> g_tuned_block;
+}  // namespace
+
+template <typename R, typename RD>
+constexpr int variant_id() {
+  return sizeof(R) == 8 ? 0 : (sizeof(RD) == 8 ? 2 : 1);  // 0 dp, 1 sp, 2 mixed
+}
+
+int tuned_block(int variant, int klev) {
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  const auto it = g_tuned_block.find({variant, klev});
+  return it == g_tuned_block.end() ? 256 : it->second;
+}
+
+void set_tuned_block(int variant, int klev, int block) {
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  if (block <= 0)
+    g_tuned_block.erase({variant, klev});
+  else
+    g_tuned_block[{variant, klev}] = block;
+}
+
+template <typename R, typename RD, typename KC, typename KCD>
+hipError_t launch_variant(int block, unsigned ncols, const KC& c, const KCD& cd, const KPtrs<R>& p, const KDims& d,
+                          int ssc_off, hipStream_t stream) {
+  const unsigned grid = (ncols + block - 1) / block;
+  switch (block) {
+    case 64:
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 64>), dim3(grid), dim3(64), 0, stream, c, cd, p, d,
+                         ssc_off);
+      break;
+    case 128:
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 128>), dim3(grid), dim3(128), 0, stream, c, cd, p,
+                         d, ssc_off);
+      break;
+    case 512:
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 512>), dim3(grid), dim3(512), 0, stream, c, cd, p,
+                         d, ssc_off);
+      break;
+    default:
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256>), dim3((ncols + 255) / 256), dim3(256), 0,
+                         stream, c, cd, p, d, ssc_off);
+  }
+  return hipGetLastError();
+}
+
 template <typename R, typename RD>
 hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
                           hipStream_t stream) {
-  constexpr int BLOCK = 256;
   const bool def = is_default_config(q);
+  const int block = def ? tuned_block(variant_id<R, RD>(), d.klev) : 256;
   // blocks per launch: the largest array (NPROMA,KLEV,5) must stay below 4 GiB for 32-bit offsets
   const int64_t per_block = (int64_t)d.nproma * d.klev * 5 * (int64_t)sizeof(R);
   const int64_t max_blocks = std::max<int64_t>(1, ((int64_t)1 << 32) / per_block - 1);
@@ -1009,20 +1062,20 @@ hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<
     dc.ngpblks = (int32_t)nb;
     dc.ngptot = (int32_t)(d.ngptot - b0 * d.nproma);
     const KPtrs<R> pc = offset_ptrs(p, b0, d.nproma, d.klev);
-    const int64_t ncols = nb * d.nproma;
-    const unsigned grid = (unsigned)((ncols + BLOCK - 1) / BLOCK);
+    const unsigned ncols = (unsigned)(nb * d.nproma);
+    hipError_t e;
     if (def) {
       const KConstDefault<R> c = make_kconst_default<R>(q, ptsphy);
       const KConstDefault<RD> cd = make_kconst_default<RD>(q, ptsphy);
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KConstDefault<R>, KConstDefault<RD>, BLOCK>), dim3(grid),
-                         dim3(BLOCK), 0, stream, c, cd, pc, dc, ssc_off);
+      e = launch_variant<R, RD>(block, ncols, c, cd, pc, dc, ssc_off, stream);
     } else {
       const KConst<R> c = make_kconst<R>(q, ptsphy);
       const KConst<RD> cd = make_kconst<RD>(q, ptsphy);
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KConst<R>, KConst<RD>, BLOCK>), dim3(grid), dim3(BLOCK), 0,
-                         stream, c, cd, pc, dc, ssc_off);
+      const unsigned grid = (ncols + 255) / 256;
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KConst<R>, KConst<RD>, 256>), dim3(grid), dim3(256), 0, stream,
+                         c, cd, pc, dc, ssc_off);
+      e = hipGetLastError();
     }
-    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -485,13 +485,15 @@ int cloudsc_surrogate_fit_norm(cloudsc_surrogate_t* s, CLOUDSC_BLOCKED_ARGS(floa
   return CLOUDSC_OK;
 }
 
-int cloudsc_surrogate_train(cloudsc_surrogate_t* s, float lr, float* loss, CLOUDSC_BLOCKED_ARGS(float)) {
+int cloudsc_surrogate_train(cloudsc_surrogate_t* s, float lr, int batch, float* loss, CLOUDSC_BLOCKED_ARGS(float)) {
   SURR_PTRS;
   int rc = check_fields(s, ydparams, ngptot, nproma, klev, ngpblks, p);
   if (rc) return rc;
   hipLaunchKernelGGL(surr::fill_f32, dim3(1), dim3(64), 0, st, s->loss_acc, 0.0f);
+  if (batch < 128 || batch % 128 || batch > s->max_batch)
+    return fail(CLOUDSC_EDIMS, "cloudsc_surrogate_train: batch must be a multiple of 128 and <= max_batch");
   int nb = 0;
-  const int M = s->max_batch;
+  const int M = batch;
   hipError_t e = hipGetLastError();
   for (int c0 = 0; c0 + M <= ngptot && e == hipSuccess; c0 += M, ++nb) {
     hipLaunchKernelGGL(surr::features, dim3(nblk(M, surr::TILE), nblk(s->nin, surr::TILE)), dim3(256), 0, st, p, d,

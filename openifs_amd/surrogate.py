@@ -201,11 +201,12 @@ class Surrogate:
         _blocked_call("cloudsc_surrogate_fit_norm", fields, params, ptsphy, ctypes.c_void_p(self.h))
 
     # -- training and inference ---------------------------------------------------------------------------------
-    def train(self, fields, params, lr, ptsphy=900.0):
-        """One pass over the columns of ``fields`` (plain-kernel inputs and outputs); returns the mean loss."""
+    def train(self, fields, params, lr, batch=4096, ptsphy=900.0):
+        """One pass over the columns of ``fields`` (plain-kernel inputs and outputs) in batches of ``batch``
+        columns, one Adam step each; returns the mean loss."""
         loss = ctypes.c_float()
         _blocked_call("cloudsc_surrogate_train", fields, params, ptsphy, ctypes.c_void_p(self.h), ctypes.c_float(lr),
-                      ctypes.byref(loss))
+                      ctypes.c_int(batch), ctypes.byref(loss))
         return loss.value
 
     def predict(self, fields, params, ptsphy=900.0):
@@ -254,7 +255,7 @@ class Surrogate:
         return s
 
 
-def train_surrogate(sur, params, n_columns, chunk=262144, lr=3e-4, seed0=1000, nproma=128, log=None,
+def train_surrogate(sur, params, n_columns, chunk=262144, lr=3e-4, seed0=1000, nproma=128, batch=4096, log=None,
                     device="cuda"):
     """Trains ``sur`` on ``n_columns`` fresh synthetic columns (seeds seed0, seed0+1, ...) labelled by the plain
     FP32 kernel, ``chunk`` columns per generated field set.  Statistics are fitted on the first chunk.
@@ -272,7 +273,7 @@ def train_surrogate(sur, params, n_columns, chunk=262144, lr=3e-4, seed0=1000, n
         f["plude"].copy_(plude)  # the features read the detrained liquid as CLOUDSC receives it
         if done == 0:
             sur.fit_norm(f, params)
-        losses.append(sur.train(f, params, lr))
+        losses.append(sur.train(f, params, lr, batch=batch))
         done += chunk
         seed += 1
         if log:
