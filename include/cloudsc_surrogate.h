@@ -55,7 +55,8 @@ int cloudsc_mlp_gemm_tn(const uint16_t *A, int lda, const uint16_t *B, int ldb, 
  *
  * Features per column: 14 per level, feature-major (index f*KLEV + k), zero-padded to a multiple of 128:
  *   0 PT, 1 PQ, 2 PA, 3-6 log10(max(PCLV(l,i,r,s),0) + 1e-10), 7 PTENDENCY_CML_T, 8 PTENDENCY_CML_Q,
- *   9 PTENDENCY_CML_A, 10-11 PTENDENCY_CML_CLD(l,i), 12 PAP, 13 log10(max(PLUDE,0) + 1e-10).
+ *   9 PTENDENCY_CML_A, 10-11 PTENDENCY_CML_CLD(l,i), 12 PQ / qsat_mix(PT, PAP) (relative humidity over the
+ *   mixed phase, FOEEWM), 13 log10(max(PLUDE,0) + 1e-10).
  * Targets per column: 6 per level, target-major (index t*KLEV + k), zero-padded to a multiple of 128:
  *   0-2 PTENDENCY_LOC_T/Q/A, 3-4 PTENDENCY_LOC_CLD(l,i), 5 PFPLSL+PFPLSN at half level k+1.
  * Both are standardised per (feature or target, level) with the statistics held by the object.
@@ -71,10 +72,14 @@ int cloudsc_surrogate_destroy(cloudsc_surrogate_t *s);
 cloudsc_mlp_t *cloudsc_surrogate_mlp(cloudsc_surrogate_t *s);
 /* Padded widths: n_in = ceil128(14*KLEV), n_out = ceil128(6*KLEV). */
 int cloudsc_surrogate_dims(const cloudsc_surrogate_t *s, int *klev, int *n_in, int *n_out, int *max_batch);
-/* Normalisation statistics (host arrays): mean and standard deviation of the 14*KLEV raw features and the
- * 6*KLEV raw targets.  get = 1 copies the object's statistics out. */
-int cloudsc_surrogate_norm(cloudsc_surrogate_t *s, float *mu_x, float *sd_x, float *mu_y, float *sd_y, int get);
-/* Computes the statistics from a blocked field set holding plain-kernel inputs and outputs (all NGPTOT columns). */
+/* Normalisation statistics (host arrays): mean and scale of the 14*KLEV raw features and the 6*KLEV raw targets,
+ * and the training weight of each target entry (wgt_y may be NULL).  get = 1 copies the object's values out. */
+int cloudsc_surrogate_norm(cloudsc_surrogate_t *s, float *mu_x, float *sd_x, float *mu_y, float *sd_y, float *wgt_y,
+                           int get);
+/* Computes the statistics from a blocked field set holding plain-kernel inputs and outputs (all NGPTOT columns):
+ * means and standard deviations per entry; constant features are left unscaled and other features get at least
+ * 1e-3 of their largest level spread; target entries whose spread is below 1e-3 of their target's largest level
+ * spread get training weight 0 and that floor as scale.  Standardised values are clipped to +-10. */
 int cloudsc_surrogate_fit_norm(cloudsc_surrogate_t *s, CLOUDSC_BLOCKED_ARGS(float));
 /* Trains on all columns of a blocked field set holding plain-kernel inputs and outputs, one Adam step per
  * consecutive batch of `batch` columns (a multiple of 128, <= max_batch; a trailing partial batch is skipped).
@@ -82,8 +87,8 @@ int cloudsc_surrogate_fit_norm(cloudsc_surrogate_t *s, CLOUDSC_BLOCKED_ARGS(floa
 int cloudsc_surrogate_train(cloudsc_surrogate_t *s, float lr, int batch, float *loss, CLOUDSC_BLOCKED_ARGS(float));
 /* Network-only CLOUDSC: writes PTENDENCY_LOC_T/Q/A, PTENDENCY_LOC_CLD(l,i) and PFPLSL/PFPLSN/PFHPSL/PFHPSN
  * from the network; PFPLSL/PFPLSN split the predicted total by the phase of the level above (liquid where
- * T > RTT); PTENDENCY_LOC_CLD(r,s), PCOVPTOT, PRAINFRAC_TOPRFZ and the ten DDH flux profiles are set to zero;
- * PLUDE is left unchanged. */
+ * T > RTT); PTENDENCY_LOC_CLD(r,s), PRAINFRAC_TOPRFZ, the ten DDH flux profiles and PCOVPTOT at levels
+ * JK >= NCLDTOP (the levels CLOUDSC writes) are set to zero; PLUDE is left unchanged. */
 int cloudsc_surrogate_predict_sp(cloudsc_surrogate_t *s, CLOUDSC_BLOCKED_ARGS(float));
 /* Hybrid: cloudsc_surrogate_predict_sp on every column, then the plain kernel (exact outputs, every array
  * cloudsc_blocked_sp_c writes) on the columns whose Philox(rerun_seed, column) uniform is below rerun_fraction

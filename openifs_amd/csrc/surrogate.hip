@@ -34,12 +34,31 @@ namespace surr {
 
 constexpr int NF = CLOUDSC_SURR_NFEAT, NT = CLOUDSC_SURR_NTARG;
 constexpr int TILE = 64;  // columns x matrix entries per transpose tile
+// Standardised features and training targets are clipped to +-CLIP standard deviations: the heavy tails of the
+// tendencies (rare columns with values thousands of standard deviations out) would otherwise dominate the loss.
+constexpr float CLIP = 10.0f;
 
 inline int pad128(int n) { return (n + 127) / 128 * 128; }
 
+// Saturation constants (TTHF, fcttre.func.h) for the relative-humidity feature and the gate.
+struct Sat {
+  float r2es, r3les, r3ies, r4les, r4ies, rtt, rtice, rtwat_rtice_r, retv;
+};
+
 struct Dims {
   int ngptot, nproma, klev, ngpblks;
+  Sat th;
 };
+
+// Saturation specific humidity over the mixed phase, FOEEWM / p limited to 0.5 (fcttre.func.h, cloudsc.F90:1087).
+__device__ __forceinline__ float qsat_mix(const Sat& c, float t, float pap) {
+  const float x = (fmaxf(c.rtice, fminf(c.rtt, t)) - c.rtice) * c.rtwat_rtice_r;
+  const float alfa = fminf(1.0f, x * x);
+  const float es = c.r2es * (alfa * expf(c.r3les * (t - c.rtt) / (t - c.r4les)) +
+                             (1.0f - alfa) * expf(c.r3ies * (t - c.rtt) / (t - c.r4ies)));
+  const float e = fminf(es / pap, 0.5f);
+  return e / (1.0f - c.retv * e);
+}
 
 // Element (column c, level k) of a (NPROMA, L, n, NGPBLKS) field, species s.
 __device__ __forceinline__ size_t fidx(int c, int k, int s, int n, int L, int nproma) {
@@ -61,7 +80,10 @@ __device__ __forceinline__ float feature(const KPtrs<float>& p, const Dims& d, i
     case 8: return p.tcml_q[fidx(c, k, 0, 1, L, np)];
     case 9: return p.tcml_a[fidx(c, k, 0, 1, L, np)];
     case 10: case 11: return p.tcml_cld[fidx(c, k, f - 10, 5, L, np)];
-    case 12: return p.pap[fidx(c, k, 0, 1, L, np)];
+    case 12: {
+      const size_t i = fidx(c, k, 0, 1, L, np);
+      return p.pq[i] / qsat_mix(d.th, p.pt[i], p.pap[i]);  // relative humidity over the mixed phase
+    }
     default: return slog(p.plude[fidx(c, k, 0, 1, L, np)]);
   }
 }
@@ -92,7 +114,7 @@ __global__ __launch_bounds__(256) void features(KPtrs<float> p, Dims d, int col0
     float v = 0.0f;
     if (e < nfk && c < M && gc < d.ngptot) {
       const int f = e / d.klev, k = e - f * d.klev;
-      v = (feature(p, d, gc, f, k) - mu[e]) * rsd[e];
+      v = fminf(fmaxf((feature(p, d, gc, f, k) - mu[e]) * rsd[e], -CLIP), CLIP);
     }
     tile[i][tx] = v;
   }
@@ -121,7 +143,7 @@ __global__ __launch_bounds__(256) void targets(KPtrs<float> p, Dims d, int col0,
     float v = 0.0f;
     if (e < ntk && c < M && gc < d.ngptot) {
       const int t = e / d.klev, k = e - t * d.klev;
-      v = (target(p, d, gc, t, k) - mu[e]) * rsd[e];
+      v = fminf(fmaxf((target(p, d, gc, t, k) - mu[e]) * rsd[e], -CLIP), CLIP);
     }
     tile[i][tx] = v;
   }
@@ -135,7 +157,8 @@ __global__ __launch_bounds__(256) void targets(KPtrs<float> p, Dims d, int col0,
 // Network output Y [M x nout] (standardised) -> CLOUDSC outputs of columns col0 .. col0+M-1.
 __global__ __launch_bounds__(256) void outputs(KPtrs<float> p, Dims d, int col0, int M, int nout,
                                                const float* __restrict__ mu, const float* __restrict__ sd,
-                                               const float* __restrict__ Y, float rtt, float rlvtt, float rlstt) {
+                                               const float* __restrict__ Y, float rtt, float rlvtt, float rlstt,
+                                               int ncldtop) {
   __shared__ float tile[TILE][TILE + 1];
   const int c0 = blockIdx.x * TILE, e0 = blockIdx.y * TILE;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
@@ -167,7 +190,7 @@ __global__ __launch_bounds__(256) void outputs(KPtrs<float> p, Dims d, int col0,
         p.pfplsn[h] = fs;
         p.pfhpsl[h] = -rlvtt * fl;
         p.pfhpsn[h] = -rlstt * fs;
-        p.pcovptot[fidx(gc, k, 0, 1, L, np)] = 0.0f;
+        if (k >= ncldtop - 1) p.pcovptot[fidx(gc, k, 0, 1, L, np)] = 0.0f;  // the plain kernel's write set
         float* ddh[10] = {p.pfsqlf, p.pfsqif, p.pfcqnng, p.pfcqlng, p.pfsqrf,
                           p.pfsqsf, p.pfcqrng, p.pfcqsng, p.pfsqltur, p.pfsqitur};
         for (int j = 0; j < 10; ++j) ddh[j][h] = 0.0f;
@@ -209,11 +232,8 @@ __global__ __launch_bounds__(256) void moments(KPtrs<float> p, Dims d, int kind,
   if (threadIdx.x == 0) {
     const double m = r1[0] / d.ngptot;
     const double var = std::max(r2[0] / d.ngptot - m * m, 0.0);
-    double s_ = std::sqrt(var);
-    // constant entries (e.g. tendencies above NCLDTOP) are left unscaled
-    if (!(s_ > 1e-12 * std::max(std::fabs(m), 1e-30)) || s_ < 1e-37) s_ = 1.0;
     mu[e] = (float)m;
-    sd[e] = (float)s_;
+    sd[e] = (float)std::sqrt(var);  // raw; floors are applied on the host (cloudsc_surrogate_fit_norm)
   }
 }
 
@@ -224,6 +244,13 @@ __global__ void reciprocal(const float* __restrict__ a, float* __restrict__ r, i
 
 __global__ void add_scalar(float* __restrict__ acc, const float* __restrict__ x) {
   if (threadIdx.x == 0 && blockIdx.x == 0) acc[0] += x[0];
+}
+
+// PRAINFRAC_TOPRFZ is zeroed for all KLON columns of every block, the padding of the last block included
+// (cloudsc.F90:1010); the network path writes the active columns in `outputs`.
+__global__ void zero_padding_columns(float* __restrict__ prainfrac, int ngptot, int ntotal) {
+  const int c = ngptot + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ntotal) prainfrac[c] = 0.0f;
 }
 
 __global__ void fill_f32(float* p, float v) {
@@ -237,25 +264,16 @@ __global__ void fill_i32(int* p, int v) {
 // Gate score and rerun selection; selected columns are appended to list (order irrelevant: columns are
 // independent).  Score: max over levels with T < 238 K of q / qsat_mix (FOEEWM with FOEALFA, fcttre.func.h).
 __global__ __launch_bounds__(256) void select_columns(KPtrs<float> p, Dims d, float frac, uint64_t seed, float thr,
-                                                      float r2es, float r3les, float r3ies, float r4les, float r4ies,
-                                                      float rtt, float rtice, float rtwat_rtice_r, float retv,
                                                       float* __restrict__ score, int* __restrict__ list,
                                                       int* __restrict__ count) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= d.ngptot) return;
   float sc = 0.0f;
   for (int k = 0; k < d.klev; ++k) {
-    const float t = p.pt[fidx(c, k, 0, 1, d.klev, d.nproma)];
+    const size_t i = fidx(c, k, 0, 1, d.klev, d.nproma);
+    const float t = p.pt[i];
     if (t >= 238.0f) continue;
-    const float q = p.pq[fidx(c, k, 0, 1, d.klev, d.nproma)];
-    const float pap = p.pap[fidx(c, k, 0, 1, d.klev, d.nproma)];
-    const float x = (fmaxf(rtice, fminf(rtt, t)) - rtice) * rtwat_rtice_r;
-    const float alfa = fminf(1.0f, x * x);
-    const float es = r2es * (alfa * expf(r3les * (t - rtt) / (t - r4les)) +
-                             (1.0f - alfa) * expf(r3ies * (t - rtt) / (t - r4ies)));
-    const float e = fminf(es / pap, 0.5f);
-    const float qs = e / (1.0f - retv * e);
-    sc = fmaxf(sc, q / qs);
+    sc = fmaxf(sc, p.pq[i] / qsat_mix(d.th, t, p.pap[i]));
   }
   if (score) score[c] = sc;
   if (!list) return;
@@ -343,10 +361,20 @@ int check_fields(const cloudsc_surrogate* s, const cloudsc_params_t* q, int ngpt
 
 inline unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+surr::Sat sat_consts(const cloudsc_params_t* q) {
+  if (!q) return surr::Sat{};
+  return surr::Sat{(float)q->r2es, (float)q->r3les, (float)q->r3ies, (float)q->r4les, (float)q->r4ies,
+                   (float)q->rtt,  (float)q->rtice, (float)q->rtwat_rtice_r, (float)q->retv};
+}
+
 // Network prediction for all columns, in batches of max_batch.
 hipError_t predict(cloudsc_surrogate* s, const cloudsc_params_t* q, const KPtrs<float>& p, const surr::Dims& d,
                    hipStream_t st) {
   hipError_t e = hipSuccess;
+  const int ntotal = d.ngpblks * d.nproma;
+  if (ntotal > d.ngptot)
+    hipLaunchKernelGGL(surr::zero_padding_columns, dim3(nblk(ntotal - d.ngptot, 256)), dim3(256), 0, st,
+                       p.prainfrac_toprfz, d.ngptot, ntotal);
   for (int c0 = 0; c0 < d.ngptot && e == hipSuccess; c0 += s->max_batch) {
     const int n = std::min(s->max_batch, d.ngptot - c0);
     const int M = surr::pad128(n);
@@ -356,7 +384,7 @@ hipError_t predict(cloudsc_surrogate* s, const cloudsc_params_t* q, const KPtrs<
     if (e != hipSuccess) break;
     hipLaunchKernelGGL(surr::outputs, dim3(nblk(M, surr::TILE), nblk(surr::NT * d.klev, surr::TILE)), dim3(256), 0,
                        st, p, d, c0, n, s->nout, s->mu_y, s->sd_y, s->Y, (float)q->rtt, (float)q->rlvtt,
-                       (float)q->rlstt);
+                       (float)q->rlstt, q->ncldtop);
     e = hipGetLastError();
   }
   return e;
@@ -375,7 +403,7 @@ hipError_t predict(cloudsc_surrogate* s, const cloudsc_params_t* q, const KPtrs<
   p.pfcqnng = pfcqnng; p.pfcqlng = pfcqlng; p.pfsqrf = pfsqrf; p.pfsqsf = pfsqsf; p.pfcqrng = pfcqrng;             \
   p.pfcqsng = pfcqsng; p.pfsqltur = pfsqltur; p.pfsqitur = pfsqitur; p.pfplsl = pfplsl; p.pfplsn = pfplsn;         \
   p.pfhpsl = pfhpsl; p.pfhpsn = pfhpsn;                                                                           \
-  const surr::Dims d{ngptot, nproma, klev, ngpblks};                                                              \
+  const surr::Dims d{ngptot, nproma, klev, ngpblks, sat_consts(ydparams)};                                                              \
   hipStream_t st = (hipStream_t)stream;                                                                           \
   (void)ptsphy
 
@@ -444,7 +472,8 @@ int cloudsc_surrogate_dims(const cloudsc_surrogate_t* s, int* klev, int* n_in, i
   return CLOUDSC_OK;
 }
 
-int cloudsc_surrogate_norm(cloudsc_surrogate_t* s, float* mu_x, float* sd_x, float* mu_y, float* sd_y, int get) {
+int cloudsc_surrogate_norm(cloudsc_surrogate_t* s, float* mu_x, float* sd_x, float* mu_y, float* sd_y, float* wgt_y,
+                           int get) {
   if (!s || !mu_x || !sd_x || !mu_y || !sd_y) return fail(CLOUDSC_ENULL, "cloudsc_surrogate_norm: NULL argument");
   const size_t nx = (size_t)surr::NF * s->klev * 4, ny = (size_t)surr::NT * s->klev * 4;
   hipError_t e = hipDeviceSynchronize();
@@ -453,11 +482,13 @@ int cloudsc_surrogate_norm(cloudsc_surrogate_t* s, float* mu_x, float* sd_x, flo
     if (e == hipSuccess) e = hipMemcpy(sd_x, s->sd_x, nx, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(mu_y, s->mu_y, ny, hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(sd_y, s->sd_y, ny, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && wgt_y) e = hipMemcpy(wgt_y, s->wgt, ny, hipMemcpyDeviceToHost);
   } else {
     if (e == hipSuccess) e = hipMemcpy(s->mu_x, mu_x, nx, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->sd_x, sd_x, nx, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->mu_y, mu_y, ny, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->sd_y, sd_y, ny, hipMemcpyHostToDevice);
+    if (e == hipSuccess && wgt_y) e = hipMemcpy(s->wgt, wgt_y, ny, hipMemcpyHostToDevice);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(surr::reciprocal, dim3(nblk(s->nin, 256)), dim3(256), 0, 0, s->sd_x, s->rsd_x,
                          surr::NF * s->klev);
@@ -476,10 +507,45 @@ int cloudsc_surrogate_fit_norm(cloudsc_surrogate_t* s, CLOUDSC_BLOCKED_ARGS(floa
   if (rc) return rc;
   hipLaunchKernelGGL(surr::moments, dim3(surr::NF * klev), dim3(256), 0, st, p, d, 0, s->mu_x, s->sd_x);
   hipLaunchKernelGGL(surr::moments, dim3(surr::NT * klev), dim3(256), 0, st, p, d, 1, s->mu_y, s->sd_y);
-  hipLaunchKernelGGL(surr::reciprocal, dim3(nblk(s->nin, 256)), dim3(256), 0, st, s->sd_x, s->rsd_x, surr::NF * klev);
-  hipLaunchKernelGGL(surr::reciprocal, dim3(nblk(s->nout, 256)), dim3(256), 0, st, s->sd_y, s->rsd_y,
-                     surr::NT * klev);
+  const int nx = surr::NF * klev, ny = surr::NT * klev;
+  std::vector<float> mx(nx), sx(nx), my(ny), sy(ny), w(s->nout, 0.0f);
   hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(mx.data(), s->mu_x, nx * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(sx.data(), s->sd_x, nx * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(my.data(), s->mu_y, ny * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(sy.data(), s->sd_y, ny * 4, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_surrogate_fit_norm", e);
+  // Features: constant entries (the model-top pressure, levels without a quantity) are not scaled; the others
+  // get at least 1e-3 of their feature's largest level spread.
+  for (int f = 0; f < surr::NF; ++f) {
+    float top = 0.0f;
+    for (int k = 0; k < klev; ++k) top = std::max(top, sx[f * klev + k]);
+    for (int k = 0; k < klev; ++k) {
+      float& v = sx[f * klev + k];
+      v = (v <= 1e-6f * std::fabs(mx[f * klev + k]) || v <= 1e-30f) ? 1.0f : std::max(v, 1e-3f * top);
+    }
+  }
+  // Targets: entries whose spread is below 1e-3 of their target's largest level spread (tendencies above
+  // NCLDTOP, levels where a process never acts) are not trained (weight 0) and are predicted at their mean.
+  for (int t = 0; t < surr::NT; ++t) {
+    float top = 0.0f;
+    for (int k = 0; k < klev; ++k) top = std::max(top, sy[t * klev + k]);
+    for (int k = 0; k < klev; ++k) {
+      float& v = sy[t * klev + k];
+      const bool live = top > 0.0f && v > 1e-3f * top;
+      w[t * klev + k] = live ? 1.0f : 0.0f;
+      v = live ? v : (top > 0.0f ? 1e-3f * top : 1.0f);
+    }
+  }
+  e = hipMemcpyAsync(s->sd_x, sx.data(), nx * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->sd_y, sy.data(), ny * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->wgt, w.data(), s->nout * 4, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(surr::reciprocal, dim3(nblk(s->nin, 256)), dim3(256), 0, st, s->sd_x, s->rsd_x, nx);
+    hipLaunchKernelGGL(surr::reciprocal, dim3(nblk(s->nout, 256)), dim3(256), 0, st, s->sd_y, s->rsd_y, ny);
+    e = hipGetLastError();
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_surrogate_fit_norm", e);
   return CLOUDSC_OK;
@@ -529,10 +595,8 @@ int cloudsc_surrogate_gate_scores(cloudsc_surrogate_t* s, float* score, CLOUDSC_
   int rc = check_fields(s, ydparams, ngptot, nproma, klev, ngpblks, p);
   if (rc) return rc;
   if (!score) return fail(CLOUDSC_ENULL, "cloudsc_surrogate_gate_scores: NULL score");
-  const cloudsc_params_t* q = ydparams;
   hipLaunchKernelGGL(surr::select_columns, dim3(nblk(ngptot, 256)), dim3(256), 0, st, p, d, 0.0f, (uint64_t)0, 0.0f,
-                     (float)q->r2es, (float)q->r3les, (float)q->r3ies, (float)q->r4les, (float)q->r4ies, (float)q->rtt,
-                     (float)q->rtice, (float)q->rtwat_rtice_r, (float)q->retv, score, (int*)nullptr, (int*)nullptr);
+                     score, (int*)nullptr, (int*)nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_surrogate_gate_scores", e);
   return CLOUDSC_OK;
@@ -561,9 +625,7 @@ int cloudsc_hybrid_sp_c(cloudsc_surrogate_t* s, float rerun_fraction, uint64_t r
   }
   hipLaunchKernelGGL(surr::fill_i32, dim3(1), dim3(64), 0, st, s->count, 0);
   hipLaunchKernelGGL(surr::select_columns, dim3(nblk(ngptot, 256)), dim3(256), 0, st, p, d, rerun_fraction,
-                     rerun_seed, gate_threshold, (float)q->r2es, (float)q->r3les, (float)q->r3ies, (float)q->r4les,
-                     (float)q->r4ies, (float)q->rtt, (float)q->rtice, (float)q->rtwat_rtice_r, (float)q->retv,
-                     (float*)nullptr, s->list, s->count);
+                     rerun_seed, gate_threshold, (float*)nullptr, s->list, s->count);
   int nsel = 0;
   e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(&nsel, s->count, sizeof(int), hipMemcpyDeviceToHost, st);
@@ -593,7 +655,7 @@ int cloudsc_hybrid_sp_c(cloudsc_surrogate_t* s, float rerun_fraction, uint64_t r
                 {(void*)p.pludeli, 4, (int)L, true, false, 0},   {(void*)p.psnde, 2, (int)L, true, false, 0},
                 {(void*)p.pmfu, 1, (int)L, true, false, 0},      {(void*)p.pmfd, 1, (int)L, true, false, 0},
                 {(void*)p.pa, 1, (int)L, true, false, 0},        {(void*)p.pclv, 5, (int)L, true, false, 0},
-                {(void*)p.pcovptot, 1, (int)L, false, true, 1},  {(void*)p.prainfrac_toprfz, 1, 1, false, true, 1},
+                {(void*)p.pcovptot, 1, (int)L, true, true, 1},  {(void*)p.prainfrac_toprfz, 1, 1, false, true, 1},
                 {(void*)p.pfsqlf, 1, (int)L + 1, false, true, 1}, {(void*)p.pfsqif, 1, (int)L + 1, false, true, 1},
                 {(void*)p.pfcqnng, 1, (int)L + 1, false, true, 1}, {(void*)p.pfcqlng, 1, (int)L + 1, false, true, 1},
                 {(void*)p.pfsqrf, 1, (int)L + 1, false, true, 1}, {(void*)p.pfsqsf, 1, (int)L + 1, false, true, 1},

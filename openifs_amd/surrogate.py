@@ -16,7 +16,7 @@ from . import cloudsc as C
 NFEAT, NTARG = 14, 6
 FEATURES = ["PT", "PQ", "PA", "log10 PCLV(l)", "log10 PCLV(i)", "log10 PCLV(r)", "log10 PCLV(s)",
             "PTENDENCY_CML_T", "PTENDENCY_CML_Q", "PTENDENCY_CML_A", "PTENDENCY_CML_CLD(l)", "PTENDENCY_CML_CLD(i)",
-            "PAP", "log10 PLUDE"]
+            "PQ/qsat_mix(PT,PAP)", "log10 PLUDE"]
 TARGETS = ["PTENDENCY_LOC_T", "PTENDENCY_LOC_Q", "PTENDENCY_LOC_A", "PTENDENCY_LOC_CLD(l)", "PTENDENCY_LOC_CLD(i)",
            "PFPLSL+PFPLSN (lower half level)"]
 
@@ -43,7 +43,7 @@ def _lib():
         L.cloudsc_surrogate_mlp.argtypes = [_vp]
         L.cloudsc_surrogate_mlp.restype = _vp
         L.cloudsc_surrogate_dims.argtypes = [_vp, _vp, _vp, _vp, _vp]
-        L.cloudsc_surrogate_norm.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]
+        L.cloudsc_surrogate_norm.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
         L._surrogate_typed = True
     return L
 
@@ -170,6 +170,7 @@ class Surrogate:
               ctypes.byref(mb))
         self.klev, self.n_in, self.n_out, self.max_batch = k.value, nin.value, nout.value, mb.value
         self.hidden, self.hidden_layers = hidden, hidden_layers
+        self.fitted = False
         self.mlp = MLP(0, 0, 0, 0, mb.value, handle=_lib().cloudsc_surrogate_mlp(self.h))
 
     def close(self):
@@ -186,19 +187,24 @@ class Surrogate:
 
     # -- normalisation ------------------------------------------------------------------------------------------
     def norm(self, value=None):
+        """Normalisation statistics and target weights (dict of numpy arrays); with ``value``, sets them."""
         nx, ny = NFEAT * self.klev, NTARG * self.klev
         if value is None:
             mx, sx = np.empty(nx, np.float32), np.empty(nx, np.float32)
-            my, sy = np.empty(ny, np.float32), np.empty(ny, np.float32)
-            _call("cloudsc_surrogate_norm", self.h, mx.ctypes.data, sx.ctypes.data, my.ctypes.data, sy.ctypes.data, 1)
-            return {"mu_x": mx, "sd_x": sx, "mu_y": my, "sd_y": sy}
+            my, sy, wy = np.empty(ny, np.float32), np.empty(ny, np.float32), np.empty(ny, np.float32)
+            _call("cloudsc_surrogate_norm", self.h, mx.ctypes.data, sx.ctypes.data, my.ctypes.data, sy.ctypes.data,
+                  wy.ctypes.data, 1)
+            return {"mu_x": mx, "sd_x": sx, "mu_y": my, "sd_y": sy, "wgt_y": wy}
         v = {k: np.ascontiguousarray(value[k], dtype=np.float32) for k in ("mu_x", "sd_x", "mu_y", "sd_y")}
+        w = np.ascontiguousarray(value["wgt_y"], dtype=np.float32) if "wgt_y" in value else None
         _call("cloudsc_surrogate_norm", self.h, v["mu_x"].ctypes.data, v["sd_x"].ctypes.data, v["mu_y"].ctypes.data,
-              v["sd_y"].ctypes.data, 0)
+              v["sd_y"].ctypes.data, None if w is None else w.ctypes.data, 0)
+        self.fitted = True
 
     def fit_norm(self, fields, params, ptsphy=900.0):
         """Statistics from a blocked field set that holds plain-kernel inputs and outputs."""
         _blocked_call("cloudsc_surrogate_fit_norm", fields, params, ptsphy, ctypes.c_void_p(self.h))
+        self.fitted = True
 
     # -- training and inference ---------------------------------------------------------------------------------
     def train(self, fields, params, lr, batch=4096, ptsphy=900.0):
@@ -242,6 +248,7 @@ class Surrogate:
             self.mlp.weight(l, st[f"w{l}"])
             self.mlp.bias(l, st[f"b{l}"])
         self.norm(st)
+        self.fitted = True
 
     def save(self, path):
         np.savez(path, **self.state())
@@ -271,7 +278,7 @@ def train_surrogate(sur, params, n_columns, chunk=262144, lr=3e-4, seed0=1000, n
         plude.copy_(f["plude"])
         C.cloudsc_blocked(f, params, 900.0)
         f["plude"].copy_(plude)  # the features read the detrained liquid as CLOUDSC receives it
-        if done == 0:
+        if done == 0 and not sur.fitted:
             sur.fit_norm(f, params)
         losses.append(sur.train(f, params, lr, batch=batch))
         done += chunk
