@@ -155,9 +155,10 @@ def lib():
         _lib.cloudsc_params_default.argtypes = [ctypes.POINTER(CloudscParams), ctypes.c_double, ctypes.c_int,
                                                 ctypes.c_void_p]
         # work-group sizes recorded by the autotuner (openifs_amd/tuning/cloudsc_tuned.json), if any
-        from . import autotune
+        if hasattr(_lib, "cloudsc_tune_set_block"):
+            from . import autotune
 
-        autotune.apply(arch="gfx950")
+            autotune.apply(arch="gfx950")
     return _lib
 
 

@@ -111,12 +111,13 @@ def test_per_block_kidia_window_fp64(params137):
     assert np.all(a_gpu["prainfrac_toprfz"][outside] == 0.0)
 
 
-@pytest.mark.parametrize("prec", ["sp", "mixed"])
-def test_fp32_as_accurate_as_fp32_oracle(prec, params137):
-    ngptot, nproma, klev = 8192, 64, 137
-    inp, out = gpu_run(params137, prec, ngptot, nproma, klev, 51)
-    ref64 = oracle_blocked("dp", params137, inp, ngptot, nproma, klev, nthreads=16)
-    ref32 = oracle_blocked(prec, params137, inp, ngptot, nproma, klev, nthreads=16)
+@pytest.mark.parametrize("prec,klev", [("sp", 137), ("mixed", 137), ("sp", 91), ("mixed", 91)])
+def test_fp32_as_accurate_as_fp32_oracle(prec, klev, params137, params91):
+    params = params137 if klev == 137 else params91
+    ngptot, nproma = 8192, 64
+    inp, out = gpu_run(params, prec, ngptot, nproma, klev, 51)
+    ref64 = oracle_blocked("dp", params, inp, ngptot, nproma, klev, nthreads=16)
+    ref32 = oracle_blocked(prec, params, inp, ngptot, nproma, klev, nthreads=16)
     report = []
     for n in FIELDS:
         g, r64, r32 = (columns(x[n], nproma) for x in (out, ref64, ref32))
@@ -152,17 +153,22 @@ def _budgets_torch(params, f, k0):
     return (wl - wr).abs() / ws, (hl - hr).abs() / hs
 
 
-@pytest.mark.parametrize("prec", ["sp", "mixed"])
-def test_full_size_fp32_properties(prec, params137):
-    """655,360 columns (TCo399-sized), the benchmark workload: every written output is
-    finite, the write set is the reference's, and the column budgets close to FP32 rounding."""
-    ngptot, nproma, klev = 655360, 128, 137
+@pytest.mark.parametrize("prec,ngptot,klev", [("sp", 655360, 137), ("mixed", 655360, 137),
+                                               ("mixed", 2621440, 137), ("mixed", 2621440, 91)])
+def test_full_size_fp32_properties(prec, ngptot, klev, params137, params91):
+    """655,360 columns (TCo399-sized, the benchmark workload) and 2,621,440 columns at L137 and L91 with the
+    autotuned NPROMA and work-group size (the mixed-precision workload): every written output is finite, the
+    write set is the reference's, and the column budgets close to FP32 rounding."""
+    from openifs_amd import autotune
+
+    params = params137 if klev == 137 else params91
+    nproma = 128 if ngptot == 655360 else autotune.recommended_nproma(prec, klev)
     f = C.BlockedFields(ngptot, nproma, klev, prec, "cuda", fill=float("nan"))
-    C.synth_device(f, params137, 61)
+    C.synth_device(f, params, 61)
     plude_in = f["plude"].clone()
-    C.cloudsc_blocked(f, params137, 900.0)
+    C.cloudsc_blocked(f, params, 900.0)
     torch.cuda.synchronize()
-    k0 = params137.ncldtop - 1
+    k0 = params.ncldtop - 1
     for n in C.OUTPUT_NAMES:
         x = f[n]
         if n == "ptendency_loc_cld":
@@ -172,8 +178,8 @@ def test_full_size_fp32_properties(prec, params137):
         else:
             assert torch.isfinite(x).all(), n
     assert torch.equal(f["plude"][:, :k0], plude_in[:, :k0]) and torch.equal(f["plude"][:, -1], plude_in[:, -1])
-    assert torch.equal(f["pfhpsl"], -params137.rlvtt * f["pfplsl"]) or \
-        torch.allclose(f["pfhpsl"], -params137.rlvtt * f["pfplsl"], rtol=1e-6, atol=0)
-    ew, eh = _budgets_torch(params137, f, k0)
+    assert torch.equal(f["pfhpsl"], -params.rlvtt * f["pfplsl"]) or \
+        torch.allclose(f["pfhpsl"], -params.rlvtt * f["pfplsl"], rtol=1e-6, atol=0)
+    ew, eh = _budgets_torch(params, f, k0)
     assert float(ew.median()) < 1e-5 and float(eh.median()) < 1e-5
     assert float(ew.max()) < 1e-2 and float(eh.max()) < 1e-2
