@@ -112,51 +112,57 @@ __device__ __forceinline__ T trunc_scale(T x, int jo, int a, int b, T zz) {
   return hit ? x * zz : x;
 }
 
-// Loads and stores at a 32-bit byte offset from an array base: the form the hardware
+// Loads and stores at 32-bit byte offsets from an array base: the form the hardware
 // addresses directly (64-bit base in SGPRs, 32-bit offset in one VGPR).  The launcher
 // splits the grid so that every array of one launch spans less than 4 GiB.
+// u: byte offset common to the wave (block and level); v: the lane's byte offset within the level.  The sum is
+// formed in 32 bits, so every array access is one vector offset on a scalar base; with wave-uniform blocks
+// (NPROMA a multiple of 64) u is a scalar and the block-dependent offsets cost no vector registers.
 template <typename T>
-__device__ __forceinline__ T LD(const T* base, uint32_t off) {
-  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off);
+__device__ __forceinline__ T LD(const T* base, uint32_t u, uint32_t v) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (u + v));
 }
 template <typename T>
-__device__ __forceinline__ void ST(T* base, uint32_t off, T v) {
-  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off) = v;
+__device__ __forceinline__ void ST(T* base, uint32_t u, uint32_t v, T x) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (u + v)) = x;
 }
 
 // KC / KCD: constants of the working precision and of the FP64 islands, either the run-time
 // structure KConst or the compile-time default configuration KConstDefault (same member names).
-template <typename R, typename RD, typename KC, typename KCD, int BLOCK>
+// UNI: every wave lies in one block (NPROMA a multiple of 64), so block-dependent offsets are wave-uniform.
+template <typename R, typename RD, typename KC, typename KCD, int BLOCK, bool UNI>
 __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p,
                                                                 const KDims d, int ssc_off) {
   const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
   const uint32_t nproma = (uint32_t)d.nproma, klev = (uint32_t)d.klev;
   if (g >= (uint32_t)d.ngpblks * nproma) return;
-  const uint32_t b = g / nproma;
+  uint32_t b = g / nproma;
+  if (UNI) b = __builtin_amdgcn_readfirstlane(b);
   const uint32_t jl = g - b * nproma;  // 0-based JL-1
   constexpr uint32_t E = sizeof(R);
-  const uint32_t o1 = (b * nproma + jl) * E, o1i = (b * nproma + jl) * 4u;
+  const uint32_t jo = jl * E, joi = jl * 4u;           // lane offsets (R and int32 arrays)
+  const uint32_t u1 = b * nproma * E, u1i = b * nproma * 4u;
   // PRAINFRAC_TOPRFZ is zeroed for all KLON columns of the block (:1010)
-  ST(p.prainfrac_toprfz, o1, R(0));
+  ST(p.prainfrac_toprfz, u1, jo, R(0));
   const int kfdia = min(d.kfdia, d.ngptot - (int)(b * nproma));
   if ((int)jl < d.kidia - 1 || (int)jl >= kfdia) return;
 
   const uint32_t s2 = nproma * klev;
   const uint32_t lstep = nproma * E;                  // one level
   const uint32_t sstep = s2 * E;                      // one species of a (NPROMA,KLEV,n) array
-  const uint32_t b2 = (b * s2 + jl) * E;              // (NPROMA,KLEV) level 0
-  const uint32_t bh = (b * (s2 + nproma) + jl) * E;   // (NPROMA,KLEV+1) level 0
-  const uint32_t b5 = (b * s2 * 5 + jl) * E, b4 = (b * s2 * 4 + jl) * E, b2x = (b * s2 * 2 + jl) * E;
+  const uint32_t b2 = b * s2 * E;                     // (NPROMA,KLEV) level 0
+  const uint32_t bh = b * (s2 + nproma) * E;          // (NPROMA,KLEV+1) level 0
+  const uint32_t b5 = b * s2 * 5 * E, b4 = b * s2 * 4 * E, b2x = b * s2 * 2 * E;
 
   const R ptsphy = c.ptsphy, zqtmst = c.zqtmst, zepsec = R(1.e-14);
   const R rlmin = c.rlmin;
 
   // Per-column inputs
-  const R plsm = LD(p.plsm, o1);
-  const int32_t ktype = LD(p.ktype, o1i);
-  const bool ldcum = LD(p.ldcum, o1i) != 0;
-  const R peis = LD(p.peis, o1);
-  const R paph_surf = LD(p.paph, bh + klev * lstep);
+  const R plsm = LD(p.plsm, u1, jo);
+  const int32_t ktype = LD(p.ktype, u1i, joi);
+  const bool ldcum = LD(p.ldcum, u1i, joi) != 0;
+  const R peis = LD(p.peis, u1, jo);
+  const R paph_surf = LD(p.paph, bh + klev * lstep, jo);
   const R zadvw = (ktype == 1 && c.rmfadvw > R(0)) ? R(1) - c.rmfadvw : R(1);                  // :696-703
   const R zadvwd = (ktype == 1 && c.rmfadvw > R(0)) ? R(1) - c.rmfadvw * c.rmfadvwdd : R(1);
   const R zconst = plsm > R(0.5) ? c.rcl_kk_cloud_num_land : c.rcl_kk_cloud_num_sea;          // :2063-2069
@@ -172,47 +178,47 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
   R za_prev = R(0), qr_prev = R(0), qs_prev = R(0), t_prev = R(0), pap_prev = R(0);
   // DDH flux running sums at the upper half level (:4231-4242)
   R sqlf = R(0), sqif = R(0), cqlng = R(0), cqnng = R(0), sqltur = R(0), sqitur = R(0);
-  ST(p.pfsqlf, bh, R(0));
-  ST(p.pfsqif, bh, R(0));
-  ST(p.pfsqrf, bh, R(0));
-  ST(p.pfsqsf, bh, R(0));
-  ST(p.pfcqlng, bh, R(0));
-  ST(p.pfcqnng, bh, R(0));
-  ST(p.pfcqrng, bh, R(0));
-  ST(p.pfcqsng, bh, R(0));
-  ST(p.pfsqltur, bh, R(0));
-  ST(p.pfsqitur, bh, R(0));
+  ST(p.pfsqlf, bh, jo, R(0));
+  ST(p.pfsqif, bh, jo, R(0));
+  ST(p.pfsqrf, bh, jo, R(0));
+  ST(p.pfsqsf, bh, jo, R(0));
+  ST(p.pfcqlng, bh, jo, R(0));
+  ST(p.pfcqnng, bh, jo, R(0));
+  ST(p.pfcqrng, bh, jo, R(0));
+  ST(p.pfcqsng, bh, jo, R(0));
+  ST(p.pfsqltur, bh, jo, R(0));
+  ST(p.pfsqitur, bh, jo, R(0));
 
-  R paph_k = LD(p.paph, bh);
-  R pmfu_k = LD(p.pmfu, b2), pmfd_k = LD(p.pmfd, b2);
+  R paph_k = LD(p.paph, bh, jo);
+  R pmfu_k = LD(p.pmfu, b2, jo), pmfd_k = LD(p.pmfd, b2, jo);
 
   for (uint32_t k = 0; k < klev; ++k) {
     const uint32_t i2 = b2 + k * lstep;        // level k of a (NPROMA,KLEV) array
     const uint32_t ih0 = bh + k * lstep;       // upper half level of layer k
     const uint32_t ih1 = ih0 + lstep;          // lower half level
     const uint32_t i5 = b5 + k * lstep;
-    const uint32_t i4 = b4 + k * lstep;
+    const uint32_t i4 = b4 + k * lstep;        // (all wave-uniform with UNI; the lane adds jo)
     const bool has_next = k + 1 < klev;
-    const R pmfu_n = has_next ? LD(p.pmfu, i2 + lstep) : R(0);
-    const R pmfd_n = has_next ? LD(p.pmfd, i2 + lstep) : R(0);
-    const R plu_n = has_next ? LD(p.plu, i2 + lstep) : R(0);
-    const R paph_k1 = LD(p.paph, ih1);
-    const R pap = LD(p.pap, i2);
-    const R pt = LD(p.pt, i2), pq = LD(p.pq, i2), pa = LD(p.pa, i2);
-    const R tcml_t = LD(p.tcml_t, i2), tcml_q = LD(p.tcml_q, i2), tcml_a = LD(p.tcml_a, i2);
-    const R pvfl = LD(p.pvfl, i2), pvfi = LD(p.pvfi, i2);
-    const R pludeli1 = LD(p.pludeli, i4);
-    const R pludeli2 = LD(p.pludeli, i4 + sstep);
+    const R pmfu_n = has_next ? LD(p.pmfu, i2 + lstep, jo) : R(0);
+    const R pmfd_n = has_next ? LD(p.pmfd, i2 + lstep, jo) : R(0);
+    const R plu_n = has_next ? LD(p.plu, i2 + lstep, jo) : R(0);
+    const R paph_k1 = LD(p.paph, ih1, jo);
+    const R pap = LD(p.pap, i2, jo);
+    const R pt = LD(p.pt, i2, jo), pq = LD(p.pq, i2, jo), pa = LD(p.pa, i2, jo);
+    const R tcml_t = LD(p.tcml_t, i2, jo), tcml_q = LD(p.tcml_q, i2, jo), tcml_a = LD(p.tcml_a, i2, jo);
+    const R pvfl = LD(p.pvfl, i2, jo), pvfi = LD(p.pvfi, i2, jo);
+    const R pludeli1 = LD(p.pludeli, i4, jo);
+    const R pludeli2 = LD(p.pludeli, i4 + sstep, jo);
 
     // ---- 1. initial values (:982-1002) ----
     const R ztp1 = pt + ptsphy * tcml_t;
     R qv = pq + ptsphy * tcml_q;
     R za = pa + ptsphy * tcml_a;
     const R zaorig = za;
-    const R ql0 = LD(p.pclv, i5) + ptsphy * LD(p.tcml_cld, i5);
-    const R qi0 = LD(p.pclv, i5 + sstep) + ptsphy * LD(p.tcml_cld, i5 + sstep);
-    const R qr0 = LD(p.pclv, i5 + 2 * sstep) + ptsphy * LD(p.tcml_cld, i5 + 2 * sstep);
-    const R qs0 = LD(p.pclv, i5 + 3 * sstep) + ptsphy * LD(p.tcml_cld, i5 + 3 * sstep);
+    const R ql0 = LD(p.pclv, i5, jo) + ptsphy * LD(p.tcml_cld, i5, jo);
+    const R qi0 = LD(p.pclv, i5 + sstep, jo) + ptsphy * LD(p.tcml_cld, i5 + sstep, jo);
+    const R qr0 = LD(p.pclv, i5 + 2 * sstep, jo) + ptsphy * LD(p.tcml_cld, i5 + 2 * sstep, jo);
+    const R qs0 = LD(p.pclv, i5 + 3 * sstep, jo) + ptsphy * LD(p.tcml_cld, i5 + 3 * sstep, jo);
     R ql = ql0, qi = qi0, qr = qr0, qs = qs0;
 
     // saturation values (:1087-1115)
@@ -272,15 +278,15 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
     // PFPLSL/PFPLSN/PFHPSL/PFHPSN at the upper half level (:4218-4224,4297-4302)
     {
       const R fl = pf_r + R(0), fn = pf_s + pf_i;
-      ST(p.pfplsl, ih0, fl);
-      ST(p.pfplsn, ih0, fn);
-      ST(p.pfhpsl, ih0, -c.rlvtt * fl);
-      ST(p.pfhpsn, ih0, -c.rlstt * fn);
+      ST(p.pfplsl, ih0, jo, fl);
+      ST(p.pfplsn, ih0, jo, fn);
+      ST(p.pfhpsl, ih0, jo, -c.rlvtt * fl);
+      ST(p.pfhpsn, ih0, jo, -c.rlstt * fn);
     }
 
     if ((int)k >= ncldtop0) {
       const bool above_top = (int)k > ncldtop0;  // JK > NCLDTOP
-      const R plude_in = LD(p.plude, i2);
+      const R plude_in = LD(p.plude, i2, jo);
       // derived variables (:1281-1324)
       const R zdp = paph_k1 - paph_k;
       const R zgdp = m_div(c.rg, zdp);
@@ -335,12 +341,12 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           plude = R(0);
         }
         if (c.lmfdsnow) {
-          csrc_r = LD(p.psnde, b2x + k * lstep) * zdtgdp;
-          csrc_s = LD(p.psnde, b2x + k * lstep + sstep) * zdtgdp;
+          csrc_r = LD(p.psnde, b2x + k * lstep, jo) * zdtgdp;
+          csrc_s = LD(p.psnde, b2x + k * lstep + sstep, jo) * zdtgdp;
           dr = dr + csrc_r;
           ds = ds + csrc_s;
         }
-        ST(p.plude, i2, plude);
+        ST(p.plude, i2, jo, plude);
       }
 
       // 3.3 subsidence from the level above (:1419-1492)
@@ -806,7 +812,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
         const RD zq_upd = RD(pq) + RD(ptsphy) * (RD(tq) + RD(tcml_q));
         const RD zt_upd = RD(pt) + RD(ptsphy) * (RD(tt) + RD(tcml_t));
         const RD za_upd = RD(pa) + RD(ptsphy) * (RD(ta) + RD(tcml_a));
-        const R pp = ssc_off ? LD(p.pap, i2 + (uint32_t)ssc_off * E) : pap;
+        const R pp = ssc_off ? LD(p.pap, i2 + (uint32_t)ssc_off * E, jo) : pap;
         RD t_adj, q_adj, a_adj, l_adj, i_adj;
         supersatcheck<RD, KCD>(cd, zt_upd, zq_upd, za_upd, RD(pp), t_adj, q_adj, a_adj, l_adj, i_adj);
         tq = tq + R(q_adj) * zqtmst;
@@ -830,17 +836,17 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           tt = tt - c.ralsdcp * zi * zqtmst;
         }
       }
-      ST(p.pcovptot, i2, zcovptot);  // (:3814-3816)
+      ST(p.pcovptot, i2, jo, zcovptot);  // (:3814-3816)
     }
 
     // store tendencies of this level (levels above NCLDTOP keep the tidy-up terms only)
-    ST(p.tloc_t, i2, tt);
-    ST(p.tloc_q, i2, tq);
-    ST(p.tloc_a, i2, ta);
-    ST(p.tloc_cld, i5, tcld_l);
-    ST(p.tloc_cld, i5 + sstep, tcld_i);
-    ST(p.tloc_cld, i5 + 2 * sstep, tcld_r);
-    ST(p.tloc_cld, i5 + 3 * sstep, tcld_s);
+    ST(p.tloc_t, i2, jo, tt);
+    ST(p.tloc_q, i2, jo, tq);
+    ST(p.tloc_a, i2, jo, ta);
+    ST(p.tloc_cld, i5, jo, tcld_l);
+    ST(p.tloc_cld, i5 + sstep, jo, tcld_i);
+    ST(p.tloc_cld, i5 + 2 * sstep, jo, tcld_r);
+    ST(p.tloc_cld, i5 + 3 * sstep, jo, tcld_s);
 
     // DDH fluxes at the lower half level (:4244-4291)
     {
@@ -855,16 +861,16 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       sqif = sqif + (qxn_i - qi0 + pvfi * ptsphy) * zgdph_r + pludeli2;
       cqnng = cqnng + zlneg_i * zgdph_r;
       sqitur = sqitur + pvfi * ptsphy * zgdph_r;
-      ST(p.pfsqlf, ih1, sqlf);
-      ST(p.pfsqif, ih1, sqif);
-      ST(p.pfsqrf, ih1, sqrf);
-      ST(p.pfsqsf, ih1, sqsf);
-      ST(p.pfcqlng, ih1, cqlng);
-      ST(p.pfcqnng, ih1, cqnng);
-      ST(p.pfcqrng, ih1, cqrng);
-      ST(p.pfcqsng, ih1, cqsng);
-      ST(p.pfsqltur, ih1, sqltur);
-      ST(p.pfsqitur, ih1, sqitur);
+      ST(p.pfsqlf, ih1, jo, sqlf);
+      ST(p.pfsqif, ih1, jo, sqif);
+      ST(p.pfsqrf, ih1, jo, sqrf);
+      ST(p.pfsqsf, ih1, jo, sqsf);
+      ST(p.pfcqlng, ih1, jo, cqlng);
+      ST(p.pfcqnng, ih1, jo, cqnng);
+      ST(p.pfcqrng, ih1, jo, cqrng);
+      ST(p.pfcqsng, ih1, jo, cqsng);
+      ST(p.pfsqltur, ih1, jo, sqltur);
+      ST(p.pfsqitur, ih1, jo, sqitur);
     }
 
     // carry to the next level
@@ -881,12 +887,12 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
   {
     const uint32_t ih = bh + klev * lstep;
     const R fl = pf_r + R(0), fn = pf_s + pf_i;
-    ST(p.pfplsl, ih, fl);
-    ST(p.pfplsn, ih, fn);
-    ST(p.pfhpsl, ih, -c.rlvtt * fl);
-    ST(p.pfhpsn, ih, -c.rlstt * fn);
+    ST(p.pfplsl, ih, jo, fl);
+    ST(p.pfplsn, ih, jo, fn);
+    ST(p.pfhpsl, ih, jo, -c.rlvtt * fl);
+    ST(p.pfhpsn, ih, jo, -c.rlstt * fn);
   }
-  ST(p.prainfrac_toprfz, o1, prainfrac);
+  ST(p.prainfrac_toprfz, u1, jo, prainfrac);
 }
 
 // ---------------------------------------------------------------------------
@@ -1025,24 +1031,29 @@ void set_tuned_block(int variant, int klev, int block) {
 }
 
 template <typename R, typename RD, typename KC, typename KCD>
-hipError_t launch_variant(int block, unsigned ncols, const KC& c, const KCD& cd, const KPtrs<R>& p, const KDims& d,
-                          int ssc_off, hipStream_t stream) {
+hipError_t launch_variant(int block, bool uni, unsigned ncols, const KC& c, const KCD& cd, const KPtrs<R>& p,
+                          const KDims& d, int ssc_off, hipStream_t stream) {
+  if (!uni) {  // blocks not aligned to waves: per-lane block offsets, 256-lane groups
+    hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, false>), dim3((ncols + 255) / 256), dim3(256), 0,
+                       stream, c, cd, p, d, ssc_off);
+    return hipGetLastError();
+  }
   const unsigned grid = (ncols + block - 1) / block;
   switch (block) {
     case 64:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 64>), dim3(grid), dim3(64), 0, stream, c, cd, p, d,
-                         ssc_off);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 64, true>), dim3(grid), dim3(64), 0, stream, c, cd, p,
+                         d, ssc_off);
       break;
     case 128:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 128>), dim3(grid), dim3(128), 0, stream, c, cd, p,
-                         d, ssc_off);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 128, true>), dim3(grid), dim3(128), 0, stream, c, cd,
+                         p, d, ssc_off);
       break;
     case 512:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 512>), dim3(grid), dim3(512), 0, stream, c, cd, p,
-                         d, ssc_off);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 512, true>), dim3(grid), dim3(512), 0, stream, c, cd,
+                         p, d, ssc_off);
       break;
     default:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256>), dim3((ncols + 255) / 256), dim3(256), 0,
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, true>), dim3((ncols + 255) / 256), dim3(256), 0,
                          stream, c, cd, p, d, ssc_off);
   }
   return hipGetLastError();
@@ -1053,6 +1064,8 @@ hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<
                           hipStream_t stream) {
   const bool def = is_default_config(q);
   const int block = def ? tuned_block(variant_id<R, RD>(), d.klev) : 256;
+  // every wave inside one block (64-lane waves, 64 | NPROMA): block offsets are uniform across the wave
+  const bool uni = d.nproma % 64 == 0;
   // blocks per launch: the largest array (NPROMA,KLEV,5) must stay below 4 GiB for 32-bit offsets
   const int64_t per_block = (int64_t)d.nproma * d.klev * 5 * (int64_t)sizeof(R);
   const int64_t max_blocks = std::max<int64_t>(1, ((int64_t)1 << 32) / per_block - 1);
@@ -1067,14 +1080,11 @@ hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<
     if (def) {
       const KConstDefault<R> c = make_kconst_default<R>(q, ptsphy);
       const KConstDefault<RD> cd = make_kconst_default<RD>(q, ptsphy);
-      e = launch_variant<R, RD>(block, ncols, c, cd, pc, dc, ssc_off, stream);
+      e = launch_variant<R, RD>(block, uni, ncols, c, cd, pc, dc, ssc_off, stream);
     } else {
       const KConst<R> c = make_kconst<R>(q, ptsphy);
       const KConst<RD> cd = make_kconst<RD>(q, ptsphy);
-      const unsigned grid = (ncols + 255) / 256;
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KConst<R>, KConst<RD>, 256>), dim3(grid), dim3(256), 0, stream,
-                         c, cd, pc, dc, ssc_off);
-      e = hipGetLastError();
+      e = launch_variant<R, RD>(256, uni, ncols, c, cd, pc, dc, ssc_off, stream);
     }
     if (e != hipSuccess) return e;
   }

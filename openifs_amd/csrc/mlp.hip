@@ -28,8 +28,10 @@ namespace mlp {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 128, BN = 128, BK = 32;
-constexpr int LDK = BK + 8;  // LDS row stride in elements (80 bytes: 16-byte aligned, staggers banks)
+// Block tiles: TBM x TBN per 256-lane group (2 x 2 waves, each (TBM/2) x (TBN/2) in 32x32 MFMA tiles), K-step TBK.
+// 128 x 128 for small products (training batches), 256 x 128 / 256 x 256 for large ones (inference), where the
+// larger wave tile reads fewer LDS bytes per MFMA (1 KB / 0.75 KB / 0.5 KB).
+constexpr int BM = 128, BN = 128, BK = 32;  // smallest tile: the granularity of every dimension
 
 enum Epi : int {
   EPI_F32 = 0,      // C (fp32)
@@ -51,42 +53,44 @@ struct EpiArgs {
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
-template <int EPI>
+template <int EPI, int TBM, int TBN, int TBK>
 __global__ __launch_bounds__(256) void gemm_tn(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
                                                int ldb, int K, EpiArgs e) {
-  __shared__ bf16 As[2][BM * LDK];
-  __shared__ bf16 Bs[2][BN * LDK];
+  constexpr int FM = TBM / 64, FN = TBN / 64;        // 32x32 fragments per wave in M and N
+  constexpr int CPR = TBK / 8, RPP = 256 / CPR;      // 16-byte chunks per row; rows per pass of 256 lanes
+  constexpr int LA = TBM / RPP, LB = TBN / RPP;      // chunks per thread per K-step, A and B
+  constexpr int LDKT = TBK + 8;                      // LDS row stride (16-byte aligned, staggers banks)
+  __shared__ bf16 As[2][TBM * LDKT];
+  __shared__ bf16 Bs[2][TBN * LDKT];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  // global -> LDS: 128 rows x 32 k per operand = 512 chunks of 8 bf16; thread t moves chunks t and t+256
-  const int r0 = tid >> 2, kq = (tid & 3) * 8;
-  const bf16* ag0 = A + (size_t)(m0 + r0) * lda + kq;
-  const bf16* ag1 = ag0 + (size_t)64 * lda;
-  const bf16* bg0 = B + (size_t)(n0 + r0) * ldb + kq;
-  const bf16* bg1 = bg0 + (size_t)64 * ldb;
-  bf16x8 ra0, ra1, rb0, rb1;
+  const int m0 = blockIdx.y * TBM, n0 = blockIdx.x * TBN;
+  // global -> LDS in chunks of 8 bf16; thread t moves rows r0, r0 + RPP, ...
+  const int r0 = tid / CPR, kq = (tid % CPR) * 8;
+  const bf16* ag = A + (size_t)(m0 + r0) * lda + kq;
+  const bf16* bg = B + (size_t)(n0 + r0) * ldb + kq;
+  bf16x8 ra[LA], rb[LB];
   auto gload = [&](int k0) {
-    ra0 = *reinterpret_cast<const bf16x8*>(ag0 + k0);
-    ra1 = *reinterpret_cast<const bf16x8*>(ag1 + k0);
-    rb0 = *reinterpret_cast<const bf16x8*>(bg0 + k0);
-    rb1 = *reinterpret_cast<const bf16x8*>(bg1 + k0);
+#pragma unroll
+    for (int i = 0; i < LA; ++i) ra[i] = *reinterpret_cast<const bf16x8*>(ag + (size_t)(RPP * i) * lda + k0);
+#pragma unroll
+    for (int i = 0; i < LB; ++i) rb[i] = *reinterpret_cast<const bf16x8*>(bg + (size_t)(RPP * i) * ldb + k0);
   };
   auto sstore = [&](int buf) {
-    *reinterpret_cast<bf16x8*>(&As[buf][r0 * LDK + kq]) = ra0;
-    *reinterpret_cast<bf16x8*>(&As[buf][(r0 + 64) * LDK + kq]) = ra1;
-    *reinterpret_cast<bf16x8*>(&Bs[buf][r0 * LDK + kq]) = rb0;
-    *reinterpret_cast<bf16x8*>(&Bs[buf][(r0 + 64) * LDK + kq]) = rb1;
+#pragma unroll
+    for (int i = 0; i < LA; ++i) *reinterpret_cast<bf16x8*>(&As[buf][(r0 + RPP * i) * LDKT + kq]) = ra[i];
+#pragma unroll
+    for (int i = 0; i < LB; ++i) *reinterpret_cast<bf16x8*>(&Bs[buf][(r0 + RPP * i) * LDKT + kq]) = rb[i];
   };
-  f32x16 acc[2][2];
+  f32x16 acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-  const int nk = K / BK;
+  const int nk = K / TBK;
   gload(0);
   sstore(0);
   __syncthreads();
@@ -94,20 +98,20 @@ __global__ __launch_bounds__(256) void gemm_tn(const bf16* __restrict__ A, int l
   const int fr = lane & 31, fk = 8 * (lane >> 5);
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * BK);
+    if (kt + 1 < nk) gload((kt + 1) * TBK);
 #pragma unroll
-    for (int ks = 0; ks < BK; ks += 16) {
-      bf16x8 a[2], b[2];
+    for (int ks = 0; ks < TBK; ks += 16) {
+      bf16x8 a[FM], b[FN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
-        a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][(wm * 64 + i * 32 + fr) * LDK + ks + fk]);
+      for (int i = 0; i < FM; ++i)
+        a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][(wm * (TBM / 2) + i * 32 + fr) * LDKT + ks + fk]);
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-        b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][(wn * 64 + j * 32 + fr) * LDK + ks + fk]);
+      for (int j = 0; j < FN; ++j)
+        b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][(wn * (TBN / 2) + j * 32 + fr) * LDKT + ks + fk]);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     if (kt + 1 < nk) sstore(cur ^ 1);
     __syncthreads();
@@ -115,14 +119,14 @@ __global__ __launch_bounds__(256) void gemm_tn(const bf16* __restrict__ A, int l
 
   // Accumulator layout: register r of lane l holds row 8*(r>>2) + 4*(l>>5) + (r&3), column l & 31.
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+    for (int j = 0; j < FN; ++j) {
+      const int col = n0 + wn * (TBN / 2) + j * 32 + (lane & 31);
       const float bias = (EPI == EPI_HIDDEN || EPI == EPI_HIDDEN_INF || EPI == EPI_OUT) ? e.bias[col] : 0.0f;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const int row = m0 + wm * (TBM / 2) + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
         const size_t o = (size_t)row * e.ldc + col;
         const float v = acc[i][j][r];
         if (EPI == EPI_F32) {
@@ -240,11 +244,23 @@ int dev_alloc(cloudsc_mlp* m, size_t bytes, void** p) {
   return CLOUDSC_OK;
 }
 
+// Tile choice: the largest tile that divides M and N and still gives at least one group per CU (256 CUs).
 template <int EPI>
 hipError_t gemm(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, int K, const mlp::EpiArgs& e,
                 hipStream_t s) {
-  dim3 grid(N / mlp::BN, M / mlp::BM);
-  hipLaunchKernelGGL(mlp::gemm_tn<EPI>, grid, dim3(256), 0, s, A, lda, B, ldb, K, e);
+  const long groups256 = (M % 256 == 0 && N % 256 == 0) ? (long)(M / 256) * (N / 256) : 0;
+  const long groups256x128 = (M % 256 == 0) ? (long)(M / 256) * (N / 128) : 0;
+  const bool k64 = K % 64 == 0;
+  if (groups256 >= 256 && k64) {
+    hipLaunchKernelGGL((mlp::gemm_tn<EPI, 256, 256, 64>), dim3(N / 256, M / 256), dim3(256), 0, s, A, lda, B, ldb, K,
+                       e);
+  } else if (groups256x128 >= 256 && k64) {
+    hipLaunchKernelGGL((mlp::gemm_tn<EPI, 256, 128, 64>), dim3(N / 128, M / 256), dim3(256), 0, s, A, lda, B, ldb, K,
+                       e);
+  } else {
+    hipLaunchKernelGGL((mlp::gemm_tn<EPI, 128, 128, 32>), dim3(N / 128, M / 128), dim3(256), 0, s, A, lda, B, ldb, K,
+                       e);
+  }
   return hipGetLastError();
 }
 

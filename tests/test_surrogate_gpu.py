@@ -22,7 +22,9 @@ def _torch():
     return torch
 
 
-@pytest.mark.parametrize("M,N,K", [(128, 128, 32), (256, 384, 1920), (512, 896, 1024), (1024, 1024, 4096)])
+# tile paths: 128x128 (small), 256x128 (M=32768, N=896), 256x256 (M=16384, N=1024)
+@pytest.mark.parametrize("M,N,K", [(128, 128, 32), (256, 384, 1920), (512, 896, 1024), (1024, 1024, 4096),
+                                   (32768, 896, 32), (16384, 1024, 64)])
 def test_gemm_tn_matches_float64(M, N, K):
     torch = _torch()
     from openifs_amd import surrogate as S
@@ -108,6 +110,21 @@ def test_mlp_inference_forward_equals_training_forward():
     m.gradients(xd, x.T.contiguous().to(DEV), t.to(DEV), w.to(DEV))
     torch.cuda.synchronize()
     assert np.array_equal(y.cpu().numpy(), m.last_output(x.shape[0]))
+
+
+def test_mlp_large_batch_forward_equals_small_batches():
+    """Large batches run on the 256-row tiles, small ones on the 128-row tiles: same numbers."""
+    torch = _torch()
+    from openifs_amd import surrogate as S
+
+    m = S.MLP(256, 1024, 2, 896, 65536, seed=9)
+    g = torch.Generator().manual_seed(9)
+    x = torch.randn(65536, 256, generator=g).bfloat16().to(DEV)
+    y_big = m.forward(x)
+    y_small = torch.cat([m.forward(x[i:i + 1024]) for i in range(0, 65536, 1024)])
+    torch.cuda.synchronize()
+    d = (y_big - y_small).abs().max().item()
+    assert d <= 1e-5 * y_small.abs().max().item(), d
 
 
 def test_mlp_training_reduces_loss():
