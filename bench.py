@@ -16,6 +16,11 @@ Reported besides the throughput:
                 CPU oracle on the first 2048 columns
   cpu_baseline  the CPU oracle (C++/OpenMP restatement, FP32) on a sample of
                 the same workload, OpenMP over NPROMA=32 blocks, SCHEDULE(DYNAMIC,1)
+  hybrid        the learned surrogate: training throughput, inference and the
+                hybrid (surrogate + rerun of gated columns) against plain FP32
+  satadj        CLOUD_SATADJ (the saturation adjustment ahead of CLOUDSC), FP32
+                and mixed, on the same grid: kernel time, algorithmic GB/s and
+                fraction of HBM peak, error against the FP64 oracle, CPU oracle
 """
 import argparse
 import json
@@ -178,6 +183,75 @@ def surrogate_bench(params, train_cols, hybrid_cols, rerun_fraction, gate_fracti
     return res
 
 
+def satadj_bench(params, nproma, steps, warmup, cpu_threads, cpu_seconds):
+    """CLOUD_SATADJ (the saturation adjustment ahead of CLOUDSC), FP32 and mixed, on the same 655,360 x L137
+    workload: kernel time from HIP events on the launch stream, algorithmic bytes, accuracy of the first 2048
+    columns against the FP64 oracle, and the FP32 oracle on 16 CPU threads over a sample."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from openifs_amd import cloudsc as C
+    from openifs_amd import satadj as S
+
+    res = {}
+    bpc = S.algorithmic_bytes(KLEV, params.ncldtop)
+    ins = [n for n, k, r in S.SATADJ_BLOCKED if r == "i"]
+    outs = [n for n, k, r in S.SATADJ_BLOCKED if r == "o"]
+    for prec in ("sp", "mixed"):
+        f = S.SatadjFields(NGPTOT, nproma, KLEV, prec, "cuda:0")
+        S.synth_device(f, params, 20261203)
+        stream = torch.cuda.current_stream()
+        for _ in range(warmup):
+            S.satadj_blocked(f, params, 900.0, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+        for a, b in ev:
+            a.record(stream)
+            S.satadj_blocked(f, params, 900.0, stream=stream.cuda_stream)
+            b.record(stream)
+        torch.cuda.synchronize()
+        ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+        nb = 2048 // nproma
+        inp = {n: f[n][:nb].cpu().numpy().astype(np.float64) for n in ins}
+        got = {n: f[n][:nb].cpu().numpy().astype(np.float64) for n in outs}
+        cpu_inp = {n: f[n][:nb].cpu().numpy().copy() for n in ins} if prec == "sp" else None
+        del f
+        torch.cuda.empty_cache()
+        ref = dict(inp)
+        for n, kind, role in S.SATADJ_BLOCKED:
+            if role == "o":
+                ref[n] = np.full(C.blocked_shape(kind, nproma, KLEV, nb), np.nan)
+        oracle.satadj_blocked("dp", params, nb * nproma, nproma, KLEV, 900.0, ref, nthreads=cpu_threads)
+        err = {}
+        for n in outs:
+            a, r = got[n], ref[n]
+            if n == "ptendency_loc_cld":
+                a, r = a[:, :2], r[:, :2]
+            m = ~np.isnan(r)
+            err[n] = float(np.sqrt(np.sum((a[m] - r[m]) ** 2) / max(np.sum(r[m] ** 2), 1e-300)))
+        gbs = bpc * NGPTOT / (ms / 1e3) / 1e9
+        res[prec] = {"kernel_ms": ms, "columns_per_s": NGPTOT / (ms / 1e3), "achieved_gbs": gbs,
+                     "frac_hbm": gbs / HBM_PEAK_GBS, "err_vs_fp64": err}
+        if cpu_inp is not None and cpu_seconds > 0:
+            # FP32 oracle over the 2048 sampled columns, repeated for about cpu_seconds
+            arr = dict(cpu_inp)
+            for n, kind, role in S.SATADJ_BLOCKED:
+                if role == "o":
+                    arr[n] = np.zeros(C.blocked_shape(kind, nproma, KLEV, nb), dtype=np.float32)
+            passes, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < cpu_seconds:
+                oracle.satadj_blocked("sp", params, nb * nproma, nproma, KLEV, 900.0, arr, nthreads=cpu_threads)
+                passes += 1
+            dt = time.perf_counter() - t0
+            res["cpu_baseline"] = {"value": passes * nb * nproma / dt, "unit": "columns/s", "cores": cpu_threads,
+                                   "kind": "port", "sample": f"{nb * nproma} columns x {passes} passes ({dt:.1f} s), "
+                                                             "FP32 oracle, OpenMP over NPROMA blocks"}
+    res["algorithmic_bytes_per_column"] = bpc
+    res["workload"] = f"CLOUD_SATADJ, NLEV={KLEV}, {NGPTOT} columns, NPROMA={nproma}, dt=900 s"
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +263,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hybrid", action="store_true")
+    ap.add_argument("--no-satadj", action="store_true")
     ap.add_argument("--train-cols", type=int, default=20_000_000)
     ap.add_argument("--hybrid-cols", type=int, default=6_599_680)
     ap.add_argument("--rerun-fraction", type=float, default=0.01)
@@ -274,6 +349,11 @@ def main():
         hybrid = surrogate_bench(params, args.train_cols, args.hybrid_cols, args.rerun_fraction, args.gate_fraction,
                                  args.infer_batch)
 
+    satadj = None
+    if not args.no_satadj:
+        satadj = satadj_bench(params, args.nproma, max(5, min(args.steps, 20)), 2,
+                              args.cpu_threads or min(16, os.cpu_count() or 1), 0.0 if args.no_cpu else 3.0)
+
     cpu = None
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -295,7 +375,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic (seeded atmospheres, BASELINE configuration distributions)",
+        "data": "synthetic (seeded atmospheres generated on the device; distributions in include/cloudsc_synth.h)",
         "config": {"workload": f"plain CLOUDSC FP32, NLEV={KLEV}, {NGPTOT} columns (TCo399-sized), "
                                f"NPROMA={args.nproma}, dt=900 s", "columns": NGPTOT, "nlev": KLEV,
                    "nproma": args.nproma, "parallelism": "single GPU"},
@@ -305,6 +385,7 @@ def main():
                      "algorithmic_bytes_per_column": bpc},
         "err_vs_fp64": err,
         "hybrid": hybrid,
+        "satadj": satadj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -36,7 +36,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"),
 ]
-SOURCES = ["cloudsc_kernel.hip", "cloudsc_capi.hip", "synth.hip", "cloudsc_params.cpp", "mlp.hip", "surrogate.hip"]
+SOURCES = ["cloudsc_kernel.hip", "cloudsc_capi.hip", "synth.hip", "cloudsc_params.cpp", "mlp.hip", "surrogate.hip",
+           "satadj.hip"]
 
 
 def _run(cmd, cwd=None):

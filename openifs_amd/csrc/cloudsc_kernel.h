@@ -105,6 +105,14 @@ struct KPtrs {
   R *pfplsl, *pfplsn, *pfhpsl, *pfhpsn;
 };
 
+// Host-side constant blocks for a launch (cloudsc_kernel.hip); is_default_config: every constant and switch
+// equals the compiled-in default configuration, so KConstDefault may be used.
+template <typename T>
+KConst<T> make_kconst(const cloudsc_params_t& q, double ptsphy);
+template <typename T>
+KConstDefault<T> make_kconst_default(const cloudsc_params_t& q, double ptsphy);
+bool is_default_config(const cloudsc_params_t& q);
+
 struct KDims {
   int32_t ngptot;   // columns in total (blocked) or KFDIA (one block)
   int32_t nproma;   // KLON

@@ -110,3 +110,44 @@ def thermo(params, name, t):
     if rc != 0:
         raise RuntimeError(name)
     return out
+
+
+def satadj_block(precision, params, kidia, kfdia, klon, klev, ptsphy, arrays):
+    """One CLOUD_SATADJ block on the CPU, same argument list as the product's cloudsc_satadj_*_c."""
+    from openifs_amd.satadj import call_block
+
+    rc = call_block(getattr(lib(), f"oracle_satadj_{precision}"), params, kidia, kfdia, klon, klev, ptsphy, arrays,
+                    precision)
+    if rc != 0:
+        raise RuntimeError(f"oracle_satadj_{precision} returned {rc}")
+    return arrays
+
+
+def satadj_blocked(precision, params, ngptot, nproma, klev, ptsphy, arrays, nthreads=0):
+    """All blocks of a CLOUD_SATADJ blocked field set (dict of C-order numpy arrays, openifs_amd.satadj)."""
+    from openifs_amd.satadj import SATADJ_BLOCKED
+
+    ngpblks = (ngptot + nproma - 1) // nproma
+    dt = np.float64 if precision == "dp" else np.float32
+    real = ctypes.c_double if precision == "dp" else ctypes.c_float
+    args = [ctypes.byref(params), ctypes.c_int(ngptot), ctypes.c_int(nproma), ctypes.c_int(klev),
+            ctypes.c_int(ngpblks), real(ptsphy)]
+    for name, kind, role in SATADJ_BLOCKED:
+        a = arrays[name]
+        if a.dtype != dt or not a.flags.c_contiguous:
+            raise TypeError(f"{name}: need a C-contiguous {np.dtype(dt).name} array")
+        args.append(ctypes.c_void_p(a.ctypes.data))
+    args.append(ctypes.c_int(nthreads))
+    rc = getattr(lib(), f"oracle_satadj_blocked_{precision}")(*args)
+    if rc != 0:
+        raise RuntimeError(f"oracle_satadj_blocked_{precision} returned {rc}")
+    return arrays
+
+
+def qsatmixadj(params, t, q, pp):
+    """QSATMIXADJ (two Newton iterations) on arrays of points, FP64."""
+    t, q, pp = (np.ascontiguousarray(x, dtype=np.float64) for x in (t, q, pp))
+    out = np.zeros_like(t)
+    lib().oracle_qsatmixadj_dp(ctypes.byref(params), ctypes.c_int(t.size), *(ctypes.c_void_p(x.ctypes.data)
+                                                                             for x in (t, q, pp, out)))
+    return out

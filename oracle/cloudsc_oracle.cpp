@@ -6,11 +6,16 @@
 //   oracle_cloudsc_{dp,sp,mixed}        one NPROMA block, CLOUDSC's argument list
 //   oracle_cloudsc_blocked_{dp,sp,mixed} all blocks, OpenMP over blocks with
 //       SCHEDULE(DYNAMIC,1) as the reference driver (ec_phys_drv.F90:430-439)
+//   oracle_satadj_{dp,sp,mixed}         one block of CLOUD_SATADJ, the argument list of
+//       include/cloudsc_satadj.h
+//   oracle_satadj_blocked_{dp,sp,mixed}  all blocks, OpenMP over blocks
 //   oracle_supersatcheck_*, oracle_lu5_dp, oracle_truncate_sinks_dp,
-//   oracle_thermo_dp                     unit-level entry points for the tests
+//   oracle_thermo_dp, oracle_qsatmixadj_dp  unit-level entry points for the tests
 #include <omp.h>
 
+#include "../include/cloudsc_satadj.h"
 #include "cloudsc_oracle.hpp"
+#include "satadj_oracle.hpp"
 
 namespace {
 
@@ -155,6 +160,65 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
       R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur,   \
       R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, int nthreads
 
+// ---- CLOUD_SATADJ ----
+template <typename R>
+int satadj_check(const cloudsc_params_t* p, int kidia, int kfdia, int klon, int klev) {
+  if (!p) return CLOUDSC_ENULL;
+  if (p->lspp || p->lcldbud) return CLOUDSC_EUNSUPPORTED;
+  if (klon < 1 || klev < 2 || kidia < 1 || kfdia < kidia || kfdia > klon) return CLOUDSC_EDIMS;
+  if (p->ncldtop < 2 || p->ncldtop > klev) return CLOUDSC_EDIMS;
+  return CLOUDSC_OK;
+}
+
+#define ORACLE_SATADJ_CALL(R, RD)                                                                              \
+  (void)ktype, (void)lddiag, (void)pmfu, (void)pmfd, (void)pgp2dspp, (void)pextra, (void)kfldx;                 \
+  const int rc = satadj_check<R>(ydparams, kidia, kfdia, klon, klev);                                          \
+  if (rc) return rc;                                                                                           \
+  oracle::SatadjArgs<R> a{kidia, kfdia, klon, klev, ptsphy, pap, paph, pt, pq, pa, pl, pi, ptendency_cml_t,    \
+                          ptendency_cml_q, ptendency_cml_a, ptendency_cml_l, ptendency_cml_i, phrsw, phrlw,     \
+                          ptendency_vdf_t, ptendency_vdf_q, pludeli, pvervel, ptendency_loc_t, ptendency_loc_q, \
+                          ptendency_loc_a, ptendency_loc_l, ptendency_loc_i};                                  \
+  oracle::satadj<R, RD>(*ydparams, a);                                                                         \
+  return CLOUDSC_OK
+
+template <typename R, typename RD>
+int satadj_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,
+                   const R* pap, const R* paph, const R* pt, const R* pq, const R* pa, const R* pclv,
+                   const R* tcml_t, const R* tcml_q, const R* tcml_a, const R* tcml_cld, const R* phrsw,
+                   const R* phrlw, const R* tvdf_t, const R* tvdf_q, const R* pludeli, const R* pvervel, R* tloc_t,
+                   R* tloc_q, R* tloc_a, R* tloc_cld, int nthreads) {
+  int rc = satadj_check<R>(p, 1, nproma, nproma, klev);
+  if (rc) return rc;
+  if (ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma) return CLOUDSC_EDIMS;
+  const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int ibl = 0; ibl < ngpblks; ++ibl) {
+    const int kfdia = std::min(nproma, ngptot - ibl * nproma);
+    const size_t o2 = ibl * s2, oh = ibl * sh, o5 = ibl * s2 * 5, o4 = ibl * s2 * 4;
+    oracle::SatadjArgs<R> a{1, kfdia, nproma, klev, ptsphy, pap + o2, paph + oh, pt + o2, pq + o2, pa + o2,
+                            pclv + o5, pclv + o5 + s2, tcml_t + o2, tcml_q + o2, tcml_a + o2, tcml_cld + o5,
+                            tcml_cld + o5 + s2, phrsw + o2, phrlw + o2, tvdf_t + o2, tvdf_q + o2, pludeli + o4,
+                            pvervel + o2, tloc_t + o2, tloc_q + o2, tloc_a + o2, tloc_cld + o5, tloc_cld + o5 + s2};
+    oracle::satadj<R, RD>(*p, a);
+  }
+  return CLOUDSC_OK;
+}
+
+#define ORACLE_SATADJ_BLOCKED_ARGS(R)                                                                   \
+  const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,            \
+      const R *pap, const R *paph, const R *pt, const R *pq, const R *pa, const R *pclv,                \
+      const R *ptendency_cml_t, const R *ptendency_cml_q, const R *ptendency_cml_a,                     \
+      const R *ptendency_cml_cld, const R *phrsw, const R *phrlw, const R *ptendency_vdf_t,             \
+      const R *ptendency_vdf_q, const R *pludeli, const R *pvervel, R *ptendency_loc_t,                 \
+      R *ptendency_loc_q, R *ptendency_loc_a, R *ptendency_loc_cld, int nthreads
+
+#define ORACLE_SATADJ_BLOCKED_CALL(R, RD)                                                                      \
+  return satadj_blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy, pap, paph, pt, pq, pa, pclv,     \
+                               ptendency_cml_t, ptendency_cml_q, ptendency_cml_a, ptendency_cml_cld, phrsw,      \
+                               phrlw, ptendency_vdf_t, ptendency_vdf_q, pludeli, pvervel, ptendency_loc_t,       \
+                               ptendency_loc_q, ptendency_loc_a, ptendency_loc_cld, nthreads)
+
 }  // namespace
 
 extern "C" {
@@ -227,6 +291,22 @@ int oracle_thermo_dp(const cloudsc_params_t* p, int fn, int n, const double* t, 
     }
     out[i] = y;
   }
+  return 0;
+}
+
+int oracle_satadj_dp(CLOUDSC_SATADJ_ARGS(double)) { ORACLE_SATADJ_CALL(double, double); }
+int oracle_satadj_sp(CLOUDSC_SATADJ_ARGS(float)) { ORACLE_SATADJ_CALL(float, float); }
+int oracle_satadj_mixed(CLOUDSC_SATADJ_ARGS(float)) { ORACLE_SATADJ_CALL(float, double); }
+
+int oracle_satadj_blocked_dp(ORACLE_SATADJ_BLOCKED_ARGS(double)) { ORACLE_SATADJ_BLOCKED_CALL(double, double); }
+int oracle_satadj_blocked_sp(ORACLE_SATADJ_BLOCKED_ARGS(float)) { ORACLE_SATADJ_BLOCKED_CALL(float, float); }
+int oracle_satadj_blocked_mixed(ORACLE_SATADJ_BLOCKED_ARGS(float)) { ORACLE_SATADJ_BLOCKED_CALL(float, double); }
+
+// QSATMIXADJ on n points: humidity after the two Newton iterations.
+int oracle_qsatmixadj_dp(const cloudsc_params_t* p, int n, const double* t, const double* q, const double* pp,
+                         double* out) {
+  const oracle::Thermo<double> th(*p);
+  for (int i = 0; i < n; ++i) out[i] = oracle::qsatmixadj<double>(th, p->retv, t[i], q[i], pp[i]);
   return 0;
 }
 

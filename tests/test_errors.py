@@ -51,3 +51,33 @@ def test_python_layer_fails_loudly_without_library(monkeypatch, tmp_path):
     monkeypatch.setattr(C, "_lib", None)
     with pytest.raises(C.CloudscLibraryError):
         C.lib()
+
+
+def _satadj_blocked_null_call(params, ngptot=64, nproma=32, klev=137, ngpblks=2):
+    from openifs_amd import satadj as S
+
+    f = C.lib().cloudsc_satadj_blocked_sp_c
+    args = [ctypes.byref(params), ctypes.c_int(ngptot), ctypes.c_int(nproma), ctypes.c_int(klev),
+            ctypes.c_int(ngpblks), ctypes.c_float(900.0)]
+    args += [ctypes.c_void_p(0)] * len(S.SATADJ_BLOCKED)
+    args.append(ctypes.c_void_p(0))
+    return f(*args)
+
+
+def test_satadj_errors():
+    from openifs_amd import satadj as S
+
+    for sw in ("lspp", "lcldbud"):
+        p = C.default_params()
+        setattr(p, sw, 1)
+        assert _satadj_blocked_null_call(p) == -2
+        assert b"not implemented" in C.lib().cloudsc_last_error()
+    p = C.default_params()
+    assert _satadj_blocked_null_call(p, ngpblks=3) == -1
+    assert _satadj_blocked_null_call(p, klev=1) == -1
+    assert _satadj_blocked_null_call(p) == -3
+    arrs = S.allocate_block(8, 137, "sp")
+    for kidia, kfdia in ((0, 8), (5, 4), (1, 9)):
+        with pytest.raises(C.CloudscError) as e:
+            S.CLOUD_SATADJ(p, kidia, kfdia, 8, 137, 900.0, arrs, precision="sp")
+        assert e.value.code == -1
