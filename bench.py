@@ -42,9 +42,9 @@ def bytes_per_column(klev, word=4):
     return word * (49 * klev + 20)
 
 
-def read_traffic(nproma):
-    """HBM bytes per dispatch measured by rocprofv3 (profiles/cloudsc_sp_l137_655360_traffic.md)."""
-    path = os.path.join(ROOT, "profiles", "cloudsc_sp_l137_655360_traffic.md")
+def read_traffic(nproma, name="cloudsc_sp_l137_655360_traffic.md"):
+    """HBM bytes per dispatch measured by rocprofv3 (profiles/<name>)."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     txt = open(path).read()
@@ -231,8 +231,11 @@ def satadj_bench(params, nproma, steps, warmup, cpu_threads, cpu_seconds):
             m = ~np.isnan(r)
             err[n] = float(np.sqrt(np.sum((a[m] - r[m]) ** 2) / max(np.sum(r[m] ** 2), 1e-300)))
         gbs = bpc * NGPTOT / (ms / 1e3) / 1e9
-        res[prec] = {"kernel_ms": ms, "columns_per_s": NGPTOT / (ms / 1e3), "achieved_gbs": gbs,
-                     "frac_hbm": gbs / HBM_PEAK_GBS, "err_vs_fp64": err}
+        res[prec] = {"kernel_ms": ms, "columns_per_s": NGPTOT / (ms / 1e3),
+                     "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": gbs / HBM_PEAK_GBS,
+                                  "traffic": read_traffic(nproma, "satadj_sp_l137_655360_traffic.md")},
+                     "err_vs_fp64": err}
         if cpu_inp is not None and cpu_seconds > 0:
             # FP32 oracle over the 2048 sampled columns, repeated for about cpu_seconds
             arr = dict(cpu_inp)
