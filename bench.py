@@ -252,7 +252,70 @@ def satadj_bench(params, nproma, steps, warmup, cpu_threads, cpu_seconds):
                                                              "FP32 oracle, OpenMP over NPROMA blocks"}
     res["algorithmic_bytes_per_column"] = bpc
     res["workload"] = f"CLOUD_SATADJ, NLEV={KLEV}, {NGPTOT} columns, NPROMA={nproma}, dt=900 s"
+    res["fused"] = fused_bench(params, nproma, steps)
     return res
+
+
+def fused_bench(params, nproma, steps):
+    """The CALLPAR sequence CLOUD_SATADJ -> TENDENCY_CML increment -> TENDENCY_SATADJ copy -> CLOUDSC in FP32:
+    separate launches (the adjustment kernel, the increment and copy as device element-wise operations, the CLOUDSC
+    kernel) against the single fused launch.  Inputs (PLUDE, the CML tendencies) are restored outside the timed
+    events before every step."""
+    import numpy as np
+    import torch
+
+    from openifs_amd import cloudsc as C
+    from openifs_amd import satadj as S
+
+    f = S.FusedFields(NGPTOT, nproma, KLEV, "sp", "cuda:0")
+    S.synth_device(f, params, 20261204)
+    keep = {n: f[n].clone() for n in ("plude", "ptendency_cml_t", "ptendency_cml_q", "ptendency_cml_a",
+                                      "ptendency_cml_cld")}
+    stream = torch.cuda.current_stream()
+
+    def restore():
+        for n, x in keep.items():
+            f[n].copy_(x, non_blocking=True)
+
+    def unfused():
+        S.satadj_blocked(f, params, 900.0, stream=stream.cuda_stream)
+        f["ptendency_satadj_t"].copy_(f["ptendency_loc_t"])
+        f["ptendency_satadj_q"].copy_(f["ptendency_loc_q"])
+        f["ptendency_satadj_a"].copy_(f["ptendency_loc_a"])
+        f["ptendency_satadj_cld"][:, :2].copy_(f["ptendency_loc_cld"][:, :2])
+        f["ptendency_cml_t"].add_(f["ptendency_satadj_t"])
+        f["ptendency_cml_q"].add_(f["ptendency_satadj_q"])
+        f["ptendency_cml_a"].add_(f["ptendency_satadj_a"])
+        f["ptendency_cml_cld"][:, :2].add_(f["ptendency_satadj_cld"][:, :2])
+        C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)
+
+    def fused():
+        S.satadj_cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)
+
+    out = {}
+    for name, run in (("unfused_ms", unfused), ("fused_ms", fused)):
+        ts = []
+        for i in range(steps + 2):
+            restore()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            run()
+            b.record(stream)
+            torch.cuda.synchronize()
+            if i >= 2:
+                ts.append(a.elapsed_time(b))
+        out[name] = float(np.mean(ts))
+    bpc = S.fused_bytes_per_column(KLEV, params.ncldtop)
+    gbs = bpc * NGPTOT / (out["fused_ms"] / 1e3) / 1e9
+    out.update({"speedup": out["unfused_ms"] / out["fused_ms"], "columns_per_s": NGPTOT / (out["fused_ms"] / 1e3),
+                "algorithmic_bytes_per_column": bpc,
+                "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+                "workload": f"CLOUD_SATADJ + CML increment + CLOUDSC FP32, NLEV={KLEV}, {NGPTOT} columns, "
+                            f"NPROMA={nproma}"})
+    del f, keep
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():

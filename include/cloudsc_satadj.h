@@ -57,6 +57,31 @@ int cloudsc_satadj_blocked_dp_c(CLOUDSC_SATADJ_BLOCKED_ARGS(double));
 int cloudsc_satadj_blocked_sp_c(CLOUDSC_SATADJ_BLOCKED_ARGS(float));
 int cloudsc_satadj_blocked_mixed_c(CLOUDSC_SATADJ_BLOCKED_ARGS(float));
 
+/* Fused CALLPAR sequence on device-resident blocked fields, one launch: CLOUD_SATADJ, the increment of the
+ * cumulated tendencies by its result (STATE_INCREMENT, callpar.F90:1278-1279), the copy of its result into
+ * TENDENCY_SATADJ (STATE_COPY, :1289) and CLOUDSC (CLOUD_LAYER, :1292-1296), with each column's adjustment done
+ * level by level inside CLOUDSC's sweep.  Arguments: those of cloudsc_blocked_*_c (the CML tendencies are INOUT
+ * here: they return incremented, as TENDENCY_CML after STATE_INCREMENT; PLUDELI carries all four species),
+ * then the SATADJ-only inputs and the TENDENCY_SATADJ outputs (T, Q, A and species 1-2 of the five-species
+ * field; may all be NULL).  Results equal cloudsc_satadj_blocked_*_c, the increment and cloudsc_blocked_*_c in
+ * sequence, bit for bit. */
+#define CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(R)                                                          \
+  const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,            \
+      const R *pt, const R *pq, R *ptendency_cml_t, R *ptendency_cml_q, R *ptendency_cml_a,             \
+      R *ptendency_cml_cld, R *ptendency_loc_t, R *ptendency_loc_q, R *ptendency_loc_a,                 \
+      R *ptendency_loc_cld, const R *pvfl, const R *pvfi, const R *pap, const R *paph, const R *plsm,   \
+      const int32_t *ldcum, const int32_t *ktype, R *peis, const R *plu, R *plude, const R *pludeli,    \
+      const R *psnde, const R *pmfu, const R *pmfd, const R *pa, const R *pclv, R *pcovptot,            \
+      R *prainfrac_toprfz, R *pfsqlf, R *pfsqif, R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf,          \
+      R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur, R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn,     \
+      const R *phrsw, const R *phrlw, const R *ptendency_vdf_t, const R *ptendency_vdf_q,               \
+      const R *pvervel, R *ptendency_satadj_t, R *ptendency_satadj_q, R *ptendency_satadj_a,            \
+      R *ptendency_satadj_cld, void *stream
+
+int cloudsc_satadj_cloudsc_blocked_dp_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(double));
+int cloudsc_satadj_cloudsc_blocked_sp_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(float));
+int cloudsc_satadj_cloudsc_blocked_mixed_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(float));
+
 /* Synthetic forcings for CLOUD_SATADJ on a blocked field set (device pointers), seeded per column as
  * cloudsc_synth_*: PHRSW ~ U(0, 2e-5) K/s, PHRLW ~ N(-2e-5, 1e-5) K/s, PTENDENCY_VDF_T ~ N(0, 1e-5) K/s,
  * PTENDENCY_VDF_Q ~ N(0, 1e-8) /s, PVERVEL ~ N(0, 0.3) Pa/s, PLUDELI(:,:,3) = PLUDE * 1e-2 (detrained vapour)

@@ -13,14 +13,12 @@
 #include <vector>
 
 #include "../../include/cloudsc_hip.h"
+#include "../../include/cloudsc_satadj.h"
 #include "../../include/cloudsc_tune.h"
 #include "capi_util.h"
 #include "cloudsc_kernel.h"
 
 namespace cloudsc_gpu {
-template <typename R, typename RD>
-hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
-                          hipStream_t stream);
 int tuned_block(int variant, int klev);
 void set_tuned_block(int variant, int klev, int block);
 }
@@ -78,7 +76,7 @@ using cloudsc_capi::fail;
 
 template <typename R, typename RD>
 int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngpblks, R ptsphy, const KPtrs<R>& p,
-            void* stream) {
+            void* stream, const cloudsc_gpu::KSatPtrs<R>* sat = nullptr) {
   int rc = check_config(q);
   if (rc) return rc;
   if (nproma < 1 || klev < 2 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma)
@@ -91,10 +89,19 @@ int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngp
                        p.pfcqrng, p.pfcqsng, p.pfsqltur, p.pfsqitur, p.pfplsl, p.pfplsn, p.pfhpsl, p.pfhpsn};
   for (const void* x : req)
     if (!x) return fail(CLOUDSC_ENULL, "cloudsc_blocked: NULL array on the default path");
+  if (sat) {
+    const void* sreq[] = {sat->phrsw, sat->phrlw, sat->tvdf_t, sat->tvdf_q, sat->pvervel};
+    for (const void* x : sreq)
+      if (!x) return fail(CLOUDSC_ENULL, "cloudsc_satadj_cloudsc_blocked: NULL CLOUD_SATADJ input");
+    const bool any = sat->tsat_t || sat->tsat_q || sat->tsat_a || sat->tsat_cld;
+    const bool all = sat->tsat_t && sat->tsat_q && sat->tsat_a && sat->tsat_cld;
+    if (any && !all)
+      return fail(CLOUDSC_ENULL, "cloudsc_satadj_cloudsc_blocked: TENDENCY_SATADJ arrays must be all set or all NULL");
+  }
   rc = check_device();
   if (rc) return rc;
   KDims d{ngptot, nproma, klev, ngpblks, 1, nproma};
-  hipError_t e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, 0, (hipStream_t)stream);
+  hipError_t e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, 0, (hipStream_t)stream, sat);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_blocked: kernel launch", e);
   return CLOUDSC_OK;
 }
@@ -263,7 +270,24 @@ KPtrs<R> blocked_ptrs(const R* pt, const R* pq, const R* tcml_t, const R* tcml_q
                                         pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn),                      \
                         stream)
 
+#define FUSED_CALL(R, RD)                                                                                        \
+  cloudsc_gpu::KSatPtrs<R> sat{phrsw, phrlw, ptendency_vdf_t, ptendency_vdf_q, pvervel, ptendency_cml_t,          \
+                               ptendency_cml_q, ptendency_cml_a, ptendency_cml_cld, ptendency_satadj_t,           \
+                               ptendency_satadj_q, ptendency_satadj_a, ptendency_satadj_cld};                     \
+  return blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy,                                          \
+                        blocked_ptrs<R>(pt, pq, ptendency_cml_t, ptendency_cml_q, ptendency_cml_a,                \
+                                        ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a,     \
+                                        ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis, plu,  \
+                                        plude, pludeli, psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz,  \
+                                        pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng,       \
+                                        pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn),                      \
+                        stream, &sat)
+
 extern "C" {
+
+int cloudsc_satadj_cloudsc_blocked_dp_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(double)) { FUSED_CALL(double, double); }
+int cloudsc_satadj_cloudsc_blocked_sp_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(float)) { FUSED_CALL(float, float); }
+int cloudsc_satadj_cloudsc_blocked_mixed_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(float)) { FUSED_CALL(float, double); }
 
 int cloudsc_dp_c(CLOUDSC_ARGS(double)) { PER_BLOCK_CALL(double, double); }
 int cloudsc_sp_c(CLOUDSC_ARGS(float)) { PER_BLOCK_CALL(float, float); }

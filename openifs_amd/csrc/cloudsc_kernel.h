@@ -105,6 +105,17 @@ struct KPtrs {
   R *pfplsl, *pfplsn, *pfhpsl, *pfhpsn;
 };
 
+// Extra device pointers of the fused saturation adjustment + CLOUDSC launch (the CALLPAR sequence
+// callpar.F90:1253-1293: CLOUD_SATADJ, STATE_INCREMENT of TENDENCY_CML, STATE_COPY to TENDENCY_SATADJ, CLOUDSC).
+// tcml_* alias the CML tendencies of KPtrs and are updated in place; tsat_* (may be NULL) receive the adjustment's
+// tendencies (TENDENCY_SATADJ; species 1 and 2 of tsat_cld).
+template <typename R>
+struct KSatPtrs {
+  const R *phrsw, *phrlw, *tvdf_t, *tvdf_q, *pvervel;
+  R *tcml_t, *tcml_q, *tcml_a, *tcml_cld;
+  R *tsat_t, *tsat_q, *tsat_a, *tsat_cld;
+};
+
 // Host-side constant blocks for a launch (cloudsc_kernel.hip); is_default_config: every constant and switch
 // equals the compiled-in default configuration, so KConstDefault may be used.
 template <typename T>
@@ -121,5 +132,12 @@ struct KDims {
   int32_t kidia;    // first active column of every block (1-based)
   int32_t kfdia;    // last active column of a full block (1-based)
 };
+
+// Blocked launch over all blocks (cloudsc_kernel.hip): splits the grid so that every array of one launch spans
+// less than 4 GiB.  ssc_off: KIDIA-1 offset of CLOUD_SUPERSATCHECK's pressure argument (per-block entry).
+// sat: NULL for CLOUDSC alone, else the fused saturation adjustment + CLOUDSC.
+template <typename R, typename RD>
+hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
+                          hipStream_t stream, const KSatPtrs<R>* sat = nullptr);
 
 }  // namespace cloudsc_gpu

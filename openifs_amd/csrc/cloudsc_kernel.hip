@@ -9,6 +9,7 @@
 
 #include "cloudsc_device.h"
 #include "cloudsc_kernel.h"
+#include "satadj_device.h"
 
 namespace cloudsc_gpu {
 
@@ -24,9 +25,13 @@ __device__ __forceinline__ T trunc_scale(T x, int jo, int a, int b, T zz) {
 // KC / KCD: constants of the working precision and of the FP64 islands, either the run-time
 // structure KConst or the compile-time default configuration KConstDefault (same member names).
 // UNI: every wave lies in one block (NPROMA a multiple of 64), so block-dependent offsets are wave-uniform.
-template <typename R, typename RD, typename KC, typename KCD, int BLOCK, bool UNI>
+// SAT: the fused CALLPAR sequence (callpar.F90:1253-1293): on every level at or below NCLDTOP the column first runs
+// CLOUD_SATADJ for the point (satadj_point), adds its tendencies to TENDENCY_CML in place (STATE_INCREMENT,
+// :1278-1279), stores them as TENDENCY_SATADJ (STATE_COPY, :1289), and CLOUDSC then reads the incremented CML
+// tendencies from registers.
+template <typename R, typename RD, typename KC, typename KCD, int BLOCK, bool UNI, bool SAT>
 __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p,
-                                                                const KDims d, int ssc_off) {
+                                                                const KDims d, int ssc_off, const KSatPtrs<R> s) {
   const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
   const uint32_t nproma = (uint32_t)d.nproma, klev = (uint32_t)d.klev;
   if (g >= (uint32_t)d.ngpblks * nproma) return;
@@ -99,18 +104,57 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
     const R paph_k1 = LD(p.paph, ih1, jo);
     const R pap = LD(p.pap, i2, jo);
     const R pt = LD(p.pt, i2, jo), pq = LD(p.pq, i2, jo), pa = LD(p.pa, i2, jo);
-    const R tcml_t = LD(p.tcml_t, i2, jo), tcml_q = LD(p.tcml_q, i2, jo), tcml_a = LD(p.tcml_a, i2, jo);
+    R tcml_t = LD(p.tcml_t, i2, jo), tcml_q = LD(p.tcml_q, i2, jo), tcml_a = LD(p.tcml_a, i2, jo);
     const R pvfl = LD(p.pvfl, i2, jo), pvfi = LD(p.pvfi, i2, jo);
     const R pludeli1 = LD(p.pludeli, i4, jo);
     const R pludeli2 = LD(p.pludeli, i4 + sstep, jo);
+    const R pcl_l = LD(p.pclv, i5, jo), pcl_i = LD(p.pclv, i5 + sstep, jo);
+    R tcml_l = LD(p.tcml_cld, i5, jo), tcml_i = LD(p.tcml_cld, i5 + sstep, jo);
+
+    if constexpr (SAT) {
+      if ((int)k >= ncldtop0) {  // CLOUD_SATADJ for this point, then TENDENCY_CML += its tendencies
+        SatIn<R> x;
+        x.pt = pt, x.pq = pq, x.pa = pa, x.pl = pcl_l, x.pi = pcl_i, x.pap = pap;
+        x.paph0 = paph_k, x.paph1 = paph_k1, x.psurf = paph_surf;
+        x.tct = tcml_t, x.tcq = tcml_q, x.tca = tcml_a, x.tcl = tcml_l, x.tci = tcml_i;
+        x.hrsw = LD(s.phrsw, i2, jo), x.hrlw = LD(s.phrlw, i2, jo);
+        x.vdft = LD(s.tvdf_t, i2, jo), x.vdfq = LD(s.tvdf_q, i2, jo);
+        x.lude3 = LD(p.pludeli, i4 + 2 * sstep, jo), x.lude4 = LD(p.pludeli, i4 + 3 * sstep, jo);
+        x.vervel = LD(s.pvervel, i2, jo);
+        const SatOut<R> y = satadj_point<R, RD, KC, KCD>(c, cd, x);
+        tcml_t = tcml_t + y.t;
+        tcml_q = tcml_q + y.q;
+        tcml_a = tcml_a + y.a;
+        tcml_l = tcml_l + y.l;
+        tcml_i = tcml_i + y.i;
+        ST(s.tcml_t, i2, jo, tcml_t);
+        ST(s.tcml_q, i2, jo, tcml_q);
+        ST(s.tcml_a, i2, jo, tcml_a);
+        ST(s.tcml_cld, i5, jo, tcml_l);
+        ST(s.tcml_cld, i5 + sstep, jo, tcml_i);
+        if (s.tsat_t) {
+          ST(s.tsat_t, i2, jo, y.t);
+          ST(s.tsat_q, i2, jo, y.q);
+          ST(s.tsat_a, i2, jo, y.a);
+          ST(s.tsat_cld, i5, jo, y.l);
+          ST(s.tsat_cld, i5 + sstep, jo, y.i);
+        }
+      } else if (s.tsat_t) {  // zero above NCLDTOP (cloud_satadj.F90:338-348); TENDENCY_CML unchanged
+        ST(s.tsat_t, i2, jo, R(0));
+        ST(s.tsat_q, i2, jo, R(0));
+        ST(s.tsat_a, i2, jo, R(0));
+        ST(s.tsat_cld, i5, jo, R(0));
+        ST(s.tsat_cld, i5 + sstep, jo, R(0));
+      }
+    }
 
     // ---- 1. initial values (:982-1002) ----
     const R ztp1 = pt + ptsphy * tcml_t;
     R qv = pq + ptsphy * tcml_q;
     R za = pa + ptsphy * tcml_a;
     const R zaorig = za;
-    const R ql0 = LD(p.pclv, i5, jo) + ptsphy * LD(p.tcml_cld, i5, jo);
-    const R qi0 = LD(p.pclv, i5 + sstep, jo) + ptsphy * LD(p.tcml_cld, i5 + sstep, jo);
+    const R ql0 = pcl_l + ptsphy * tcml_l;
+    const R qi0 = pcl_i + ptsphy * tcml_i;
     const R qr0 = LD(p.pclv, i5 + 2 * sstep, jo) + ptsphy * LD(p.tcml_cld, i5 + 2 * sstep, jo);
     const R qs0 = LD(p.pclv, i5 + 3 * sstep, jo) + ptsphy * LD(p.tcml_cld, i5 + 3 * sstep, jo);
     R ql = ql0, qi = qi0, qr = qr0, qs = qs0;
@@ -924,38 +968,57 @@ void set_tuned_block(int variant, int klev, int block) {
     g_tuned_block[{variant, klev}] = block;
 }
 
-template <typename R, typename RD, typename KC, typename KCD>
+template <bool SAT, typename R, typename RD, typename KC, typename KCD>
 hipError_t launch_variant(int block, bool uni, unsigned ncols, const KC& c, const KCD& cd, const KPtrs<R>& p,
-                          const KDims& d, int ssc_off, hipStream_t stream) {
+                          const KDims& d, int ssc_off, const KSatPtrs<R>& s, hipStream_t stream) {
   if (!uni) {  // blocks not aligned to waves: per-lane block offsets, 256-lane groups
-    hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, false>), dim3((ncols + 255) / 256), dim3(256), 0,
-                       stream, c, cd, p, d, ssc_off);
+    hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, false, SAT>), dim3((ncols + 255) / 256), dim3(256),
+                       0, stream, c, cd, p, d, ssc_off, s);
     return hipGetLastError();
   }
   const unsigned grid = (ncols + block - 1) / block;
   switch (block) {
     case 64:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 64, true>), dim3(grid), dim3(64), 0, stream, c, cd, p,
-                         d, ssc_off);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 64, true, SAT>), dim3(grid), dim3(64), 0, stream, c, cd,
+                         p, d, ssc_off, s);
       break;
     case 128:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 128, true>), dim3(grid), dim3(128), 0, stream, c, cd,
-                         p, d, ssc_off);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 128, true, SAT>), dim3(grid), dim3(128), 0, stream, c,
+                         cd, p, d, ssc_off, s);
       break;
     case 512:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 512, true>), dim3(grid), dim3(512), 0, stream, c, cd,
-                         p, d, ssc_off);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 512, true, SAT>), dim3(grid), dim3(512), 0, stream, c,
+                         cd, p, d, ssc_off, s);
       break;
     default:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, true>), dim3((ncols + 255) / 256), dim3(256), 0,
-                         stream, c, cd, p, d, ssc_off);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, true, SAT>), dim3((ncols + 255) / 256), dim3(256),
+                         0, stream, c, cd, p, d, ssc_off, s);
   }
   return hipGetLastError();
 }
 
+template <typename R, typename RD, typename KC, typename KCD>
+hipError_t launch_either(bool sat, int block, bool uni, unsigned ncols, const KC& c, const KCD& cd,
+                         const KPtrs<R>& p, const KDims& d, int ssc_off, const KSatPtrs<R>& s, hipStream_t stream) {
+  return sat ? launch_variant<true, R, RD>(block, uni, ncols, c, cd, p, d, ssc_off, s, stream)
+             : launch_variant<false, R, RD>(block, uni, ncols, c, cd, p, d, ssc_off, s, stream);
+}
+
+template <typename R>
+KSatPtrs<R> offset_sat_ptrs(const KSatPtrs<R>& p, int64_t b0, int64_t nproma, int64_t klev) {
+  const int64_t s2 = nproma * klev;
+  KSatPtrs<R> q = p;
+  q.phrsw += b0 * s2; q.phrlw += b0 * s2; q.tvdf_t += b0 * s2; q.tvdf_q += b0 * s2; q.pvervel += b0 * s2;
+  q.tcml_t += b0 * s2; q.tcml_q += b0 * s2; q.tcml_a += b0 * s2; q.tcml_cld += b0 * s2 * 5;
+  if (q.tsat_t) {
+    q.tsat_t += b0 * s2; q.tsat_q += b0 * s2; q.tsat_a += b0 * s2; q.tsat_cld += b0 * s2 * 5;
+  }
+  return q;
+}
+
 template <typename R, typename RD>
 hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
-                          hipStream_t stream) {
+                          hipStream_t stream, const KSatPtrs<R>* sat) {
   const bool def = is_default_config(q);
   const int block = def ? tuned_block(variant_id<R, RD>(), d.klev) : 256;
   // every wave inside one block (64-lane waves, 64 | NPROMA): block offsets are uniform across the wave
@@ -969,16 +1032,17 @@ hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<
     dc.ngpblks = (int32_t)nb;
     dc.ngptot = (int32_t)(d.ngptot - b0 * d.nproma);
     const KPtrs<R> pc = offset_ptrs(p, b0, d.nproma, d.klev);
+    const KSatPtrs<R> sc = sat ? offset_sat_ptrs(*sat, b0, d.nproma, d.klev) : KSatPtrs<R>{};
     const unsigned ncols = (unsigned)(nb * d.nproma);
     hipError_t e;
     if (def) {
       const KConstDefault<R> c = make_kconst_default<R>(q, ptsphy);
       const KConstDefault<RD> cd = make_kconst_default<RD>(q, ptsphy);
-      e = launch_variant<R, RD>(block, uni, ncols, c, cd, pc, dc, ssc_off, stream);
+      e = launch_either<R, RD>(sat != nullptr, block, uni, ncols, c, cd, pc, dc, ssc_off, sc, stream);
     } else {
       const KConst<R> c = make_kconst<R>(q, ptsphy);
       const KConst<RD> cd = make_kconst<RD>(q, ptsphy);
-      e = launch_variant<R, RD>(256, uni, ncols, c, cd, pc, dc, ssc_off, stream);
+      e = launch_either<R, RD>(sat != nullptr, 256, uni, ncols, c, cd, pc, dc, ssc_off, sc, stream);
     }
     if (e != hipSuccess) return e;
   }
@@ -991,10 +1055,10 @@ template KConstDefault<float> make_kconst_default<float>(const cloudsc_params_t&
 template KConstDefault<double> make_kconst_default<double>(const cloudsc_params_t&, double);
 
 template hipError_t launch_cloudsc<double, double>(const cloudsc_params_t&, double, const KPtrs<double>&,
-                                                   const KDims&, int, hipStream_t);
+                                                   const KDims&, int, hipStream_t, const KSatPtrs<double>*);
 template hipError_t launch_cloudsc<float, float>(const cloudsc_params_t&, double, const KPtrs<float>&, const KDims&,
-                                                 int, hipStream_t);
+                                                 int, hipStream_t, const KSatPtrs<float>*);
 template hipError_t launch_cloudsc<float, double>(const cloudsc_params_t&, double, const KPtrs<float>&, const KDims&,
-                                                  int, hipStream_t);
+                                                  int, hipStream_t, const KSatPtrs<float>*);
 
 }  // namespace cloudsc_gpu

@@ -20,9 +20,6 @@
 #include "synth.h"
 
 namespace cloudsc_gpu {
-template <typename R, typename RD>
-hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
-                          hipStream_t stream);
 }
 
 using cloudsc_capi::fail;

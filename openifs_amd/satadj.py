@@ -141,3 +141,54 @@ def algorithmic_bytes(klev, ncldtop, word=4):
     five tendencies on all KLEV levels (zeros above NCLDTOP)."""
     active = klev - ncldtop + 1
     return word * (18 * active + (active + 1) + 5 * klev)
+
+
+# ---------------------------------------------------------------------------
+# fused CALLPAR sequence: CLOUD_SATADJ + TENDENCY_CML increment + TENDENCY_SATADJ copy + CLOUDSC, one launch
+# ---------------------------------------------------------------------------
+SATADJ_EXTRA_INPUTS = ["phrsw", "phrlw", "ptendency_vdf_t", "ptendency_vdf_q", "pvervel"]
+SATADJ_TENDENCY_OUT = [("ptendency_satadj_t", "2d"), ("ptendency_satadj_q", "2d"), ("ptendency_satadj_a", "2d"),
+                       ("ptendency_satadj_cld", "c5")]
+
+
+class FusedFields(C.BlockedFields):
+    """Every CLOUDSC blocked field plus the CLOUD_SATADJ inputs and the TENDENCY_SATADJ outputs, on one device."""
+
+    def __init__(self, ngptot, nproma, klev, precision="sp", device="cuda", fill=None, satadj_out=True):
+        import torch
+
+        super().__init__(ngptot, nproma, klev, precision, device, fill=fill)
+        for name in SATADJ_EXTRA_INPUTS:
+            self.t[name] = torch.empty(C.blocked_shape("2d", self.nproma, self.klev, self.ngpblks),
+                                       dtype=self.dtype, device=device)
+        if satadj_out:
+            for name, kind in SATADJ_TENDENCY_OUT:
+                shape = C.blocked_shape(kind, self.nproma, self.klev, self.ngpblks)
+                self.t[name] = (torch.full(shape, fill, dtype=self.dtype, device=device) if fill is not None
+                                else torch.empty(shape, dtype=self.dtype, device=device))
+
+
+def satadj_cloudsc_blocked(fields, params, ptsphy, stream=None):
+    """cloudsc_satadj_cloudsc_blocked_{dp,sp,mixed}_c over a FusedFields: the CML tendencies return incremented by
+    the saturation adjustment, TENDENCY_LOC holds CLOUDSC's result, TENDENCY_SATADJ the adjustment's."""
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream(fields.device).cuda_stream
+    fn = f"cloudsc_satadj_cloudsc_blocked_{fields.precision}_c"
+    ptr = lambda n: ctypes.c_void_p(fields[n].data_ptr() if n in fields.t else 0)
+    args = [ctypes.byref(params), ctypes.c_int(fields.ngptot), ctypes.c_int(fields.nproma), ctypes.c_int(fields.klev),
+            ctypes.c_int(fields.ngpblks), C._ctype_real(fields.precision)(ptsphy)]
+    args += C.blocked_args(fields.t, lambda x: x.data_ptr())
+    args += [ptr(n) for n in SATADJ_EXTRA_INPUTS]
+    args += [ptr(n) for n, _ in SATADJ_TENDENCY_OUT]
+    args.append(ctypes.c_void_p(stream))
+    C._check(fn, getattr(C.lib(), fn)(*args))
+
+
+def fused_bytes_per_column(klev, ncldtop, word=4):
+    """Algorithmic HBM bytes per column of the fused launch: CLOUDSC's 49*KLEV+20 words, plus on the active levels
+    the seven SATADJ-only inputs (PHRSW, PHRLW, two VDF tendencies, PVERVEL, PLUDELI(:,:,3:4)) and the five
+    incremented CML tendencies written back, plus TENDENCY_SATADJ (five fields, all levels)."""
+    active = klev - ncldtop + 1
+    return word * ((49 * klev + 20) + 7 * active + 5 * active + 5 * klev)

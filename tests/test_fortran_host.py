@@ -81,3 +81,41 @@ def test_fortran_host_fp32_equals_python_host(tmp_path, params137):
     C.CLOUDSC(params137, 1, klon, klon, klev, 900.0, ref, precision="sp")
     for n in out:
         np.testing.assert_array_equal(out[n], ref[n], err_msg=n)
+
+
+def run_satadj_driver(tmp_path, arrs, prec, klon, klev, kidia, kfdia):
+    from openifs_amd import satadj as S
+
+    d = str(tmp_path)
+    _write_common(d, klon, klev, kidia, kfdia)
+    for n, kind, role in S.SATADJ_ARRAYS:
+        if role != "u":
+            np.asfortranarray(arrs[n]).ravel(order="F").tofile(os.path.join(d, f"{n}.bin"))
+    subprocess.run([DRIVER, "satadj", d, prec], check=True, timeout=120)
+    return {n: np.fromfile(os.path.join(d, f"{n}.out"), dtype=arrs[n].dtype).reshape(arrs[n].shape, order="F")
+            for n in S.SATADJ_OUTPUTS}
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["dp", "sp", "mixed"])
+def test_fortran_host_satadj(tmp_path, params137, prec):
+    """CLOUD_SATADJ through CLOUDSC_SATADJ_*_C from Fortran equals the Python host bit for bit, and the FP64
+    variant equals the oracle to 1e-12."""
+    import oracle
+    from openifs_amd import satadj as S
+    from tests.satadj_cases import block_arrays, blocked_case
+
+    klon, klev = 48, 137
+    arrs = block_arrays(blocked_case(params137, 31, 2 * klon, klon, klev, "dp"), 1, prec)
+    out = run_satadj_driver(tmp_path, arrs, prec, klon, klev, 2, klon - 1)
+    ref = {k: v.copy(order="F") for k, v in arrs.items()}
+    S.CLOUD_SATADJ(params137, 2, klon - 1, klon, klev, 900.0, ref, precision=prec)
+    for n in out:
+        np.testing.assert_array_equal(out[n], ref[n], err_msg=n)
+    if prec == "dp":
+        orc = {k: v.copy(order="F") for k, v in arrs.items()}
+        oracle.satadj_block("dp", params137, 2, klon - 1, klon, klev, 900.0, orc)
+        for n in out:
+            np.testing.assert_array_equal(np.isnan(out[n]), np.isnan(orc[n]), err_msg=n)
+            assert relrms(out[n], orc[n]) <= 1e-12, n

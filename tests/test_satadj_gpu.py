@@ -157,3 +157,95 @@ def test_synth_forcings(params137):
     lde, lude = f1["pludeli"].double(), f1["plude"].double()
     torch.testing.assert_close(lde[:, 2], 1e-2 * lude, rtol=1e-6, atol=0)
     torch.testing.assert_close(lde[:, 3], 1e3 * lude, rtol=1e-6, atol=0)
+
+
+# ---------------------------------------------------------------------------
+# fused CALLPAR sequence (one launch) against the unfused sequence and the oracle
+# ---------------------------------------------------------------------------
+CML = ["ptendency_cml_t", "ptendency_cml_q", "ptendency_cml_a", "ptendency_cml_cld"]
+SATOUT = ["ptendency_satadj_t", "ptendency_satadj_q", "ptendency_satadj_a", "ptendency_satadj_cld"]
+
+
+def unfused_sequence(f, params, ptsphy=900.0):
+    """CLOUD_SATADJ, STATE_INCREMENT of the CML tendencies (callpar.F90:1278-1279), STATE_COPY to
+    TENDENCY_SATADJ (:1289) and CLOUDSC as separate steps on a FusedFields (the satadj tendencies go through
+    TENDENCY_LOC, as in CALLPAR)."""
+    S.satadj_blocked(f, params, ptsphy)
+    torch.cuda.synchronize()
+    sat = {"ptendency_satadj_t": f["ptendency_loc_t"].clone(), "ptendency_satadj_q": f["ptendency_loc_q"].clone(),
+           "ptendency_satadj_a": f["ptendency_loc_a"].clone(),
+           "ptendency_satadj_cld": f["ptendency_loc_cld"][:, :2].clone()}
+    f["ptendency_cml_t"].add_(sat["ptendency_satadj_t"])
+    f["ptendency_cml_q"].add_(sat["ptendency_satadj_q"])
+    f["ptendency_cml_a"].add_(sat["ptendency_satadj_a"])
+    f["ptendency_cml_cld"][:, :2].add_(sat["ptendency_satadj_cld"])
+    C.cloudsc_blocked(f, params, ptsphy)
+    torch.cuda.synchronize()
+    return sat
+
+
+@pytest.mark.parametrize("prec,ngptot,nproma,klev", [("dp", 2048, 64, 137), ("sp", 8192, 128, 137),
+                                                     ("mixed", 8192, 128, 137), ("sp", 4096, 64, 91)])
+def test_fused_equals_unfused_sequence(prec, ngptot, nproma, klev, params137, params91):
+    """One fused launch reproduces CLOUD_SATADJ + increment + copy + CLOUDSC bit for bit: every CLOUDSC output,
+    the incremented CML tendencies and TENDENCY_SATADJ."""
+    p = params137 if klev == 137 else params91
+    f = S.FusedFields(ngptot, nproma, klev, prec, "cuda", fill=float("nan"))
+    S.synth_device(f, p, 41)
+    g = S.FusedFields(ngptot, nproma, klev, prec, "cuda", fill=float("nan"))
+    for n in f.t:
+        g.t[n].copy_(f.t[n])
+    S.satadj_cloudsc_blocked(f, p, 900.0)
+    torch.cuda.synchronize()
+    sat = unfused_sequence(g, p)
+    for n in C.OUTPUT_NAMES + CML[:3]:
+        assert torch.equal(torch.nan_to_num(f[n], nan=7.0), torch.nan_to_num(g[n], nan=7.0)), n
+    assert torch.equal(f["ptendency_cml_cld"], g["ptendency_cml_cld"])
+    for n in SATOUT[:3]:
+        assert torch.equal(f[n], sat[n]), n
+    assert torch.equal(f["ptendency_satadj_cld"][:, :2], sat["ptendency_satadj_cld"])
+    assert torch.isnan(f["ptendency_satadj_cld"][:, 2:]).all()
+
+
+def test_fused_fp64_matches_oracle_sequence(params137):
+    """The fused FP64 launch against the oracle's CLOUD_SATADJ, the increment and the oracle's CLOUDSC."""
+    from tests.helpers import columns, oracle_blocked
+
+    p, ngptot, nproma, klev = params137, 1000, 64, 137
+    f = S.FusedFields(ngptot, nproma, klev, "dp", "cuda", fill=float("nan"))
+    S.synth_device(f, p, 43)
+    inp = {n: f[n].cpu().numpy().copy() for n in f.t}
+    S.satadj_cloudsc_blocked(f, p, 900.0)
+    torch.cuda.synchronize()
+    sref = oracle_run("dp", p, {n: inp[n] for n in INS}, ngptot, nproma, klev)
+    cin = {n: inp[n].copy() for n in C.INPUT_NAMES}
+    cin["ptendency_cml_t"] += np.nan_to_num(sref["ptendency_loc_t"])
+    cin["ptendency_cml_q"] += np.nan_to_num(sref["ptendency_loc_q"])
+    cin["ptendency_cml_a"] += np.nan_to_num(sref["ptendency_loc_a"])
+    cin["ptendency_cml_cld"][:, :2] += np.nan_to_num(sref["ptendency_loc_cld"][:, :2])
+    ref = oracle_blocked("dp", p, cin, ngptot, nproma, klev, nthreads=16)
+    for n in C.OUTPUT_NAMES:
+        g, r = columns(f[n].cpu().numpy(), nproma)[:ngptot], columns(ref[n], nproma)[:ngptot]
+        np.testing.assert_array_equal(np.isnan(g), np.isnan(r), err_msg=n)
+        assert relrms(g, r) <= 1e-12, n
+    for n in CML:
+        g, r = columns(f[n].cpu().numpy(), nproma)[:ngptot], columns(cin[n], nproma)[:ngptot]
+        assert relrms(g, r) <= 1e-12, n
+
+
+def test_fused_full_size_equals_unfused(params137):
+    """655,360 columns x L137 FP32 (the benchmark workload): fused = unfused bit for bit."""
+    p = params137
+    f = S.FusedFields(655360, 128, 137, "sp", "cuda", satadj_out=False)
+    S.synth_device(f, p, 45)
+    snap = {n: f[n].clone() for n in C.INPUT_NAMES + S.SATADJ_EXTRA_INPUTS}
+    S.satadj_cloudsc_blocked(f, p, 900.0)
+    torch.cuda.synchronize()
+    fused = {n: f[n].clone() for n in C.OUTPUT_NAMES + CML}
+    for n, x in snap.items():
+        f[n].copy_(x)
+    del snap
+    unfused_sequence(f, p)
+    for n in C.OUTPUT_NAMES + CML:
+        a, b = fused[n], f[n]
+        assert torch.equal(torch.nan_to_num(a, nan=7.0), torch.nan_to_num(b, nan=7.0)), n
