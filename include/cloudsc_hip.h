@@ -61,6 +61,7 @@ extern "C" {
 #define CLOUDSC_EHIP (-4)         /* a HIP runtime call failed */
 #define CLOUDSC_ENODEVICE (-5)    /* no gfx950 device visible */
 #define CLOUDSC_ENOMEM (-6)       /* device allocation failed */
+#define CLOUDSC_ESTATE (-7)       /* a field is not present where it is asked for (include/cloudsc_field.h) */
 
 /*
  * Flattened configuration: every TECLDP / TTHF / TCST / TECUMF / TERAD /
