@@ -119,3 +119,29 @@ def test_fortran_host_satadj(tmp_path, params137, prec):
         for n in out:
             np.testing.assert_array_equal(np.isnan(out[n]), np.isnan(orc[n]), err_msg=n)
             assert relrms(out[n], orc[n]) <= 1e-12, n
+
+
+@needs_driver
+@pytest.mark.gpu
+def test_fortran_block_loop_equals_step_on_fields(tmp_path, params137):
+    """From Fortran: the per-block CLOUDSC_SP_C loop over NGPBLKS blocks (the EC_PHYS_DRV pattern) and one
+    CLOUDSC_STEP_FIELDS_RUN on field wrappers around the same blocked arrays give identical results, including
+    the partial last block."""
+    ngptot, nproma, klev = 300, 128, 137
+    ngpblks = (ngptot + nproma - 1) // nproma
+    d = str(tmp_path)
+    _write_common(d, nproma, klev, 1, nproma)
+    np.array([ngptot, ngpblks], dtype=np.int32).tofile(os.path.join(d, "blocked.bin"))
+    inp = C.synth_host(params137, 91, ngptot, nproma, klev, precision="sp")
+    for n, kind, role in C.BLOCKED_ARRAYS:
+        shape = C.blocked_shape(kind, nproma, klev, ngpblks)
+        dt = np.int32 if kind == "1di" else np.float32
+        a = inp[n] if n in inp else np.full(shape, np.nan, dtype=dt)
+        np.ascontiguousarray(a, dtype=dt).tofile(os.path.join(d, f"{n}.bin"))
+    subprocess.run([DRIVER, "fields", d, "sp"], check=True, timeout=120)
+    for n, kind, role in C.BLOCKED_ARRAYS:
+        if role in ("o", "io"):
+            blk = np.fromfile(os.path.join(d, f"{n}.blk"), dtype=np.float32)
+            fld = np.fromfile(os.path.join(d, f"{n}.fld"), dtype=np.float32)
+            assert np.array_equal(blk, fld, equal_nan=True), n
+            assert np.isfinite(fld).any(), n
