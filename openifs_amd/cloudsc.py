@@ -147,6 +147,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise CloudscLibraryError(f"{LIB_PATH} is not built (run python -m openifs_amd.build)")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 (SONAME libamdhip64.so.7, needed as
+        # libamdhip64.so).  Loaded first, it also satisfies this library's libamdhip64.so.7, so torch tensors and
+        # streams and this library share one runtime.  Loaded after /opt/rocm's copy, it would start a second
+        # runtime that finds no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         try:
             _lib = ctypes.CDLL(LIB_PATH)
         except OSError as e:

@@ -151,3 +151,38 @@ def qsatmixadj(params, t, q, pp):
     lib().oracle_qsatmixadj_dp(ctypes.byref(params), ctypes.c_int(t.size), *(ctypes.c_void_p(x.ctypes.data)
                                                                              for x in (t, q, pp, out)))
     return out
+
+
+def supersatcheck_k(params, kftliqice, t, q, a, pp, psurf):
+    """CLOUD_SUPERSATCHECK in FP64 with an explicit KFTLIQICE; returns (t, q, a, l, i) adjustments."""
+    t, q, a, pp, psurf = (np.ascontiguousarray(x, dtype=np.float64) for x in (t, q, a, pp, psurf))
+    outs = [np.zeros(t.size) for _ in range(5)]
+    lib().oracle_supersatcheck_k_dp(ctypes.byref(params), ctypes.c_int(t.size), ctypes.c_int(kftliqice),
+                                    *(ctypes.c_void_p(x.ctypes.data) for x in (t, q, a, pp, psurf)),
+                                    *(ctypes.c_void_p(x.ctypes.data) for x in outs))
+    return outs
+
+
+def sltend_supersat_blocked(precision, params, ngptot, nproma, klev, tsphy, arrays, nthreads=0):
+    """SLTEND section 3 on blocked C-order numpy arrays (names of openifs_amd.supersat.SLTEND_ARRAYS; padj*
+    optional), updated in place."""
+    from openifs_amd.supersat import SLTEND_ARRAYS
+
+    ngpblks = (ngptot + nproma - 1) // nproma
+    dt = np.float64 if precision == "dp" else np.float32
+    real = ctypes.c_double if precision == "dp" else ctypes.c_float
+    args = [ctypes.byref(params), ctypes.c_int(ngptot), ctypes.c_int(nproma), ctypes.c_int(klev),
+            ctypes.c_int(ngpblks), real(tsphy)]
+    for n in SLTEND_ARRAYS:
+        a = arrays.get(n)
+        if a is None:
+            args.append(ctypes.c_void_p(0))
+            continue
+        if a.dtype != dt or not a.flags.c_contiguous:
+            raise TypeError(f"{n}: need a C-contiguous {np.dtype(dt).name} array")
+        args.append(ctypes.c_void_p(a.ctypes.data))
+    args.append(ctypes.c_int(nthreads))
+    rc = getattr(lib(), f"oracle_sltend_supersat_blocked_{precision}")(*args)
+    if rc != 0:
+        raise RuntimeError(f"oracle_sltend_supersat_blocked_{precision} returned {rc}")
+    return arrays

@@ -9,13 +9,15 @@
 //   oracle_satadj_{dp,sp,mixed}         one block of CLOUD_SATADJ, the argument list of
 //       include/cloudsc_satadj.h
 //   oracle_satadj_blocked_{dp,sp,mixed}  all blocks, OpenMP over blocks
-//   oracle_supersatcheck_*, oracle_lu5_dp, oracle_truncate_sinks_dp,
-//   oracle_thermo_dp, oracle_qsatmixadj_dp  unit-level entry points for the tests
+//   oracle_sltend_supersat_blocked_{dp,sp,mixed}  SLTEND's supersaturation adjustment, all blocks
+//   oracle_supersatcheck_*, oracle_supersatcheck_k_dp (with KFTLIQICE), oracle_lu5_dp,
+//   oracle_truncate_sinks_dp, oracle_thermo_dp, oracle_qsatmixadj_dp  unit-level entry points for the tests
 #include <omp.h>
 
 #include "../include/cloudsc_satadj.h"
 #include "cloudsc_oracle.hpp"
 #include "satadj_oracle.hpp"
+#include "sltend_oracle.hpp"
 
 namespace {
 
@@ -219,6 +221,37 @@ int satadj_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, 
                                phrlw, ptendency_vdf_t, ptendency_vdf_q, pludeli, pvervel, ptendency_loc_t,       \
                                ptendency_loc_q, ptendency_loc_a, ptendency_loc_cld, nthreads)
 
+template <typename R, typename RD>
+int sltend_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int ngpblks, R tsphy, const R* pt,
+                   const R* pq, const R* pa, const R* prsf1, const R* prs1, const R* pcent, const R* pcenq,
+                   const R* pcena, R* ptent, R* ptenq, R* ptena, R* ptencld, R* adja, R* adjl, R* adji,
+                   int nthreads) {
+  if (!p) return CLOUDSC_ENULL;
+  if (nproma < 1 || klev < 2 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma) return CLOUDSC_EDIMS;
+  if (p->ncldtop < 1 || p->ncldtop > klev) return CLOUDSC_EDIMS;
+  const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int ibl = 0; ibl < ngpblks; ++ibl) {
+    const int kfdia = std::min(nproma, ngptot - ibl * nproma);
+    const size_t o2 = ibl * s2, oh = ibl * sh, o5 = ibl * s2 * 5;
+    oracle::SltendArgs<R> a{1, kfdia, nproma, klev, tsphy, pt + o2, pq + o2, pa + o2, prsf1 + o2, prs1 + oh,
+                            pcent + o2, pcenq + o2, pcena + o2, ptent + o2, ptenq + o2, ptena + o2, ptencld + o5,
+                            adja ? adja + o2 : nullptr, adjl ? adjl + o2 : nullptr, adji ? adji + o2 : nullptr};
+    oracle::sltend_supersat<R, RD>(*p, a);
+  }
+  return CLOUDSC_OK;
+}
+
+#define ORACLE_SLTEND_ARGS(R)                                                                                  \
+  const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R tsphy, const R *pt,       \
+      const R *pq, const R *pa, const R *prsf1, const R *prs1, const R *pcent, const R *pcenq, const R *pcena, \
+      R *ptent, R *ptenq, R *ptena, R *ptencld, R *padja, R *padjl, R *padji, int nthreads
+
+#define ORACLE_SLTEND_CALL(R, RD)                                                                              \
+  return sltend_blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, tsphy, pt, pq, pa, prsf1, prs1, pcent, \
+                               pcenq, pcena, ptent, ptenq, ptena, ptencld, padja, padjl, padji, nthreads)
+
 }  // namespace
 
 extern "C" {
@@ -301,6 +334,18 @@ int oracle_satadj_mixed(CLOUDSC_SATADJ_ARGS(float)) { ORACLE_SATADJ_CALL(float, 
 int oracle_satadj_blocked_dp(ORACLE_SATADJ_BLOCKED_ARGS(double)) { ORACLE_SATADJ_BLOCKED_CALL(double, double); }
 int oracle_satadj_blocked_sp(ORACLE_SATADJ_BLOCKED_ARGS(float)) { ORACLE_SATADJ_BLOCKED_CALL(float, float); }
 int oracle_satadj_blocked_mixed(ORACLE_SATADJ_BLOCKED_ARGS(float)) { ORACLE_SATADJ_BLOCKED_CALL(float, double); }
+
+// CLOUD_SUPERSATCHECK on n points with an explicit KFTLIQICE (1 or 2).
+int oracle_supersatcheck_k_dp(const cloudsc_params_t* p, int n, int kftliqice, const double* t, const double* q,
+                              const double* a, const double* pp, const double* psurf, double* t_adj,
+                              double* q_adj, double* a_adj, double* l_adj, double* i_adj) {
+  oracle::supersatcheck<double, double>(*p, 1, n, t, q, a, pp, psurf, t_adj, q_adj, a_adj, l_adj, i_adj, kftliqice);
+  return 0;
+}
+
+int oracle_sltend_supersat_blocked_dp(ORACLE_SLTEND_ARGS(double)) { ORACLE_SLTEND_CALL(double, double); }
+int oracle_sltend_supersat_blocked_sp(ORACLE_SLTEND_ARGS(float)) { ORACLE_SLTEND_CALL(float, float); }
+int oracle_sltend_supersat_blocked_mixed(ORACLE_SLTEND_ARGS(float)) { ORACLE_SLTEND_CALL(float, double); }
 
 // QSATMIXADJ on n points: humidity after the two Newton iterations.
 int oracle_qsatmixadj_dp(const cloudsc_params_t* p, int n, const double* t, const double* q, const double* pp,

@@ -88,14 +88,17 @@ struct Thermo {  // fcttre.func.h / fccld.func.h statement functions, constants 
 };
 
 // ---------------------------------------------------------------------------
-// CLOUD_SUPERSATCHECK with KFTLIQICE = 1 (cloud_supersatcheck.F90:111-226),
-// computed in RD.  PP and PSURF are indexed from 1 as the callee sees the
-// sections PAP(KIDIA:KFDIA,JK), PAPH(KIDIA:KFDIA,KLEV+1) passed by CLOUDSC
-// (cloudsc.F90:3774-3777): pp[jl-1] is the caller's column KIDIA+jl-1.
+// CLOUD_SUPERSATCHECK (cloud_supersatcheck.F90:111-226), computed in RD.
+// KFTLIQICE = 1 splits the condensate by FOEALFA, 2 makes it all liquid where
+// PT+PT_ADJ >= RTHOMO and all ice below (:211-223).  PP and PSURF are indexed
+// from 1 as the callee sees the sections PAP(KIDIA:KFDIA,JK),
+// PAPH(KIDIA:KFDIA,KLEV+1) passed by CLOUDSC (cloudsc.F90:3774-3777): pp[jl-1]
+// is the caller's column KIDIA+jl-1.
 // ---------------------------------------------------------------------------
 template <typename R, typename RD>
 void supersatcheck(const cloudsc_params_t& p, int kidia, int kfdia, const RD* pt, const RD* pq, const RD* pa,
-                   const R* pp, const R* psurf, RD* pt_adj, RD* pq_adj, RD* pa_adj, RD* pl_adj, RD* pi_adj) {
+                   const R* pp, const R* psurf, RD* pt_adj, RD* pq_adj, RD* pa_adj, RD* pl_adj, RD* pi_adj,
+                   int kftliqice = 1) {
   const Thermo<RD> th(p);
   const RD ramid = RD(p.ramid), retv = RD(p.retv), rssicefactor = RD(p.rssicefactor);
   const RD zepsec = RD(1.e-14);
@@ -138,8 +141,14 @@ void supersatcheck(const cloudsc_params_t& p, int kidia, int kfdia, const RD* pt
     pt_adj[i] = th.foeldcpm(zt) * zcond;
     pq_adj[i] = -zcond;
     pa_adj[i] = zda;
-    pl_adj[i] = zcond * zfoealfa;
-    pi_adj[i] = zcond * (RD(1) - zfoealfa);
+    if (kftliqice == 2) {
+      const bool liquid = pt[i] + pt_adj[i] >= RD(p.rthomo);
+      pl_adj[i] = liquid ? zcond : RD(0);
+      pi_adj[i] = liquid ? RD(0) : zcond;
+    } else {
+      pl_adj[i] = zcond * zfoealfa;
+      pi_adj[i] = zcond * (RD(1) - zfoealfa);
+    }
   }
 }
 
