@@ -105,7 +105,7 @@ typedef struct cloudsc_params_t {
   int32_t lrad_cloud_inhomog; /* TEPHY: must be 0 (ice FSD / PFSD not built) */
   int32_t laeraware;        /* any of LAERLIQAUTOLSP/LAERLIQCOLL/LAERICESED/LAERICEAUTO: must be 0 */
   int32_t lspp;             /* TSPP_CONFIG%LSPP: must be 0 */
-  int32_t lcldbud;          /* any LCLDBUD* / LCLDBUDGET / LBUD23 diagnostics: must be 0 */
+  int32_t lcldbud;          /* budget diagnostics, CLOUDSC_BUD_* bits below (0 = all off, the default) */
   int32_t pad_;
 } cloudsc_params_t;
 
@@ -156,6 +156,21 @@ int cloudsc_mixed_c(CLOUDSC_ARGS(float));
  * KFDIA = MIN(NPROMA, NGPTOT-(IBL-1)*NPROMA) (ec_phys_drv.F90:441-465).
  * stream: a hipStream_t (NULL = the null stream).  The call is asynchronous.
  * ------------------------------------------------------------------------- */
+/* Bits of cloudsc_params_t::lcldbud: the LCLDBUD* switches of YDECLDP (cloudsc.F90:706-713), LBUD23 of YDEPHY
+ * (PEXTRA fields start after 26, :3832-3836) and LCLDBUDGET (its check, :1032-1082 and :4146-4212, has no output,
+ * so the bit is accepted and changes nothing).  The C, L, I, T and VERTINT budgets write PEXTRA: they are honoured
+ * by the per-block entries (PEXTRA and KFLDX are in the argument list) and by cloudsc_blocked_budget_*_c, and
+ * rejected with CLOUDSC_EUNSUPPORTED by the other entries.  VERTINT needs KLEV >= 83 (the slots the reference
+ * writes) and all of them need KFLDX >= the number of fields they fill. */
+#define CLOUDSC_BUD_C 1         /* LCLDBUDC: cloud fraction budget, 12 fields */
+#define CLOUDSC_BUD_L 2         /* LCLDBUDL: cloud liquid budget, 22 fields */
+#define CLOUDSC_BUD_I 4         /* LCLDBUDI: cloud ice budget, 20 fields */
+#define CLOUDSC_BUD_T 8         /* LCLDBUDT: cloud process temperature budget, 13 fields */
+#define CLOUDSC_BUD_VERTINT 16  /* LCLDBUD_VERTINT: vertical integrals in one field */
+#define CLOUDSC_BUD_TIMEINT 32  /* LCLDBUD_TIMEINT: accumulate rate * PTSPHY instead of the rate */
+#define CLOUDSC_BUD_LBUD23 64   /* LBUD23: the first 26 PEXTRA fields belong to other budgets */
+#define CLOUDSC_BUD_CHECK 128   /* LCLDBUDGET */
+
 #define CLOUDSC_BLOCKED_ARGS(R)                                                                         \
   const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,            \
       const R *pt, const R *pq, const R *ptendency_cml_t, const R *ptendency_cml_q,                     \
@@ -170,6 +185,27 @@ int cloudsc_mixed_c(CLOUDSC_ARGS(float));
 int cloudsc_blocked_dp_c(CLOUDSC_BLOCKED_ARGS(double));
 int cloudsc_blocked_sp_c(CLOUDSC_BLOCKED_ARGS(float));
 int cloudsc_blocked_mixed_c(CLOUDSC_BLOCKED_ARGS(float));
+
+/* Blocked entry with the budget diagnostics (params->lcldbud): the arguments of cloudsc_blocked_*_c, then the
+ * inputs only the budgets read (PVFA, PDYNA, PDYNL, PDYNI, PDYNS, PHRSW, PHRLW as (NPROMA,KLEV,NGPBLKS);
+ * KPBLTYPE, KCTOP as (NPROMA,NGPBLKS)) and PEXTRA (NPROMA,KLEV,KFLDX,NGPBLKS), INOUT: the budgets accumulate into
+ * it as the reference does.  Replaces the CLOUDSC call of CLOUD_LAYER in diagnostic runs. */
+#define CLOUDSC_BLOCKED_BUDGET_ARGS(R)                                                                  \
+  const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,            \
+      const R *pt, const R *pq, const R *ptendency_cml_t, const R *ptendency_cml_q,                     \
+      const R *ptendency_cml_a, const R *ptendency_cml_cld, R *ptendency_loc_t, R *ptendency_loc_q,     \
+      R *ptendency_loc_a, R *ptendency_loc_cld, const R *pvfl, const R *pvfi, const R *pap,             \
+      const R *paph, const R *plsm, const int32_t *ldcum, const int32_t *ktype, R *peis,                \
+      const R *plu, R *plude, const R *pludeli, const R *psnde, const R *pmfu, const R *pmfd,           \
+      const R *pa, const R *pclv, R *pcovptot, R *prainfrac_toprfz, R *pfsqlf, R *pfsqif,              \
+      R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur,   \
+      R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, const R *pvfa, const R *pdyna, const R *pdynl,        \
+      const R *pdyni, const R *pdyns, const R *phrsw, const R *phrlw, const int32_t *kpbltype,          \
+      const int32_t *kctop, R *pextra, int kfldx, void *stream
+
+int cloudsc_blocked_budget_dp_c(CLOUDSC_BLOCKED_BUDGET_ARGS(double));
+int cloudsc_blocked_budget_sp_c(CLOUDSC_BLOCKED_BUDGET_ARGS(float));
+int cloudsc_blocked_budget_mixed_c(CLOUDSC_BLOCKED_BUDGET_ARGS(float));
 
 /* Last HIP error text of the calling thread (empty string if none). */
 const char *cloudsc_last_error(void);

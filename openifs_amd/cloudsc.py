@@ -371,3 +371,46 @@ def cloudsc_blocked(fields, params, ptsphy, stream=None):
     args += blocked_args(fields.t, lambda x: x.data_ptr())
     args.append(ctypes.c_void_p(stream))
     _check(fn, f(*args))
+
+
+# ---------------------------------------------------------------------------
+# budget diagnostics (LCLDBUD*) into PEXTRA
+# ---------------------------------------------------------------------------
+# Bits of cloudsc_params_t.lcldbud: the LCLDBUD* switches (cloudsc.F90:706-713), LBUD23 and LCLDBUDGET.
+BUD_C, BUD_L, BUD_I, BUD_T, BUD_VERTINT, BUD_TIMEINT, BUD_LBUD23, BUD_CHECK = 1, 2, 4, 8, 16, 32, 64, 128
+# Arrays only the budgets read, in the argument order of cloudsc_blocked_budget_*_c (PEXTRA and KFLDX follow).
+BUDGET_ARRAYS = [("pvfa", "2d"), ("pdyna", "2d"), ("pdynl", "2d"), ("pdyni", "2d"), ("pdyns", "2d"),
+                 ("phrsw", "2d"), ("phrlw", "2d"), ("kpbltype", "1di"), ("kctop", "1di")]
+
+
+def budget_fields(flags):
+    """PEXTRA fields the switches fill (IS after the last budget block, cloudsc.F90:3832-4101)."""
+    n = 26 if flags & BUD_LBUD23 else 0
+    for bit, k in ((BUD_VERTINT, 1), (BUD_C, 12), (BUD_L, 22), (BUD_I, 20), (BUD_T, 13)):
+        if flags & bit:
+            n += k
+    return n
+
+
+def extra_shape(nproma, klev, kfldx, ngpblks):
+    """C-order shape of the blocked PEXTRA (Fortran (NPROMA,KLEV,KFLDX,NGPBLKS))."""
+    return (ngpblks, kfldx, klev, nproma)
+
+
+def cloudsc_blocked_budget(fields, budget, params, ptsphy, stream=None):
+    """One launch of cloudsc_blocked_budget_{dp,sp,mixed}_c: ``fields`` a BlockedFields, ``budget`` a dict of
+    device tensors with the BUDGET_ARRAYS and ``pextra`` (extra_shape), which the budgets accumulate into."""
+    import torch
+
+    if stream is None:
+        stream = torch.cuda.current_stream(fields.device).cuda_stream
+    fn = f"cloudsc_blocked_budget_{fields.precision}_c"
+    px = budget["pextra"]
+    if px.dim() != 4 or (px.shape[0], px.shape[2], px.shape[3]) != (fields.ngpblks, fields.klev, fields.nproma):
+        raise ValueError(f"pextra: need shape (NGPBLKS, KFLDX, KLEV, NPROMA), got {tuple(px.shape)}")
+    args = [ctypes.byref(params), ctypes.c_int(fields.ngptot), ctypes.c_int(fields.nproma), ctypes.c_int(fields.klev),
+            ctypes.c_int(fields.ngpblks), _ctype_real(fields.precision)(ptsphy)]
+    args += blocked_args(fields.t, lambda x: x.data_ptr())
+    args += [ctypes.c_void_p(budget[n].data_ptr()) for n, _ in BUDGET_ARRAYS]
+    args += [ctypes.c_void_p(px.data_ptr()), ctypes.c_int(px.shape[1]), ctypes.c_void_p(stream)]
+    _check(fn, getattr(lib(), fn)(*args))

@@ -39,11 +39,28 @@ int fail(int code, const char* what, hipError_t e) {
   return code;
 }
 
-int check_config(const cloudsc_params_t* p) {
+int check_config(const cloudsc_params_t* p, bool budget) {
   if (!p) return fail(CLOUDSC_ENULL, "cloudsc: NULL parameter block");
-  if (p->lcloud_inhomog || p->lrad_cloud_inhomog || p->laeraware || p->lspp || p->lcldbud)
+  if (p->lcloud_inhomog || p->lrad_cloud_inhomog || p->laeraware || p->lspp)
+    return fail(CLOUDSC_EUNSUPPORTED, "cloudsc: LCLOUD_INHOMOG/LRAD_CLOUD_INHOMOG/aerosol/SPP are not implemented");
+  const int32_t out = CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T | CLOUDSC_BUD_VERTINT;
+  if ((p->lcldbud & out) && !budget)
     return fail(CLOUDSC_EUNSUPPORTED,
-                "cloudsc: LCLOUD_INHOMOG/LRAD_CLOUD_INHOMOG/aerosol/SPP/budget switches are not implemented");
+                "cloudsc: the budget diagnostics write PEXTRA: use the per-block entry or cloudsc_blocked_budget_*_c");
+  if (p->lcldbud & ~255) return fail(CLOUDSC_EUNSUPPORTED, "cloudsc: unknown budget switch bits");
+  return CLOUDSC_OK;
+}
+
+// KFLDX and KLEV for the budget switches in force; NULL arrays only when no budget is written.
+int check_budget(const cloudsc_params_t* p, int klev, int kfldx, const void* const* arrays, int narrays) {
+  const int32_t out = CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T | CLOUDSC_BUD_VERTINT;
+  if (!(p->lcldbud & out)) return CLOUDSC_OK;
+  for (int i = 0; i < narrays; ++i)
+    if (!arrays[i]) return fail(CLOUDSC_ENULL, "cloudsc: NULL array the budget diagnostics read or write");
+  if (kfldx < cloudsc_gpu::budget_fields(p->lcldbud))
+    return fail(CLOUDSC_EDIMS, "cloudsc: KFLDX is smaller than the PEXTRA fields the budget switches fill");
+  if ((p->lcldbud & CLOUDSC_BUD_VERTINT) && klev < 83)
+    return fail(CLOUDSC_EDIMS, "cloudsc: the vertical-integral budget needs KLEV >= 83");
   return CLOUDSC_OK;
 }
 
@@ -71,13 +88,15 @@ int check_device() {
 namespace {
 
 using cloudsc_capi::check_device;
+using cloudsc_capi::check_budget;
 using cloudsc_capi::check_config;
 using cloudsc_capi::fail;
 
 template <typename R, typename RD>
 int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngpblks, R ptsphy, const KPtrs<R>& p,
-            void* stream, const cloudsc_gpu::KSatPtrs<R>* sat = nullptr) {
-  int rc = check_config(q);
+            void* stream, const cloudsc_gpu::KSatPtrs<R>* sat = nullptr,
+            const cloudsc_gpu::KBudPtrs<R>* bud = nullptr) {
+  int rc = check_config(q, bud != nullptr);
   if (rc) return rc;
   if (nproma < 1 || klev < 2 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma)
     return fail(CLOUDSC_EDIMS, "cloudsc_blocked: need NGPBLKS = ceil(NGPTOT/NPROMA), KLEV >= 2");
@@ -98,10 +117,19 @@ int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngp
     if (any && !all)
       return fail(CLOUDSC_ENULL, "cloudsc_satadj_cloudsc_blocked: TENDENCY_SATADJ arrays must be all set or all NULL");
   }
+  const bool budgets = bud && (q->lcldbud & (CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T |
+                                             CLOUDSC_BUD_VERTINT));
+  if (bud) {
+    const void* breq[] = {bud->pvfa, bud->pdyna, bud->pdynl, bud->pdyni, bud->pdyns, bud->phrsw, bud->phrlw,
+                          bud->kpbltype, bud->kctop, bud->pextra};
+    rc = check_budget(q, klev, bud->kfldx, breq, 10);
+    if (rc) return rc;
+  }
   rc = check_device();
   if (rc) return rc;
   KDims d{ngptot, nproma, klev, ngpblks, 1, nproma};
-  hipError_t e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, 0, (hipStream_t)stream, sat);
+  hipError_t e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, 0, (hipStream_t)stream, sat,
+                                                    budgets ? bud : nullptr);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_blocked: kernel launch", e);
   return CLOUDSC_OK;
 }
@@ -145,8 +173,10 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
               const int32_t* ldcum, const int32_t* ktype, R* peis, const R* plu, R* plude, const R* pludeli,
               const R* psnde, const R* pmfu, const R* pmfd, const R* pa, const R* pclv, R* pcovptot,
               R* prainfrac_toprfz, R* pfsqlf, R* pfsqif, R* pfcqnng, R* pfcqlng, R* pfsqrf, R* pfsqsf, R* pfcqrng,
-              R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn, R* pfhpsl, R* pfhpsn) {
-  int rc = check_config(q);
+              R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn, R* pfhpsl, R* pfhpsn, const R* pvfa,
+              const R* pdyna, const R* pdynl, const R* pdyni, const R* pdyns, const R* phrsw, const R* phrlw,
+              const int32_t* kpbltype, const int32_t* kctop, R* pextra, int kfldx) {
+  int rc = check_config(q, true);
   if (rc) return rc;
   if (klon < 1 || klev < 2 || kidia < 1 || kfdia < kidia || kfdia > klon || kidia + kfdia - 1 > klon)
     return fail(CLOUDSC_EDIMS, "cloudsc: need 1 <= KIDIA <= KFDIA <= KLON and KIDIA+KFDIA-1 <= KLON");
@@ -157,6 +187,13 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
                        pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn};
   for (const void* x : req)
     if (!x) return fail(CLOUDSC_ENULL, "cloudsc: NULL array on the default path");
+  const bool budgets = q->lcldbud & (CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T |
+                                     CLOUDSC_BUD_VERTINT);
+  {
+    const void* breq[] = {pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra};
+    rc = check_budget(q, klev, kfldx, breq, 10);
+    if (rc) return rc;
+  }
   rc = check_device();
   if (rc) return rc;
 
@@ -188,7 +225,15 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
                 {pfcqrng, pfcqrng, nh, sizeof(R)},     {pfcqsng, pfcqsng, nh, sizeof(R)},
                 {pfsqltur, pfsqltur, nh, sizeof(R)},   {pfsqitur, pfsqitur, nh, sizeof(R)},
                 {pfplsl, pfplsl, nh, sizeof(R)},       {pfplsn, pfplsn, nh, sizeof(R)},
-                {pfhpsl, pfhpsl, nh, sizeof(R)},       {pfhpsn, pfhpsn, nh, sizeof(R)}};
+                {pfhpsl, pfhpsl, nh, sizeof(R)},       {pfhpsn, pfhpsn, nh, sizeof(R)},
+                // budget diagnostics (copied only when a budget is written)
+                {pvfa, nullptr, budgets ? n2 : 0, sizeof(R)},   {pdyna, nullptr, budgets ? n2 : 0, sizeof(R)},
+                {pdynl, nullptr, budgets ? n2 : 0, sizeof(R)},  {pdyni, nullptr, budgets ? n2 : 0, sizeof(R)},
+                {pdyns, nullptr, budgets ? n2 : 0, sizeof(R)},  {phrsw, nullptr, budgets ? n2 : 0, sizeof(R)},
+                {phrlw, nullptr, budgets ? n2 : 0, sizeof(R)},
+                {kpbltype, nullptr, budgets ? n1 : 0, sizeof(int32_t)},
+                {kctop, nullptr, budgets ? n1 : 0, sizeof(int32_t)},
+                {pextra, budgets ? pextra : nullptr, budgets ? n2 * (size_t)kfldx : 0, sizeof(R)}};
   const int narr = sizeof(arrs) / sizeof(arrs[0]);
   size_t off[sizeof(arrs) / sizeof(arrs[0])];
   size_t total = 0;
@@ -202,7 +247,8 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
   char* base = (char*)w->dev;
   hipError_t e = hipSuccess;
   for (int i = 0; i < narr && e == hipSuccess; ++i)
-    e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * arrs[i].esz, hipMemcpyHostToDevice, w->stream);
+    if (arrs[i].n)
+      e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * arrs[i].esz, hipMemcpyHostToDevice, w->stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: copy to device", e);
   auto D = [&](int i) { return (R*)(base + off[i]); };
   KPtrs<R> p;
@@ -216,10 +262,18 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
   p.pfplsn = D(39); p.pfhpsl = D(40); p.pfhpsn = D(41);
   // one block: KFDIA bounds the active columns, KIDIA-1 shifts the pressure section of the supersaturation check
   KDims d{kfdia, klon, klev, 1, kidia, kfdia};
-  e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, kidia - 1, w->stream);
+  cloudsc_gpu::KBudPtrs<R> bp{};
+  if (budgets) {
+    bp.pvfa = D(42); bp.pdyna = D(43); bp.pdynl = D(44); bp.pdyni = D(45); bp.pdyns = D(46); bp.phrsw = D(47);
+    bp.phrlw = D(48); bp.kpbltype = (const int32_t*)D(49); bp.kctop = (const int32_t*)D(50); bp.pextra = D(51);
+    bp.pvfl = p.pvfl; bp.pvfi = p.pvfi;
+    bp.kfldx = kfldx; bp.flags = q->lcldbud; bp.reisthsc = R(q->reisthsc); bp.ralfdcp = R(q->ralfdcp);
+  }
+  e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, kidia - 1, w->stream, nullptr,
+                                         budgets ? &bp : nullptr);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: kernel launch", e);
   for (int i = 0; i < narr && e == hipSuccess; ++i)
-    if (arrs[i].host_out)
+    if (arrs[i].host_out && arrs[i].n)
       e = hipMemcpyAsync(arrs[i].host_out, base + off[i], arrs[i].n * arrs[i].esz, hipMemcpyDeviceToHost, w->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: copy to host", e);
@@ -249,16 +303,27 @@ KPtrs<R> blocked_ptrs(const R* pt, const R* pq, const R* tcml_t, const R* tcml_q
 }  // namespace
 
 #define PER_BLOCK_CALL(R, RD)                                                                                   \
-  (void)lddiag, (void)pvfa, (void)pdyna, (void)pdynl, (void)pdyni, (void)pdynr, (void)pdyns, (void)phrsw,       \
-      (void)phrlw, (void)pvervel, (void)pgeom, (void)pgaw, (void)kctop, (void)kpbltype, (void)pgp2dspp,           \
-      (void)ldslphy, (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice, (void)pfsd,      \
-      (void)pextra, (void)kfldx;                                                                                  \
+  (void)lddiag, (void)pdynr, (void)pvervel, (void)pgeom, (void)pgaw, (void)pgp2dspp, (void)ldslphy,              \
+      (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice, (void)pfsd;                   \
   return per_block<R, RD>(ydparams, kidia, kfdia, klon, klev, ptsphy, pt, pq, ptendency_cml_t, ptendency_cml_q,  \
                           ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a,   \
                           ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis, plu, plude, pludeli, \
                           psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng,        \
                           pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl,   \
-                          pfhpsn)
+                          pfhpsn, pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx)
+
+#define BUDGET_CALL(R, RD)                                                                                       \
+  cloudsc_gpu::KBudPtrs<R> bud{pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, pvfl, pvfi, kpbltype, kctop,        \
+                               pextra, kfldx, ydparams ? ydparams->lcldbud : 0,                                  \
+                               R(ydparams ? ydparams->reisthsc : 0.0), R(ydparams ? ydparams->ralfdcp : 0.0)};    \
+  return blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy,                                          \
+                        blocked_ptrs<R>(pt, pq, ptendency_cml_t, ptendency_cml_q, ptendency_cml_a,                \
+                                        ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a,     \
+                                        ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis, plu,  \
+                                        plude, pludeli, psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz,  \
+                                        pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng,       \
+                                        pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn),                      \
+                        stream, nullptr, &bud)
 
 #define BLOCKED_CALL(R, RD)                                                                                      \
   return blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy,                                          \
@@ -288,6 +353,10 @@ extern "C" {
 int cloudsc_satadj_cloudsc_blocked_dp_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(double)) { FUSED_CALL(double, double); }
 int cloudsc_satadj_cloudsc_blocked_sp_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(float)) { FUSED_CALL(float, float); }
 int cloudsc_satadj_cloudsc_blocked_mixed_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(float)) { FUSED_CALL(float, double); }
+
+int cloudsc_blocked_budget_dp_c(CLOUDSC_BLOCKED_BUDGET_ARGS(double)) { BUDGET_CALL(double, double); }
+int cloudsc_blocked_budget_sp_c(CLOUDSC_BLOCKED_BUDGET_ARGS(float)) { BUDGET_CALL(float, float); }
+int cloudsc_blocked_budget_mixed_c(CLOUDSC_BLOCKED_BUDGET_ARGS(float)) { BUDGET_CALL(float, double); }
 
 int cloudsc_dp_c(CLOUDSC_ARGS(double)) { PER_BLOCK_CALL(double, double); }
 int cloudsc_sp_c(CLOUDSC_ARGS(float)) { PER_BLOCK_CALL(float, float); }

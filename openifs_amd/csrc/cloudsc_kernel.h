@@ -25,6 +25,7 @@
 #include <limits>
 
 #include "../../include/cloudsc_hip.h"
+#include "cloudsc_budget.h"
 #include "cloudsc_defaults.h"
 
 namespace cloudsc_gpu {
@@ -135,9 +136,10 @@ struct KDims {
 
 // Blocked launch over all blocks (cloudsc_kernel.hip): splits the grid so that every array of one launch spans
 // less than 4 GiB.  ssc_off: KIDIA-1 offset of CLOUD_SUPERSATCHECK's pressure argument (per-block entry).
-// sat: NULL for CLOUDSC alone, else the fused saturation adjustment + CLOUDSC.
+// sat: NULL for CLOUDSC alone, else the fused saturation adjustment + CLOUDSC.  bud: NULL, or the budget inputs and
+// PEXTRA for the budget-instrumented kernel (cloudsc_budget.h; not combined with sat).
 template <typename R, typename RD>
 hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
-                          hipStream_t stream, const KSatPtrs<R>* sat = nullptr);
+                          hipStream_t stream, const KSatPtrs<R>* sat = nullptr, const KBudPtrs<R>* bud = nullptr);
 
 }  // namespace cloudsc_gpu

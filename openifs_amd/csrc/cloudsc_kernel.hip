@@ -9,6 +9,7 @@
 
 #include "cloudsc_device.h"
 #include "cloudsc_kernel.h"
+#include "cloudsc_budget.h"
 #include "satadj_device.h"
 
 namespace cloudsc_gpu {
@@ -29,9 +30,11 @@ __device__ __forceinline__ T trunc_scale(T x, int jo, int a, int b, T zz) {
 // CLOUD_SATADJ for the point (satadj_point), adds its tendencies to TENDENCY_CML in place (STATE_INCREMENT,
 // :1278-1279), stores them as TENDENCY_SATADJ (STATE_COPY, :1289), and CLOUDSC then reads the incremented CML
 // tendencies from registers.
-template <typename R, typename RD, typename KC, typename KCD, int BLOCK, bool UNI, bool SAT>
+// BUD: the process budgets (LCLDBUD*) are recorded per level and written into PEXTRA (cloudsc_budget.h).
+template <typename R, typename RD, typename KC, typename KCD, int BLOCK, bool UNI, bool SAT, bool BUD>
 __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p,
-                                                                const KDims d, int ssc_off, const KSatPtrs<R> s) {
+                                                                const KDims d, int ssc_off, const KSatPtrs<R> s,
+                                                                const KBudPtrs<R> bp) {
   const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
   const uint32_t nproma = (uint32_t)d.nproma, klev = (uint32_t)d.klev;
   if (g >= (uint32_t)d.ngpblks * nproma) return;
@@ -224,6 +227,8 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
 
     if ((int)k >= ncldtop0) {
       const bool above_top = (int)k > ncldtop0;  // JK > NCLDTOP
+      Bud<R, BUD> bud;  // ZBUDCC/ZBUDL/ZBUDI of this level (:1271-1273)
+      R zrainaut = R(0), zrainacc = R(0), zsnowaut = R(0), zsnowrime = R(0);  // (:1239-1242)
       const R plude_in = LD(p.plude, i2, jo);
       // derived variables (:1281-1324)
       const R zdp = paph_k1 - paph_k;
@@ -275,6 +280,9 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           csrc_i = pludeli2 * zdtgdp;
           dl = dl + csrc_l;
           di = di + csrc_i;
+          bud.set(L3, csrc_l * zqtmst);  // (:1378-1380)
+          bud.set(I3, csrc_i * zqtmst);
+          bud.set(CC3, m_div(zqtmst * plude, plu_n));
         } else {
           plude = R(0);
         }
@@ -306,6 +314,8 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           zlfinalsum = zlfinalsum + zlfinal;
           dl = dl + lcust_l;
           vl = vl + zevap;
+          bud.set(L4, lcust_l * zqtmst);  // (:1459-1460)
+          bud.set(L5, -zevap * zqtmst);
         }
         {
           R zlfinal = m_max(R(0), lcust_i - zdqs);
@@ -314,9 +324,15 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           zlfinalsum = zlfinalsum + zlfinal;
           di = di + lcust_i;
           vi = vi + zevap;
+          bud.set(I4, lcust_i * zqtmst);  // (:1475-1476)
+          bud.set(I5, -zevap * zqtmst);
         }
         zsolac = zsolac + zacust;
-        if (zlfinalsum < zepsec) zsolac = zsolac - zacust;
+        bud.set(CC4, zacust * zqtmst);  // (:1484-1488)
+        if (zlfinalsum < zepsec) {
+          zsolac = zsolac - zacust;
+          bud.set(CC5, -zacust * zqtmst);
+        }
       }
       // subsidence sink to the level below, implicit (:1499-1514)
       if (has_next) {
@@ -345,6 +361,9 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           zsolac = zsolac - zaeros;
           vl = vl + zliqfrac * zleros;
           vi = vi + zicefrac * zleros;
+          bud.set(L7, -zliqfrac * zleros * zqtmst);  // (:1749-1751)
+          bud.set(I7, -zicefrac * zleros * zqtmst);
+          bud.set(CC7, -zaeros * zqtmst);
         }
       }
 
@@ -357,6 +376,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       if (above_top) {
         fsrc_i = pf_i * zdtgdp;
         di = di + fsrc_i;
+        bud.set(I12, fsrc_i * zqtmst);  // (:1864-1866)
         qxfg_i = qxfg_i + fsrc_i;
         zqpretot = zqpretot + qxfg_i;
         fsrc_r = pf_r * zdtgdp;
@@ -390,9 +410,8 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
 
       // 4.3 warm rain, implicit KK2000 (:2050-2254)
       if (zliqcld > zepsec) {
-        const R zrainaut = c.c_aut * m_pow(zliqcld, c.rcl_kkbauq - R(1)) * zconst_pow;
+        zrainaut = c.c_aut * m_pow(zliqcld, c.rcl_kkbauq - R(1)) * zconst_pow;
         qbrl = qbrl + zrainaut;
-        R zrainacc = R(0);
         if (zraincld > zepsec)
           zrainacc = c.c_acc * m_pow(zliqcld, c.rcl_kkbac - R(1)) * m_pow(zraincld, c.rcl_kkbac);
         qbrl = qbrl + zrainacc;
@@ -428,6 +447,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           const R zevap = m_min(zdpevap, qxfg_r);
           vr = vr + zevap;
           qxfg_r = qxfg_r - zevap;
+          bud.set(L20, -zevap * zqtmst);  // (:2363)
         }
       }
 
@@ -455,6 +475,8 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           il = il + zdepos;
           qxfg_i = qxfg_i + zdepos;
           qxfg_l = qxfg_l - zdepos;
+          bud.set(L11, -zdepos * zqtmst);  // (:2495-2496)
+          bud.set(I11, zdepos * zqtmst);
         }
       }
 
@@ -472,7 +494,8 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
                            m_div(c.rcl_const3s * m_sqrt(zcorrfac) * m_sqrt(zrho) * m_pow(zpr02, c.rcl_const5s), m_sqrt(zcorrfac2));
           R zpsdep = m_max(zcovptot * zterm1 * zterm2 * ptsphy, R(0));
           zpsdep = m_div(m_min(zpsdep, qv - zqsmix), zcorqsmix);
-          vs = vs - zpsdep;  // ZSOLQA(S,V) += ZPSDEP, ZSOLQA(V,S) -= ZPSDEP
+          vs = vs - zpsdep;
+          bud.set(PSDEP, zpsdep);  // ZSOLQA(S,V) += ZPSDEP, ZSOLQA(V,S) -= ZPSDEP
         }
       }
 
@@ -485,7 +508,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
         const R zzco = ptsphy * c.rsnowlin1 * m_exp(c.rsnowlin2 * (ztp1 - c.rtt));
         const R r = m_div(zicecld, c.rlcritsnow);
         const R ztemp = R(-1.) * (r * r);
-        const R zsnowaut = (ztemp < R(50.0)) ? zzco * (R(1) - m_exp(ztemp)) : zzco;
+        zsnowaut = (ztemp < R(50.0)) ? zzco * (R(1) - m_exp(ztemp)) : zzco;
         qbsi = qbsi + zsnowaut;
       }
 
@@ -505,11 +528,14 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           qxfg_i = qxfg_i - zmelt;
           qxfg_r = qxfg_r + zmelt;
           ri = ri + zmelt;  // ZSOLQA(R,I) += melt
+          bud.set(I15, -zmelt * zqtmst);  // (:2856-2858)
+          bud.set(L17, zmelt * zqtmst);
           zalfa = m_div(qxfg_s, zicetot);
           zmelt = m_min(qxfg_s, zalfa * zmeltmax);
           qxfg_s = qxfg_s - zmelt;
           qxfg_r = qxfg_r + zmelt;
           rs = rs + zmelt;  // ZSOLQA(R,S) += melt
+          bud.set(I16, -zmelt * zqtmst);
         }
       }
 
@@ -534,6 +560,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           if (zfrzmax > zepsec) {
             const R zfrz = m_min(qr, zfrzmax);
             rs = rs - zfrz;  // ZSOLQA(S,R) += frz
+            bud.set(L18, -zfrz * zqtmst);  // (:2927)
           }
         }
       }
@@ -544,6 +571,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
         if (zfrzmax > zepsec && qxfg_l > zepsec) {
           const R zfrz = m_min(qxfg_l, zfrzmax);
           il = il + zfrz;
+          bud.set(L19, -zfrz * zqtmst);  // (:2958)
         }
       }
 
@@ -551,7 +579,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       if (ztp1 <= c.rtt && zliqcld > zepsec) {
         if (zsnowcld > zepsec && zcovptot > R(0.01)) {
           const R zfallcorr3 = m_pow(m_div(c.rdensref, zrho), R(0.4f));
-          R zsnowrime = c.rcl_effrime * zcovptot * ptsphy * c.rcl_const7s * zfallcorr3 *
+          zsnowrime = c.rcl_effrime * zcovptot * ptsphy * c.rcl_const7s * zfallcorr3 *
                         m_pow(zrho * zsnowcld * c.rcl_const1s, c.rcl_const8s);
           zsnowrime = m_min(zsnowrime, R(1));
           qbsl = qbsl + zsnowrime;
@@ -570,6 +598,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           const R zrat = m_div(zlamr, zlams);
           const R zpsacr = m_div(zcovptot * ptsphy * zfac * zfalldiff * ((R(0.5) * zrat + R(2.)) * zrat + R(5.)), lr3);
           rs = rs - zpsacr;  // ZSOLQA(S,R) += psacr
+          bud.set(PSACR, zpsacr);
         }
         if (zicecld > zepsec) {
           const R zlami = m_pow(m_div(c.rcl_lam1s, zrho * zicecld), c.rcl_lam2s);
@@ -578,6 +607,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           const R zrat = m_div(zlamr, zlami);
           const R zpiacr = m_div(zcovptot * ptsphy * zfac * zfalldiff * ((R(0.5) * zrat + R(2.)) * zrat + R(5.)), lr3);
           ri = ri - zpiacr;  // ZSOLQA(I,R) += piacr
+          bud.set(PIACR, zpiacr);
         }
       }
 
@@ -598,6 +628,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           const R zdpevap = zdpr * zdtgdp;
           const R zevap = m_min(zdpevap, qxfg_s);
           vs = vs + zevap;
+          bud.set(I17, -zevap * zqtmst);  // (:3304)
           zcovptot = m_max(c.rcovpmin, zcovptot - m_max(R(0), m_div((zcovptot - za) * zevap, qxfg_s)));
           qxfg_s = qxfg_s - zevap;
         }
@@ -720,6 +751,15 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       qxnm1_i = qxn_i;
       qxnm1_r = qxn_r;
       qxnm1_s = qxn_s;
+      // diagnosed rates of the implicit processes (:3674-3698), IWARMRAIN = 3 without rain freezing
+      bud.set(L14, -zrainaut * qxn_l * zqtmst);
+      bud.set(L15, -zrainacc * qxn_l * zqtmst);
+      bud.set(CC6, -csink * zanewm1 * zqtmst);
+      bud.set(L6, -csink * qxn_l * zqtmst);
+      bud.set(I6, -csink * qxn_i * zqtmst);
+      bud.set(I13, -fsink_i * qxn_i * zqtmst);
+      bud.set(L21, -zsnowrime * qxn_l * zqtmst);
+      bud.set(I14, -zsnowaut * qxn_i * zqtmst);
 
       // 5.3 fluxes to the next level (:3658-3672)
       pf_i = fsink_i * qxn_i * zrdtgdp;
@@ -758,6 +798,9 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
         ta = ta + R(a_adj) * zqtmst;
         tcld_l = tcld_l + R(l_adj) * zqtmst;
         tcld_i = tcld_i + R(i_adj) * zqtmst;
+        bud.set(CC2, R(a_adj) * zqtmst);  // (:3787-3789)
+        bud.set(L2, R(l_adj) * zqtmst);
+        bud.set(I2, R(i_adj) * zqtmst);
       }
       // negative clean-up (:3793-3807)
       {
@@ -775,6 +818,15 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
         }
       }
       ST(p.pcovptot, i2, jo, zcovptot);  // (:3814-3816)
+      if constexpr (BUD) {  // section 7 (:3826-4101)
+        const uint32_t kfx = (uint32_t)bp.kfldx;
+        auto ext = [&](int fld, uint32_t kk) {
+          return (((b * kfx + (uint32_t)(fld - 1)) * klev + (kk - 1)) * nproma) * E + jo;
+        };
+        const R zdz = m_div(zdp, zrho * c.rg);
+        budget_write<R>(bud, bp, ext, k + 1, i2 + jo, u1i + joi, zqtmst, ptsphy, zdz, qxn_l, qxn_i, prainfrac, ktype,
+                        peis, c.ralvdcp, c.ralsdcp, bp.ralfdcp);
+      }
     }
 
     // store tendencies of this level (levels above NCLDTOP keep the tidy-up terms only)
@@ -971,35 +1023,42 @@ void set_tuned_block(int variant, int klev, int block) {
 template <bool SAT, typename R, typename RD, typename KC, typename KCD>
 hipError_t launch_variant(int block, bool uni, unsigned ncols, const KC& c, const KCD& cd, const KPtrs<R>& p,
                           const KDims& d, int ssc_off, const KSatPtrs<R>& s, hipStream_t stream) {
+  const KBudPtrs<R> nb{};
   if (!uni) {  // blocks not aligned to waves: per-lane block offsets, 256-lane groups
-    hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, false, SAT>), dim3((ncols + 255) / 256), dim3(256),
-                       0, stream, c, cd, p, d, ssc_off, s);
+    hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, false, SAT, false>), dim3((ncols + 255) / 256),
+                       dim3(256), 0, stream, c, cd, p, d, ssc_off, s, nb);
     return hipGetLastError();
   }
   const unsigned grid = (ncols + block - 1) / block;
   switch (block) {
     case 64:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 64, true, SAT>), dim3(grid), dim3(64), 0, stream, c, cd,
-                         p, d, ssc_off, s);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 64, true, SAT, false>), dim3(grid), dim3(64), 0,
+                         stream, c, cd, p, d, ssc_off, s, nb);
       break;
     case 128:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 128, true, SAT>), dim3(grid), dim3(128), 0, stream, c,
-                         cd, p, d, ssc_off, s);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 128, true, SAT, false>), dim3(grid), dim3(128), 0,
+                         stream, c, cd, p, d, ssc_off, s, nb);
       break;
     case 512:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 512, true, SAT>), dim3(grid), dim3(512), 0, stream, c,
-                         cd, p, d, ssc_off, s);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 512, true, SAT, false>), dim3(grid), dim3(512), 0,
+                         stream, c, cd, p, d, ssc_off, s, nb);
       break;
     default:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, true, SAT>), dim3((ncols + 255) / 256), dim3(256),
-                         0, stream, c, cd, p, d, ssc_off, s);
+      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, true, SAT, false>), dim3((ncols + 255) / 256),
+                         dim3(256), 0, stream, c, cd, p, d, ssc_off, s, nb);
   }
   return hipGetLastError();
 }
 
 template <typename R, typename RD, typename KC, typename KCD>
 hipError_t launch_either(bool sat, int block, bool uni, unsigned ncols, const KC& c, const KCD& cd,
-                         const KPtrs<R>& p, const KDims& d, int ssc_off, const KSatPtrs<R>& s, hipStream_t stream) {
+                         const KPtrs<R>& p, const KDims& d, int ssc_off, const KSatPtrs<R>& s,
+                         const KBudPtrs<R>* bud, hipStream_t stream) {
+  if (bud) {  // budget-instrumented kernel: one instantiation per variant (diagnostic runs)
+    hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, false, false, true>), dim3((ncols + 255) / 256),
+                       dim3(256), 0, stream, c, cd, p, d, ssc_off, s, *bud);
+    return hipGetLastError();
+  }
   return sat ? launch_variant<true, R, RD>(block, uni, ncols, c, cd, p, d, ssc_off, s, stream)
              : launch_variant<false, R, RD>(block, uni, ncols, c, cd, p, d, ssc_off, s, stream);
 }
@@ -1016,15 +1075,27 @@ KSatPtrs<R> offset_sat_ptrs(const KSatPtrs<R>& p, int64_t b0, int64_t nproma, in
   return q;
 }
 
+template <typename R>
+KBudPtrs<R> offset_bud_ptrs(const KBudPtrs<R>& p, int64_t b0, int64_t nproma, int64_t klev) {
+  const int64_t s2 = nproma * klev;
+  KBudPtrs<R> q = p;
+  q.pvfa += b0 * s2; q.pdyna += b0 * s2; q.pdynl += b0 * s2; q.pdyni += b0 * s2; q.pdyns += b0 * s2;
+  q.phrsw += b0 * s2; q.phrlw += b0 * s2; q.pvfl += b0 * s2; q.pvfi += b0 * s2;
+  q.kpbltype += b0 * nproma; q.kctop += b0 * nproma;
+  q.pextra += b0 * s2 * q.kfldx;
+  return q;
+}
+
 template <typename R, typename RD>
 hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
-                          hipStream_t stream, const KSatPtrs<R>* sat) {
+                          hipStream_t stream, const KSatPtrs<R>* sat, const KBudPtrs<R>* bud) {
   const bool def = is_default_config(q);
   const int block = def ? tuned_block(variant_id<R, RD>(), d.klev) : 256;
   // every wave inside one block (64-lane waves, 64 | NPROMA): block offsets are uniform across the wave
   const bool uni = d.nproma % 64 == 0;
-  // blocks per launch: the largest array (NPROMA,KLEV,5) must stay below 4 GiB for 32-bit offsets
-  const int64_t per_block = (int64_t)d.nproma * d.klev * 5 * (int64_t)sizeof(R);
+  // blocks per launch: the largest array, (NPROMA,KLEV,5) or PEXTRA (NPROMA,KLEV,KFLDX), stays below 4 GiB
+  const int64_t nfld = bud ? std::max<int64_t>(5, bud->kfldx) : 5;
+  const int64_t per_block = (int64_t)d.nproma * d.klev * nfld * (int64_t)sizeof(R);
   const int64_t max_blocks = std::max<int64_t>(1, ((int64_t)1 << 32) / per_block - 1);
   for (int64_t b0 = 0; b0 < d.ngpblks; b0 += max_blocks) {
     const int64_t nb = std::min<int64_t>(max_blocks, d.ngpblks - b0);
@@ -1033,16 +1104,19 @@ hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<
     dc.ngptot = (int32_t)(d.ngptot - b0 * d.nproma);
     const KPtrs<R> pc = offset_ptrs(p, b0, d.nproma, d.klev);
     const KSatPtrs<R> sc = sat ? offset_sat_ptrs(*sat, b0, d.nproma, d.klev) : KSatPtrs<R>{};
+    const KBudPtrs<R> bc = bud ? offset_bud_ptrs(*bud, b0, d.nproma, d.klev) : KBudPtrs<R>{};
     const unsigned ncols = (unsigned)(nb * d.nproma);
     hipError_t e;
     if (def) {
       const KConstDefault<R> c = make_kconst_default<R>(q, ptsphy);
       const KConstDefault<RD> cd = make_kconst_default<RD>(q, ptsphy);
-      e = launch_either<R, RD>(sat != nullptr, block, uni, ncols, c, cd, pc, dc, ssc_off, sc, stream);
+      e = launch_either<R, RD>(sat != nullptr, block, uni, ncols, c, cd, pc, dc, ssc_off, sc, bud ? &bc : nullptr,
+                               stream);
     } else {
       const KConst<R> c = make_kconst<R>(q, ptsphy);
       const KConst<RD> cd = make_kconst<RD>(q, ptsphy);
-      e = launch_either<R, RD>(sat != nullptr, 256, uni, ncols, c, cd, pc, dc, ssc_off, sc, stream);
+      e = launch_either<R, RD>(sat != nullptr, 256, uni, ncols, c, cd, pc, dc, ssc_off, sc, bud ? &bc : nullptr,
+                               stream);
     }
     if (e != hipSuccess) return e;
   }
@@ -1055,10 +1129,11 @@ template KConstDefault<float> make_kconst_default<float>(const cloudsc_params_t&
 template KConstDefault<double> make_kconst_default<double>(const cloudsc_params_t&, double);
 
 template hipError_t launch_cloudsc<double, double>(const cloudsc_params_t&, double, const KPtrs<double>&,
-                                                   const KDims&, int, hipStream_t, const KSatPtrs<double>*);
+                                                   const KDims&, int, hipStream_t, const KSatPtrs<double>*,
+                                                   const KBudPtrs<double>*);
 template hipError_t launch_cloudsc<float, float>(const cloudsc_params_t&, double, const KPtrs<float>&, const KDims&,
-                                                 int, hipStream_t, const KSatPtrs<float>*);
+                                                 int, hipStream_t, const KSatPtrs<float>*, const KBudPtrs<float>*);
 template hipError_t launch_cloudsc<float, double>(const cloudsc_params_t&, double, const KPtrs<float>&, const KDims&,
-                                                  int, hipStream_t, const KSatPtrs<float>*);
+                                                  int, hipStream_t, const KSatPtrs<float>*, const KBudPtrs<float>*);
 
 }  // namespace cloudsc_gpu

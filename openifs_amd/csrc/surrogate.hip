@@ -344,6 +344,8 @@ int salloc(cloudsc_surrogate* s, size_t bytes, void** p) {
 int check_fields(const cloudsc_surrogate* s, const cloudsc_params_t* q, int ngptot, int nproma, int klev,
                  int ngpblks, const KPtrs<float>& p) {
   if (!s || !q) return fail(CLOUDSC_ENULL, "cloudsc_surrogate: NULL object or parameters");
+  const int rc = cloudsc_capi::check_config(q);  // the plain kernel reruns columns: the same switches apply
+  if (rc) return rc;
   if (klev != s->klev) return fail(CLOUDSC_EDIMS, "cloudsc_surrogate: KLEV differs from the surrogate's");
   if (nproma < 1 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma)
     return fail(CLOUDSC_EDIMS, "cloudsc_surrogate: need NGPBLKS = ceil(NGPTOT/NPROMA)");

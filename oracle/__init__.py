@@ -39,12 +39,12 @@ def lib():
     return _lib
 
 
-def cloudsc_block(precision, params, kidia, kfdia, klon, klev, ptsphy, arrays):
+def cloudsc_block(precision, params, kidia, kfdia, klon, klev, ptsphy, arrays, kfldx=0):
     """One CLOUDSC block on the CPU, same argument list as the product's cloudsc_*_c."""
     from openifs_amd.cloudsc import call_block
 
     f = getattr(lib(), f"oracle_cloudsc_{precision}")
-    rc = call_block(f, params, kidia, kfdia, klon, klev, ptsphy, arrays, precision)
+    rc = call_block(f, params, kidia, kfdia, klon, klev, ptsphy, arrays, precision, kfldx=kfldx)
     if rc != 0:
         raise RuntimeError(f"oracle_cloudsc_{precision} returned {rc}")
     return arrays
@@ -69,6 +69,29 @@ def cloudsc_blocked(precision, params, ngptot, nproma, klev, ptsphy, arrays, nth
     if rc != 0:
         raise RuntimeError(f"oracle_cloudsc_blocked_{precision} returned {rc}")
     return arrays
+
+
+def cloudsc_blocked_budget(precision, params, ngptot, nproma, klev, ptsphy, arrays, nthreads=0):
+    """cloudsc_blocked plus the budget diagnostics: ``arrays`` also holds the BUDGET_ARRAYS of
+    openifs_amd.cloudsc and ``pextra`` as (NGPBLKS, KFLDX, KLEV, NPROMA); returns the return code."""
+    from openifs_amd.cloudsc import BLOCKED_ARRAYS, BUDGET_ARRAYS
+
+    ngpblks = (ngptot + nproma - 1) // nproma
+    real = ctypes.c_double if precision == "dp" else ctypes.c_float
+    rdt = np.float64 if precision == "dp" else np.float32
+    args = [ctypes.byref(params), ctypes.c_int(ngptot), ctypes.c_int(nproma), ctypes.c_int(klev),
+            ctypes.c_int(ngpblks), real(ptsphy)]
+    for name, kind in [(n, k) for n, k, _ in BLOCKED_ARRAYS] + BUDGET_ARRAYS + [("pextra", "extra")]:
+        a = arrays[name]
+        want = np.int32 if kind == "1di" else rdt
+        if a.dtype != want or not a.flags.c_contiguous:
+            raise TypeError(f"{name}: need a C-contiguous {np.dtype(want).name} array")
+        args.append(ctypes.c_void_p(a.ctypes.data))
+    px = arrays["pextra"]
+    if px.shape != (ngpblks, px.shape[1], klev, nproma):
+        raise ValueError("pextra: need (NGPBLKS, KFLDX, KLEV, NPROMA)")
+    args += [ctypes.c_int(px.shape[1]), ctypes.c_int(nthreads)]
+    return getattr(lib(), f"oracle_cloudsc_blocked_budget_{precision}")(*args)
 
 
 def supersatcheck(params, t, q, a, pp, psurf, precision="dp"):

@@ -6,6 +6,7 @@
 //   oracle_cloudsc_{dp,sp,mixed}        one NPROMA block, CLOUDSC's argument list
 //   oracle_cloudsc_blocked_{dp,sp,mixed} all blocks, OpenMP over blocks with
 //       SCHEDULE(DYNAMIC,1) as the reference driver (ec_phys_drv.F90:430-439)
+//   oracle_cloudsc_blocked_budget_{dp,sp,mixed}  the same with the LCLDBUD* budgets into PEXTRA
 //   oracle_satadj_{dp,sp,mixed}         one block of CLOUD_SATADJ, the argument list of
 //       include/cloudsc_satadj.h
 //   oracle_satadj_blocked_{dp,sp,mixed}  all blocks, OpenMP over blocks
@@ -21,10 +22,36 @@
 
 namespace {
 
-int check_config(const cloudsc_params_t* p) {
+constexpr int32_t kBudOut = CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T | CLOUDSC_BUD_VERTINT;
+
+// The same rules as the product library: budgets that write PEXTRA only where PEXTRA is an argument.
+int check_config(const cloudsc_params_t* p, bool budget = false) {
   if (!p) return CLOUDSC_ENULL;
-  if (p->lcloud_inhomog || p->lrad_cloud_inhomog || p->laeraware || p->lspp || p->lcldbud)
-    return CLOUDSC_EUNSUPPORTED;
+  if (p->lcloud_inhomog || p->lrad_cloud_inhomog || p->laeraware || p->lspp) return CLOUDSC_EUNSUPPORTED;
+  if ((p->lcldbud & kBudOut) && !budget) return CLOUDSC_EUNSUPPORTED;
+  if (p->lcldbud & ~255) return CLOUDSC_EUNSUPPORTED;
+  return CLOUDSC_OK;
+}
+
+// PEXTRA fields the switches fill: the value of IS after the last budget block (cloudsc.F90:3832-4101).
+int budget_fields(int32_t f) {
+  int is = (f & CLOUDSC_BUD_LBUD23) ? 26 : 0;
+  if (f & CLOUDSC_BUD_VERTINT) is += 1;
+  if (f & CLOUDSC_BUD_C) is += 12;
+  if (f & CLOUDSC_BUD_L) is += 22;
+  if (f & CLOUDSC_BUD_I) is += 20;
+  if (f & CLOUDSC_BUD_T) is += 13;
+  return is;
+}
+
+template <typename B>
+int check_budget(const cloudsc_params_t* p, int klev, const B& b) {
+  if (!(p->lcldbud & kBudOut)) return CLOUDSC_OK;
+  const void* req[] = {b.pvfa, b.pdyna, b.pdynl, b.pdyni, b.pdyns, b.phrsw, b.phrlw, b.kpbltype, b.kctop, b.pextra};
+  for (const void* x : req)
+    if (!x) return CLOUDSC_ENULL;
+  if (b.kfldx < budget_fields(p->lcldbud)) return CLOUDSC_EDIMS;
+  if ((p->lcldbud & CLOUDSC_BUD_VERTINT) && klev < 83) return CLOUDSC_EDIMS;  // slots 4..83 of the field
   return CLOUDSC_OK;
 }
 
@@ -44,31 +71,33 @@ int run_block(const cloudsc_params_t* p, int kidia, int kfdia, int klon, int kle
               const R* pludeli, const R* psnde, const R* pmfu, const R* pmfd, const R* pa, const R* pclv,
               R* pcovptot, R* prainfrac_toprfz, R* pfsqlf, R* pfsqif, R* pfcqnng, R* pfcqlng, R* pfsqrf,
               R* pfsqsf, R* pfcqrng, R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn, R* pfhpsl,
-              R* pfhpsn) {
-  int rc = check_config(p);
+              R* pfhpsn, const oracle::BudArgs<R>& bud) {
+  int rc = check_config(p, true);
   if (rc) return rc;
   rc = check_dims(p, kidia, kfdia, klon, klev);
+  if (rc) return rc;
+  rc = check_budget(p, klev, bud);
   if (rc) return rc;
   oracle::Args<R> a{kidia,  kfdia,  klon,    klev,    ptsphy,  pt,       pq,      tcml_t,  tcml_q,  tcml_a,
                     tcml_cld, tloc_t, tloc_q, tloc_a, tloc_cld, pvfl, pvfi, pap, paph, plsm,
                     ldcum,  ktype,  peis,    plu,     plude,   pludeli,  psnde,   pmfu,    pmfd,    pa,
                     pclv,   pcovptot, prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf,
-                    pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn};
+                    pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn,
+                    (p->lcldbud & kBudOut) ? &bud : nullptr};
   oracle::cloudsc<R, RD>(*p, a);
   return CLOUDSC_OK;
 }
 
 #define ORACLE_BLOCK_CALL(R, RD)                                                                              \
-  (void)lddiag, (void)pvfa, (void)pdyna, (void)pdynl, (void)pdyni, (void)pdynr, (void)pdyns, (void)phrsw,     \
-      (void)phrlw, (void)pvervel, (void)pgeom, (void)pgaw, (void)kctop, (void)kpbltype, (void)pgp2dspp,         \
-      (void)ldslphy, (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice, (void)pfsd,    \
-      (void)pextra, (void)kfldx;                                                                                \
+  (void)lddiag, (void)pdynr, (void)pvervel, (void)pgeom, (void)pgaw, (void)pgp2dspp, (void)ldslphy,          \
+      (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice, (void)pfsd;                   \
+  const oracle::BudArgs<R> bud{pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx}; \
   return run_block<R, RD>(ydparams, kidia, kfdia, klon, klev, ptsphy, pt, pq, ptendency_cml_t, ptendency_cml_q, \
                           ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a, \
                           ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis, plu, plude,        \
                           pludeli, psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz, pfsqlf, pfsqif,      \
                           pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl,        \
-                          pfplsn, pfhpsl, pfhpsn)
+                          pfplsn, pfhpsl, pfhpsn, bud)
 
 template <typename R, typename RD>
 int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int ngpblks, R ptsphy, const R* pt,
@@ -78,12 +107,17 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
                 const R* pludeli, const R* psnde, const R* pmfu, const R* pmfd, const R* pa, const R* pclv,
                 R* pcovptot, R* prainfrac_toprfz, R* pfsqlf, R* pfsqif, R* pfcqnng, R* pfcqlng, R* pfsqrf,
                 R* pfsqsf, R* pfcqrng, R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn, R* pfhpsl,
-                R* pfhpsn, int nthreads) {
-  int rc = check_config(p);
+                R* pfhpsn, int nthreads, const oracle::BudArgs<R>* bud = nullptr) {
+  int rc = check_config(p, bud != nullptr);
   if (rc) return rc;
   if (nproma < 1 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma) return CLOUDSC_EDIMS;
   rc = check_dims(p, 1, nproma, nproma, klev);
   if (rc) return rc;
+  if (bud) {
+    rc = check_budget(p, klev, *bud);
+    if (rc) return rc;
+  }
+  const bool budgets = bud && (p->lcldbud & kBudOut);
   const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1), s1 = nproma;
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -91,6 +125,10 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
     const int kfdia = std::min(nproma, ngptot - ibl * nproma);
     const size_t o2 = ibl * s2, oh = ibl * sh, o1 = ibl * s1, o5 = ibl * s2 * 5, o4 = ibl * s2 * 4,
                  o2b = ibl * s2 * 2;
+    oracle::BudArgs<R> bb{};
+    if (budgets)  // PEXTRA block IBL is (NPROMA,KLEV,KFLDX)
+      bb = {bud->pvfa + o2,  bud->pdyna + o2, bud->pdynl + o2, bud->pdyni + o2, bud->pdyns + o2, bud->phrsw + o2,
+            bud->phrlw + o2, bud->kpbltype + o1, bud->kctop + o1, bud->pextra + o2 * bud->kfldx, bud->kfldx};
     oracle::Args<R> a{1,
                       kfdia,
                       nproma,
@@ -137,7 +175,8 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
                       pfplsl + oh,
                       pfplsn + oh,
                       pfhpsl + oh,
-                      pfhpsn + oh};
+                      pfhpsn + oh,
+                      budgets ? &bb : nullptr};
     oracle::cloudsc<R, RD>(*p, a);
   }
   return CLOUDSC_OK;
@@ -161,6 +200,30 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
       const R *pa, const R *pclv, R *pcovptot, R *prainfrac_toprfz, R *pfsqlf, R *pfsqif,              \
       R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur,   \
       R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, int nthreads
+
+// The blocked oracle with the budget diagnostics: the arguments of cloudsc_blocked_budget_*_c in their order
+// (host arrays), then the thread count.
+#define ORACLE_BLOCKED_BUDGET_ARGS(R)                                                                   \
+  const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,            \
+      const R *pt, const R *pq, const R *ptendency_cml_t, const R *ptendency_cml_q,                     \
+      const R *ptendency_cml_a, const R *ptendency_cml_cld, R *ptendency_loc_t, R *ptendency_loc_q,     \
+      R *ptendency_loc_a, R *ptendency_loc_cld, const R *pvfl, const R *pvfi, const R *pap,             \
+      const R *paph, const R *plsm, const int32_t *ldcum, const int32_t *ktype, R *peis,                \
+      const R *plu, R *plude, const R *pludeli, const R *psnde, const R *pmfu, const R *pmfd,           \
+      const R *pa, const R *pclv, R *pcovptot, R *prainfrac_toprfz, R *pfsqlf, R *pfsqif,              \
+      R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur,   \
+      R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, const R *pvfa, const R *pdyna, const R *pdynl,        \
+      const R *pdyni, const R *pdyns, const R *phrsw, const R *phrlw, const int32_t *kpbltype,          \
+      const int32_t *kctop, R *pextra, int kfldx, int nthreads
+
+#define ORACLE_BLOCKED_BUDGET_CALL(R, RD)                                                                       \
+  const oracle::BudArgs<R> bud{pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx};   \
+  return run_blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy, pt, pq, ptendency_cml_t,          \
+                            ptendency_cml_q, ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, \
+                            ptendency_loc_a, ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis,  \
+                            plu, plude, pludeli, psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz, pfsqlf, \
+                            pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng, pfsqltur, pfsqitur,       \
+                            pfplsl, pfplsn, pfhpsl, pfhpsn, nthreads, &bud)
 
 // ---- CLOUD_SATADJ ----
 template <typename R>
@@ -263,6 +326,12 @@ int oracle_cloudsc_mixed(CLOUDSC_ARGS(float)) { ORACLE_BLOCK_CALL(float, double)
 int oracle_cloudsc_blocked_dp(ORACLE_BLOCKED_ARGS(double)) { ORACLE_BLOCKED_CALL(double, double); }
 int oracle_cloudsc_blocked_sp(ORACLE_BLOCKED_ARGS(float)) { ORACLE_BLOCKED_CALL(float, float); }
 int oracle_cloudsc_blocked_mixed(ORACLE_BLOCKED_ARGS(float)) { ORACLE_BLOCKED_CALL(float, double); }
+
+int oracle_cloudsc_blocked_budget_dp(ORACLE_BLOCKED_BUDGET_ARGS(double)) { ORACLE_BLOCKED_BUDGET_CALL(double, double); }
+int oracle_cloudsc_blocked_budget_sp(ORACLE_BLOCKED_BUDGET_ARGS(float)) { ORACLE_BLOCKED_BUDGET_CALL(float, float); }
+int oracle_cloudsc_blocked_budget_mixed(ORACLE_BLOCKED_BUDGET_ARGS(float)) {
+  ORACLE_BLOCKED_BUDGET_CALL(float, double);
+}
 
 // CLOUD_SUPERSATCHECK on n points (KIDIA = 1, KFDIA = n), outputs in RD.
 int oracle_supersatcheck_dp(const cloudsc_params_t* p, int n, const double* t, const double* q, const double* a,

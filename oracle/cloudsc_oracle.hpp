@@ -13,9 +13,11 @@
 // reference's (cloudsc.F90:727-793): IWARMRAIN=3, IRAINACC=1, ISUBLSNOW=1,
 // IDEPSNOW=1, IDEPICE=1, ISUBLICE=0, IP_SNOW_ACCRETES_RAIN=1,
 // IP_ICE_ACCRETES_RAIN=1, IP_RAIN_ACCRETES_SNOW=0, IP_RAIN_ACCRETES_ICE=0,
-// IVARFALL_RAIN=1, IVARFALL_ICE=0, ITURBEROSION=3.  Paths behind switches
-// that are off by default (SPP, aerosol, inhomogeneity, budgets) are not
-// restated; the driver rejects configurations that would need them.
+// IVARFALL_RAIN=1, IVARFALL_ICE=0, ITURBEROSION=3.  The budget diagnostics
+// (LCLDBUD*, LBUD23; cloudsc.F90:1271-1273 and the ZBUD* sites, PEXTRA writes
+// :3826-4101) are restated when Args::bud is set.  Paths behind switches that
+// are off by default (SPP, aerosol, inhomogeneity) are not restated; the
+// driver rejects configurations that would need them.
 //
 // Template parameters: R = JPRB (storage and bulk arithmetic), RD = type of
 // the "FP64 islands" (CLOUD_SUPERSATCHECK and the 5x5 solve).  <double,double>
@@ -243,6 +245,16 @@ void implicit_solve(const R qa[NCLV][NCLV], const R qb[NCLV][NCLV], const R fall
   for (int m = 0; m < NCLV; ++m) qxn[m] = R(x[m]);
 }
 
+// Inputs and output of the budget diagnostics (LCLDBUD*, cloudsc.F90:3826-4101) for one call, Fortran layout:
+// (KLON,KLEV) arrays, KPBLTYPE/KCTOP (KLON), PEXTRA (KLON,KLEV,KFLDX).
+template <typename R>
+struct BudArgs {
+  const R *pvfa, *pdyna, *pdynl, *pdyni, *pdyns, *phrsw, *phrlw;
+  const int32_t *kpbltype, *kctop;
+  R* pextra;
+  int kfldx;
+};
+
 // Arguments of one CLOUDSC call, Fortran layout (JL fastest).
 template <typename R>
 struct Args {
@@ -258,6 +270,7 @@ struct Args {
   R *pcovptot, *prainfrac_toprfz;
   R *pfsqlf, *pfsqif, *pfcqnng, *pfcqlng, *pfsqrf, *pfsqsf, *pfcqrng, *pfcqsng, *pfsqltur, *pfsqitur;
   R *pfplsl, *pfplsn, *pfhpsl, *pfhpsn;
+  const BudArgs<R>* bud = nullptr;  // budget diagnostics: switched by params.lcldbud, NULL = none
 };
 
 // ---------------------------------------------------------------------------
@@ -280,6 +293,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
   const R RG = R(p.rg), RD_ = R(p.rd), RV = R(p.rv), RETV = R(p.retv), RTT = R(p.rtt), RLSTT = R(p.rlstt);
   const R RLVTT = R(p.rlvtt), RPI = R(p.rpi);
   const R RALVDCP = R(p.ralvdcp), RALSDCP = R(p.ralsdcp), R5LES = R(p.r5les), R5IES = R(p.r5ies);
+  const R RALFDCP = R(p.ralfdcp);  // budget diagnostics only
   const R R4LES = R(p.r4les), R4IES = R(p.r4ies);
   const R RAMID = R(p.ramid), RAMIN = R(p.ramin), RLMIN = R(p.rlmin), RCLDIFF = R(p.rcldiff);
   const R RCLDIFF_CONVI = R(p.rcldiff_convi);  // ZLCRIT (RCLCRIT_*) is not used by IWARMRAIN=3
@@ -453,9 +467,18 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
   auto QA = [&](int jl, int m, int n) -> R& { return zsolqa[((size_t)n * NCLV + m) * KL + (jl - 1)]; };
   auto QB = [&](int jl, int m, int n) -> R& { return zsolqb[((size_t)n * NCLV + m) * KL + (jl - 1)]; };
   auto V = [&](std::vector<R>& v, int jl, int m) -> R& { return v[(size_t)m * KL + (jl - 1)]; };
+  // Budget arrays ZBUDCC(KLON,12), ZBUDL(KLON,22), ZBUDI(KLON,18) (:557-559), 1-based second index.
+  const int lcldbud = a.bud ? p.lcldbud : 0;
+  std::vector<R> zbudcc(KL * 12), zbudl(KL * 22), zbudi(KL * 18);
+  auto BCC = [&](int jl, int n) -> R& { return zbudcc[(size_t)(n - 1) * KL + (jl - 1)]; };
+  auto BL = [&](int jl, int n) -> R& { return zbudl[(size_t)(n - 1) * KL + (jl - 1)]; };
+  auto BI = [&](int jl, int n) -> R& { return zbudi[(size_t)(n - 1) * KL + (jl - 1)]; };
 
   // ======================= level loop (:1219-4128) =======================
   for (int jk = ncldtop; jk <= klev; ++jk) {
+    std::fill(zbudcc.begin(), zbudcc.end(), R(0));  // :1271-1273
+    std::fill(zbudl.begin(), zbudl.end(), R(0));
+    std::fill(zbudi.begin(), zbudi.end(), R(0));
     for (int m = 0; m < NCLV; ++m)
       for (int jl = kidia; jl <= kfdia; ++jl) V(zqxfg, jl, m) = zqx[i3(jl, jk, m + 1)];  // :1228-1232
     for (int jl = kidia; jl <= kfdia; ++jl) {  // :1238-1268
@@ -525,6 +548,9 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
           V(zconvsrce, jl, QI) = a.pludeli[i3(jl, jk, 2)] * zdtgdp[i];
           QA(jl, QL, QL) = QA(jl, QL, QL) + V(zconvsrce, jl, QL);
           QA(jl, QI, QI) = QA(jl, QI, QI) + V(zconvsrce, jl, QI);
+          BL(jl, 3) = V(zconvsrce, jl, QL) * ZQTMST;  // :1378-1380
+          BI(jl, 3) = V(zconvsrce, jl, QI) * ZQTMST;
+          BCC(jl, 3) = ZQTMST * a.plude[k] / a.plu[i2(jl, jk + 1)];
         } else {
           a.plude[k] = R(0);
         }
@@ -569,12 +595,23 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
           QA(jl, m, m) = QA(jl, m, m) + V(zlcust, jl, m);
           QA(jl, QV, m) = QA(jl, QV, m) + zevap;
           QA(jl, m, QV) = QA(jl, m, QV) - zevap;
+          if (m == QL) {  // :1459-1460
+            BL(jl, 4) = V(zlcust, jl, m) * ZQTMST;
+            BL(jl, 5) = -zevap * ZQTMST;
+          } else {  // :1475-1476
+            BI(jl, 4) = V(zlcust, jl, m) * ZQTMST;
+            BI(jl, 5) = -zevap * ZQTMST;
+          }
         }
       }
       for (int jl = kidia; jl <= kfdia; ++jl) {
         const int i = jl - 1;
         zsolac[i] = zsolac[i] + zacust[i];
-        if (zlfinalsum[i] < ZEPSEC) zsolac[i] = zsolac[i] - zacust[i];
+        BCC(jl, 4) = zacust[i] * ZQTMST;  // :1484
+        if (zlfinalsum[i] < ZEPSEC) {
+          zsolac[i] = zsolac[i] - zacust[i];
+          BCC(jl, 5) = -zacust[i] * ZQTMST;  // :1488
+        }
       }
     }
     // subsidence sink to the level below, implicit (:1499-1514)
@@ -615,6 +652,9 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         QA(jl, QL, QV) = QA(jl, QL, QV) - zliqfrac[k] * zleros;
         QA(jl, QV, QI) = QA(jl, QV, QI) + zicefrac[k] * zleros;
         QA(jl, QI, QV) = QA(jl, QI, QV) - zicefrac[k] * zleros;
+        BL(jl, 7) = -zliqfrac[k] * zleros * ZQTMST;  // :1749-1751
+        BI(jl, 7) = -zicefrac[k] * zleros * ZQTMST;
+        BCC(jl, 7) = -zaeros * ZQTMST;
       }
     }
 
@@ -637,6 +677,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         if (jk > ncldtop) {
           V(zfallsrce, jl, m) = zpfplsx[ih3(jl, jk, m + 1)] * zdtgdp[i];
           QA(jl, m, m) = QA(jl, m, m) + V(zfallsrce, jl, m);
+          if (m == QI) BI(jl, 12) = V(zfallsrce, jl, m) * ZQTMST;  // :1865
           V(zqxfg, jl, m) = V(zqxfg, jl, m) + V(zfallsrce, jl, m);
           zqpretot[i] = zqpretot[i] + V(zqxfg, jl, m);
         }
@@ -719,6 +760,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         const R zevap = std::min(zdpevap, V(zqxfg, jl, QR));
         QA(jl, QV, QR) = QA(jl, QV, QR) + zevap;
         QA(jl, QR, QV) = QA(jl, QR, QV) - zevap;
+        BL(jl, 20) = -zevap * ZQTMST;  // :2363
         V(zqxfg, jl, QR) = V(zqxfg, jl, QR) - zevap;
       }
     }
@@ -753,6 +795,8 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         QA(jl, QL, QI) = QA(jl, QL, QI) - zdepos;
         V(zqxfg, jl, QI) = V(zqxfg, jl, QI) + zdepos;
         V(zqxfg, jl, QL) = V(zqxfg, jl, QL) - zdepos;
+        BL(jl, 11) = -zdepos * ZQTMST;  // :2495-2496
+        BI(jl, 11) = zdepos * ZQTMST;
       }
     }
     // 4.6 ice sublimation: ISUBLICE = 0, process rate stays zero (:2613-2615)
@@ -836,6 +880,9 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
           V(zqxfg, jl, jn) = V(zqxfg, jl, jn) + zmelt;
           QA(jl, jn, jm) = QA(jl, jn, jm) + zmelt;
           QA(jl, jm, jn) = QA(jl, jm, jn) - zmelt;
+          if (jm == QI) BI(jl, 15) = -zmelt * ZQTMST;  // :2856-2858
+          if (jm == QI) BL(jl, 17) = zmelt * ZQTMST;
+          if (jm == QS) BI(jl, 16) = -zmelt * ZQTMST;
         }
       }
     }
@@ -866,6 +913,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
             const R zfrz = std::min(qr, zfrzmax[i]);
             QA(jl, QS, QR) = QA(jl, QS, QR) + zfrz;
             QA(jl, QR, QS) = QA(jl, QR, QS) - zfrz;
+            BL(jl, 18) = -zfrz * ZQTMST;  // :2927
           }
         }
       }
@@ -879,6 +927,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         const R zfrz = std::min(V(zqxfg, jl, QL), zfrzmax[i]);
         QA(jl, QI, QL) = QA(jl, QI, QL) + zfrz;
         QA(jl, QL, QI) = QA(jl, QL, QI) - zfrz;
+        BL(jl, 19) = -zfrz * ZQTMST;  // :2958
       }
     }
 
@@ -968,6 +1017,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         const R zevap = std::min(zdpevap, V(zqxfg, jl, QS));
         QA(jl, QV, QS) = QA(jl, QV, QS) + zevap;
         QA(jl, QS, QV) = QA(jl, QS, QV) - zevap;
+        BI(jl, 17) = -zevap * ZQTMST;  // :3304
         zcovptot[i] = std::max(RCOVPMIN, zcovptot[i] - std::max(R(0), (zcovptot[i] - za[k]) * zevap / V(zqxfg, jl, QS)));
         V(zqxfg, jl, QS) = V(zqxfg, jl, QS) - zevap;
       }
@@ -1034,6 +1084,18 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
       zqpretot[jl - 1] = zpfplsx[ih3(jl, jk + 1, QS + 1)] + zpfplsx[ih3(jl, jk + 1, QR + 1)];
       if (zqpretot[jl - 1] < ZEPSEC) zcovptot[jl - 1] = R(0);
     }
+    // diagnosed rates of the implicit processes (:3674-3698); IWARMRAIN = 3 with IP_SNOW_ACCRETES_RAIN = 1
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      const int i = jl - 1;
+      BL(jl, 14) = -zrainaut[i] * V(zqxn, jl, QL) * ZQTMST;
+      BL(jl, 15) = -zrainacc[i] * V(zqxn, jl, QL) * ZQTMST;
+      BCC(jl, 6) = -V(zconvsink, jl, QL) * zanewm1[i] * ZQTMST;
+      BL(jl, 6) = -V(zconvsink, jl, QL) * V(zqxn, jl, QL) * ZQTMST;
+      BI(jl, 6) = -V(zconvsink, jl, QI) * V(zqxn, jl, QI) * ZQTMST;
+      BI(jl, 13) = -V(zfallsink, jl, QI) * V(zqxn, jl, QI) * ZQTMST;
+      BL(jl, 21) = -zsnowrime[i] * V(zqxn, jl, QL) * ZQTMST;
+      BI(jl, 14) = -zsnowaut[i] * V(zqxn, jl, QI) * ZQTMST;
+    }
 
     // 6.1 temperature and condensate tendencies (:3713-3745)
     for (int jm = 0; jm < NCLV - 1; ++jm) {
@@ -1077,6 +1139,9 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
       a.tloc_a[k] = a.tloc_a[k] + R(za_adj[i]) * ZQTMST;
       a.tloc_cld[i3(jl, jk, QL + 1)] = a.tloc_cld[i3(jl, jk, QL + 1)] + R(zl_adj[i]) * ZQTMST;
       a.tloc_cld[i3(jl, jk, QI + 1)] = a.tloc_cld[i3(jl, jk, QI + 1)] + R(zi_adj[i]) * ZQTMST;
+      BCC(jl, 2) = R(za_adj[i]) * ZQTMST;  // :3787-3789
+      BL(jl, 2) = R(zl_adj[i]) * ZQTMST;
+      BI(jl, 2) = R(zi_adj[i]) * ZQTMST;
       const R zl = zqx0[i3(jl, jk, QL + 1)] + a.tloc_cld[i3(jl, jk, QL + 1)] * ptsphy;
       if (zl < R(0)) {
         a.tloc_cld[i3(jl, jk, QL + 1)] = a.tloc_cld[i3(jl, jk, QL + 1)] - zl * ZQTMST;
@@ -1091,6 +1156,170 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
       }
     }
     for (int jl = kidia; jl <= kfdia; ++jl) a.pcovptot[i2(jl, jk)] = zcovptot[jl - 1];  // (:3814-3816)
+
+    // ---- 7. cloud budget diagnostics (:3826-4101) ----
+    if (lcldbud) {
+      const BudArgs<R>& b = *a.bud;
+      auto PX = [&](int jl, int k, int f) -> R& {  // PEXTRA(JL,K,F)
+        return b.pextra[((size_t)(f - 1) * klev + (k - 1)) * klon + (jl - 1)];
+      };
+      int is = (lcldbud & CLOUDSC_BUD_LBUD23) ? 26 : 0;
+      const R zdt = (lcldbud & CLOUDSC_BUD_TIMEINT) ? ptsphy : R(1);
+      // Rates that the default configuration never sets keep their zero from :1271-1273 (ZBUDCC 1, 10-12;
+      // ZBUDL 1, 8-10, 12-13, 16, 22; ZBUDI 1, 8-10, 18) and are read as such below.  ZPRACS, ZPRACI and
+      // ZPIEVAP are zero likewise (:3130, :3207, :2613 with IP_RAIN_ACCRETES_SNOW = 0, ISUBLICE = 0).
+      const R zpracs = R(0), zpraci = R(0), zpievap = R(0);
+      if (lcldbud & CLOUDSC_BUD_VERTINT) {  // :3849-3943
+        for (int jl = kidia; jl <= kfdia; ++jl) {
+          const int i = jl - 1;
+          const size_t k = i2(jl, jk);
+          const R zdz = zdp[i] / (zrho[i] * RG);
+          const int f = is + 1;
+          int ik = 0;
+          PX(jl, ik + 4, f) = PX(jl, ik + 4, f) + BCC(jl, 2) * zdz * zdt;
+          PX(jl, ik + 6, f) = PX(jl, ik + 6, f) + BCC(jl, 3) * zdz * zdt;
+          PX(jl, ik + 7, f) = PX(jl, ik + 7, f) + (BCC(jl, 4) + BCC(jl, 6)) * zdz * zdt;
+          PX(jl, ik + 8, f) = PX(jl, ik + 8, f) + BCC(jl, 5) * zdz * zdt;
+          PX(jl, ik + 9, f) = PX(jl, ik + 9, f) + BCC(jl, 7) * zdz * zdt;
+          PX(jl, ik + 10, f) = PX(jl, ik + 10, f) + BCC(jl, 12) * zdz * zdt;
+          PX(jl, ik + 11, f) = PX(jl, ik + 11, f) + b.pvfa[k] * zdz * zdt;
+          PX(jl, ik + 12, f) = PX(jl, ik + 12, f) + b.pdyna[k] * zdz * zdt;
+          ik = 14;
+          PX(jl, ik + 5, f) = PX(jl, ik + 5, f) + BL(jl, 2) * zdz * zdt;
+          PX(jl, ik + 7, f) = PX(jl, ik + 7, f) + BL(jl, 3) * zdz * zdt;
+          PX(jl, ik + 8, f) = PX(jl, ik + 8, f) + (BL(jl, 4) + BL(jl, 6)) * zdz * zdt;
+          PX(jl, ik + 9, f) = PX(jl, ik + 9, f) + BL(jl, 5) * zdz * zdt;
+          PX(jl, ik + 10, f) = PX(jl, ik + 10, f) + BL(jl, 7) * zdz * zdt;
+          PX(jl, ik + 11, f) = PX(jl, ik + 11, f) + BL(jl, 11) * zdz * zdt;
+          PX(jl, ik + 12, f) = PX(jl, ik + 12, f) + BL(jl, 14) * zdz * zdt;
+          PX(jl, ik + 13, f) = PX(jl, ik + 13, f) + BL(jl, 15) * zdz * zdt;
+          PX(jl, ik + 14, f) = PX(jl, ik + 14, f) + BL(jl, 12) * zdz * zdt;
+          PX(jl, ik + 15, f) = PX(jl, ik + 15, f) + BL(jl, 13) * zdz * zdt;
+          PX(jl, ik + 16, f) = PX(jl, ik + 16, f) + BL(jl, 17) * zdz * zdt;
+          PX(jl, ik + 17, f) = PX(jl, ik + 17, f) + (BL(jl, 18) + BL(jl, 19)) * zdz * zdt;
+          PX(jl, ik + 18, f) = PX(jl, ik + 18, f) + BL(jl, 20) * zdz * zdt;
+          PX(jl, ik + 19, f) = PX(jl, ik + 19, f) + BL(jl, 21) * zdz * zdt;
+          PX(jl, ik + 20, f) = PX(jl, ik + 20, f) + zpracs * zdz * zdt * ZQTMST;
+          PX(jl, ik + 21, f) = PX(jl, ik + 21, f) + zpsacr[i] * zdz * zdt * ZQTMST;
+          PX(jl, ik + 22, f) = PX(jl, ik + 22, f) + zpraci * zdz * zdt * ZQTMST;
+          PX(jl, ik + 23, f) = PX(jl, ik + 23, f) + zpiacr[i] * zdz * zdt * ZQTMST;
+          PX(jl, ik + 24, f) = PX(jl, ik + 24, f) + a.pvfl[k] * zdz * zdt;
+          PX(jl, ik + 25, f) = PX(jl, ik + 25, f) + b.pdynl[k] * zdz * zdt;
+          ik = 39;
+          PX(jl, ik + 5, f) = PX(jl, ik + 5, f) + BI(jl, 2) * zdz * zdt;
+          PX(jl, ik + 7, f) = PX(jl, ik + 7, f) + BI(jl, 3) * zdz * zdt;
+          PX(jl, ik + 8, f) = PX(jl, ik + 8, f) + (BI(jl, 4) + BI(jl, 6)) * zdz * zdt;
+          PX(jl, ik + 9, f) = PX(jl, ik + 9, f) + BI(jl, 5) * zdz * zdt;
+          PX(jl, ik + 10, f) = PX(jl, ik + 10, f) + BI(jl, 7) * zdz * zdt;
+          PX(jl, ik + 11, f) = PX(jl, ik + 11, f) + BI(jl, 11) * zdz * zdt;
+          PX(jl, ik + 12, f) = PX(jl, ik + 12, f) + (BI(jl, 12) + BI(jl, 13)) * zdz * zdt;
+          PX(jl, ik + 13, f) = PX(jl, ik + 13, f) + BI(jl, 14) * zdz * zdt;
+          PX(jl, ik + 14, f) = PX(jl, ik + 14, f) + BI(jl, 16) * zdz * zdt;
+          PX(jl, ik + 15, f) = PX(jl, ik + 15, f) + BI(jl, 17) * zdz * zdt;
+          PX(jl, ik + 16, f) = PX(jl, ik + 16, f) + zpsdep[i] * ZQTMST * zdz * zdt;
+          PX(jl, ik + 17, f) = PX(jl, ik + 17, f) + a.pvfi[k] * zdz * zdt;
+          PX(jl, ik + 18, f) = PX(jl, ik + 18, f) + b.pdyni[k] * zdz * zdt;
+          PX(jl, ik + 19, f) = PX(jl, ik + 19, f) + b.pdyns[k] * zdz * zdt;
+          PX(jl, 60, f) = a.prainfrac_toprfz[i];
+          ik = 70;
+          const int kt = a.ktype[i], kp = b.kpbltype[i];
+          const R eis = a.peis[i];
+          if (kt == 0) PX(jl, ik, f) = PX(jl, ik, f) + R(1);
+          if (kt == is + 1) PX(jl, ik + 1, f) = PX(jl, ik + 1, f) + R(1);  // IS+1 as in the reference
+          if (kt == 2) PX(jl, ik + 2, f) = PX(jl, ik + 2, f) + R(1);
+          if (kt == 3) PX(jl, ik + 3, f) = PX(jl, ik + 3, f) + R(1);
+          if (kp == 0) PX(jl, ik + 4, f) = PX(jl, ik + 4, f) + R(1);
+          if (kp == is + 1) PX(jl, ik + 5, f) = PX(jl, ik + 5, f) + R(1);
+          if (kp == 2) PX(jl, ik + 6, f) = PX(jl, ik + 6, f) + R(1);
+          if (kp == 3) PX(jl, ik + 7, f) = PX(jl, ik + 7, f) + R(1);
+          PX(jl, ik + 8, f) = PX(jl, ik + 8, f) + eis;
+          if (eis > R(6.0)) PX(jl, ik + 9, f) = PX(jl, ik + 9, f) + R(1);
+          if (eis > R(8.0)) PX(jl, ik + 10, f) = PX(jl, ik + 10, f) + R(1);
+          if (eis > R(10.0)) PX(jl, ik + 11, f) = PX(jl, ik + 11, f) + R(1);
+          if (kt >= 2 && eis < R(p.reisthsc)) PX(jl, ik + 12, f) = PX(jl, ik + 12, f) + R(1);
+          if (kt > 0) PX(jl, ik + 13, f) = R(b.kctop[i]);
+        }
+        is = is + 1;
+      }
+      if (lcldbud & CLOUDSC_BUD_C) {  // :3948-3966
+        for (int jl = kidia; jl <= kfdia; ++jl) {
+          const size_t k = i2(jl, jk);
+          PX(jl, jk, is + 4) = PX(jl, jk, is + 4) + BCC(jl, 2) * zdt;
+          PX(jl, jk, is + 6) = PX(jl, jk, is + 6) + BCC(jl, 3) * zdt;
+          PX(jl, jk, is + 7) = PX(jl, jk, is + 7) + (BCC(jl, 4) + BCC(jl, 6)) * zdt;
+          PX(jl, jk, is + 8) = PX(jl, jk, is + 8) + BCC(jl, 5) * zdt;
+          PX(jl, jk, is + 9) = PX(jl, jk, is + 9) + BCC(jl, 7) * zdt;
+          PX(jl, jk, is + 11) = PX(jl, jk, is + 11) + b.pvfa[k] * zdt;
+          PX(jl, jk, is + 12) = PX(jl, jk, is + 12) + b.pdyna[k] * zdt;
+        }
+        is = is + 12;
+      }
+      if (lcldbud & CLOUDSC_BUD_L) {  // :3971-4003
+        for (int jl = kidia; jl <= kfdia; ++jl) {
+          const size_t k = i2(jl, jk);
+          PX(jl, jk, is + 5) = PX(jl, jk, is + 5) + BL(jl, 2) * zdt;
+          PX(jl, jk, is + 7) = PX(jl, jk, is + 7) + BL(jl, 3) * zdt;
+          PX(jl, jk, is + 8) = PX(jl, jk, is + 8) + (BL(jl, 4) + BL(jl, 6)) * zdt;
+          PX(jl, jk, is + 9) = PX(jl, jk, is + 9) + BL(jl, 5) * zdt;
+          PX(jl, jk, is + 10) = PX(jl, jk, is + 10) + BL(jl, 7) * zdt;
+          PX(jl, jk, is + 11) = PX(jl, jk, is + 11) + BL(jl, 11) * zdt;
+          PX(jl, jk, is + 12) = PX(jl, jk, is + 12) + BL(jl, 12) * zdt;
+          PX(jl, jk, is + 13) = PX(jl, jk, is + 13) + BL(jl, 13) * zdt;
+          PX(jl, jk, is + 14) = PX(jl, jk, is + 14) + BL(jl, 14) * zdt;
+          PX(jl, jk, is + 15) = PX(jl, jk, is + 15) + BL(jl, 15) * zdt;
+          PX(jl, jk, is + 16) = PX(jl, jk, is + 16) + BL(jl, 17) * zdt;
+          PX(jl, jk, is + 17) = PX(jl, jk, is + 17) + (BL(jl, 18) + BL(jl, 19)) * zdt;
+          PX(jl, jk, is + 18) = PX(jl, jk, is + 18) + BL(jl, 20) * zdt;
+          PX(jl, jk, is + 19) = PX(jl, jk, is + 19) + BL(jl, 21) * zdt;
+          PX(jl, jk, is + 20) = PX(jl, jk, is + 20) + a.pvfl[k] * zdt;
+          PX(jl, jk, is + 21) = PX(jl, jk, is + 21) + b.pdynl[k] * zdt;
+          PX(jl, jk, is + 22) = PX(jl, jk, is + 22) + V(zqxn, jl, QL);
+        }
+        is = is + 22;
+      }
+      if (lcldbud & CLOUDSC_BUD_I) {  // :4008-4041
+        for (int jl = kidia; jl <= kfdia; ++jl) {
+          const int i = jl - 1;
+          const size_t k = i2(jl, jk);
+          PX(jl, jk, is + 5) = PX(jl, jk, is + 5) + BI(jl, 2) * zdt;
+          PX(jl, jk, is + 7) = PX(jl, jk, is + 7) + BI(jl, 3) * zdt;
+          PX(jl, jk, is + 8) = PX(jl, jk, is + 8) + (BI(jl, 4) + BI(jl, 6)) * zdt;
+          PX(jl, jk, is + 9) = PX(jl, jk, is + 9) + BI(jl, 5) * zdt;
+          PX(jl, jk, is + 10) = PX(jl, jk, is + 10) + BI(jl, 7) * zdt;
+          PX(jl, jk, is + 11) = PX(jl, jk, is + 11) + BI(jl, 11) * zdt;
+          PX(jl, jk, is + 12) = PX(jl, jk, is + 12) + (BI(jl, 12) + BI(jl, 13)) * zdt;
+          PX(jl, jk, is + 13) = PX(jl, jk, is + 13) + BI(jl, 14) * zdt;
+          PX(jl, jk, is + 14) = PX(jl, jk, is + 14) + (BI(jl, 15) + BI(jl, 16)) * zdt;
+          PX(jl, jk, is + 15) = PX(jl, jk, is + 15) + zpsdep[i] * ZQTMST * zdt;
+          PX(jl, jk, is + 16) = PX(jl, jk, is + 16) + BI(jl, 17) * zdt;
+          PX(jl, jk, is + 17) = PX(jl, jk, is + 17) + a.pvfi[k] * zdt;
+          PX(jl, jk, is + 18) = PX(jl, jk, is + 18) + zpievap * ZQTMST * zdt;
+          PX(jl, jk, is + 19) = PX(jl, jk, is + 19) + b.pdyni[k] * zdt;
+          PX(jl, jk, is + 20) = PX(jl, jk, is + 20) + V(zqxn, jl, QI);
+        }
+        is = is + 20;
+      }
+      if (lcldbud & CLOUDSC_BUD_T) {  // :4052-4101
+        for (int jl = kidia; jl <= kfdia; ++jl) {
+          const size_t k = i2(jl, jk);
+          PX(jl, jk, is + 1) = PX(jl, jk, is + 1) + b.phrsw[k];
+          PX(jl, jk, is + 2) = PX(jl, jk, is + 2) + b.phrlw[k];
+          PX(jl, jk, is + 3) = PX(jl, jk, is + 3) +
+                               ((BL(jl, 1) + BL(jl, 2) + BL(jl, 9) + BL(jl, 10)) * RALVDCP - BI(jl, 11) * RALVDCP) * zdt;
+          PX(jl, jk, is + 4) = PX(jl, jk, is + 4) +
+                               (BI(jl, 1) + BI(jl, 2) + BI(jl, 9) + BI(jl, 10) + BI(jl, 11)) * RALSDCP * zdt;
+          PX(jl, jk, is + 5) = PX(jl, jk, is + 5) + BI(jl, 11) * RALFDCP * zdt;
+          PX(jl, jk, is + 6) = PX(jl, jk, is + 6) + (BL(jl, 7) + BL(jl, 8) + BL(jl, 5)) * RALVDCP * zdt;
+          PX(jl, jk, is + 7) = PX(jl, jk, is + 7) + (BI(jl, 7) + BI(jl, 8) + BI(jl, 5)) * RALSDCP * zdt;
+          PX(jl, jk, is + 8) = PX(jl, jk, is + 8) + BL(jl, 20) * RALVDCP * zdt;
+          PX(jl, jk, is + 9) = PX(jl, jk, is + 9) + BI(jl, 17) * RALSDCP * zdt;
+          PX(jl, jk, is + 10) = PX(jl, jk, is + 10) + BI(jl, 15) * RALFDCP * zdt;
+          PX(jl, jk, is + 11) = PX(jl, jk, is + 11) + BI(jl, 16) * RALFDCP * zdt;
+          PX(jl, jk, is + 12) = PX(jl, jk, is + 12) - (BL(jl, 12) + BL(jl, 13) + BL(jl, 18) + BL(jl, 19)) * RALFDCP * zdt;
+          PX(jl, jk, is + 13) = PX(jl, jk, is + 13) + BL(jl, 21) * RALFDCP * zdt;
+        }
+      }
+    }
   }  // level loop
 
   // ---- 8. fluxes (:4218-4303) ----

@@ -1,0 +1,484 @@
+"""CLOUDSC's budget diagnostics: the process rates ZBUDCC/ZBUDL/ZBUDI (zeroed per level at cloudsc.F90:1271-1273,
+set where the processes run) written into PEXTRA by the LCLDBUD* switches (:3826-4101).
+
+CPU pins of the oracle, each chosen so that a wrong sign, a dropped term or a wrong field index fails:
+  - closure: where the tidy-up, the sink truncation and the cloud-cover clamps do not act, the change of cloud
+    liquid (above RTHOMO), cloud ice (above RTT) and cloud cover over the step equals dt times the sum of the
+    process fields; the final-condensate fields and the supersaturation fields reproduce the tendencies;
+  - the temperature budget against the liquid and ice budgets (latent heats times the matching fields);
+  - the vertical integrals against the 3D budgets integrated with dz = dp/(rho g), and the closed forms of the
+    convection, boundary-layer and stability counters (slots 60-83);
+  - LCLDBUD_TIMEINT scaling, the LBUD23 offset (with the reference's KTYPE == IS+1 comparison), accumulation
+    into PEXTRA, and tendencies unchanged by the budgets.
+GPU: the budget entries against the oracle (FP64 1e-12; FP32 and mixed as accurate as the FP32 oracle), and the
+per-block entry against the blocked one bit for bit."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from openifs_amd import cloudsc as C
+from tests.helpers import columns, relrms
+
+TSPHY = 900.0
+THREE_D = C.BUD_C | C.BUD_L | C.BUD_I | C.BUD_T
+ALL = THREE_D | C.BUD_VERTINT
+TENDENCIES = ["ptendency_loc_t", "ptendency_loc_q", "ptendency_loc_a", "ptendency_loc_cld", "pcovptot",
+              "prainfrac_toprfz", "pfplsl", "pfplsn", "pfsqlf", "pfsqif"]
+
+
+def with_flags(p, flags):
+    q = type(p).from_buffer_copy(p)
+    q.lcldbud = flags
+    return q
+
+
+def offsets(flags):
+    """IS before each budget block (PEXTRA(:,:,IS+n) is field n of that block), as the reference advances it."""
+    is_, off = (26 if flags & C.BUD_LBUD23 else 0), {}
+    for key, bit, n in (("V", C.BUD_VERTINT, 1), ("C", C.BUD_C, 12), ("L", C.BUD_L, 22), ("I", C.BUD_I, 20),
+                        ("T", C.BUD_T, 13)):
+        if flags & bit:
+            off[key] = is_
+            is_ += n
+    return off
+
+
+def budget_inputs(p, ngptot, nproma, klev, seed):
+    """The synthetic atmosphere plus random inputs that only the budgets read."""
+    s = C.synth_host(p, seed, ngptot, nproma, klev, precision="dp")
+    rng = np.random.default_rng(seed + 7919)
+    ngpblks = (ngptot + nproma - 1) // nproma
+    sh2, sh1 = (ngpblks, klev, nproma), (ngpblks, nproma)
+    for n, sd in (("pvfa", 1e-5), ("pdyna", 1e-5), ("pdynl", 1e-9), ("pdyni", 1e-9), ("pdyns", 1e-9),
+                  ("phrsw", 1e-5), ("phrlw", 1e-5)):
+        s[n] = rng.normal(0.0, sd, sh2)
+    s["kpbltype"] = rng.integers(0, 4, sh1).astype(np.int32)
+    s["kctop"] = rng.integers(1, klev + 1, sh1).astype(np.int32)
+    return s
+
+
+def run_oracle(prec, p, inp, ngptot, nproma, klev, flags, kfldx=None, pextra0=None):
+    dt = np.float64 if prec == "dp" else np.float32
+    ngpblks = (ngptot + nproma - 1) // nproma
+    kfldx = C.budget_fields(flags) if kfldx is None else kfldx
+    arr = {k: (v.astype(dt) if v.dtype.kind == "f" else v.copy()) for k, v in inp.items()}
+    for name, kind, role in C.BLOCKED_ARRAYS:
+        if role == "o":
+            arr[name] = np.full(C.blocked_shape(kind, nproma, klev, ngpblks), np.nan, dtype=dt)
+    shape = C.extra_shape(nproma, klev, max(kfldx, 1), ngpblks)
+    arr["pextra"] = np.zeros(shape, dtype=dt) if pextra0 is None else pextra0.astype(dt).copy()
+    rc = oracle.cloudsc_blocked_budget(prec, with_flags(p, flags), ngptot, nproma, klev, TSPHY, arr, nthreads=8)
+    assert rc == 0, rc
+    return arr
+
+
+class Fields:
+    """PEXTRA fields and state of one oracle run as (columns, levels) FP64 arrays."""
+
+    def __init__(self, p, inp, out, flags, nproma, ngptot):
+        self.off, self.px, self.nproma, self.n = offsets(flags), out["pextra"], nproma, ngptot
+        col = lambda x: columns(np.asarray(x, dtype=np.float64), nproma)[:ngptot]
+        self.col = col
+        self.out = {k: col(v) for k, v in out.items() if k != "pextra"}
+        self.inp = {k: col(v) for k, v in inp.items()}
+        klev = self.out["ptendency_loc_t"].shape[1]
+        self.lev = np.arange(klev) >= p.ncldtop - 1
+
+    def __call__(self, key, n):
+        return self.col(self.px[:, self.off[key] + n - 1])
+
+
+@pytest.fixture(scope="module")
+def case(params137):
+    p, ngptot, nproma, klev = params137, 2048, 64, 137
+    inp = budget_inputs(p, ngptot, nproma, klev, 5)
+    out = run_oracle("dp", p, inp, ngptot, nproma, klev, ALL)
+    return p, inp, out, Fields(p, inp, out, ALL, nproma, ngptot)
+
+
+def _initial_state(p, f):
+    """ZQX after the tidy-up (cloudsc.F90:1127-1175) for liquid and ice, ZAORIG, and the tidy mask."""
+    cml = f.inp["ptendency_cml_cld"]
+    ql0 = f.inp["pclv"][:, 0] + TSPHY * cml[:, 0]
+    qi0 = f.inp["pclv"][:, 1] + TSPHY * cml[:, 1]
+    a0 = f.inp["pa"] + TSPHY * f.inp["ptendency_cml_a"]
+    tidy = (ql0 + qi0 < p.rlmin) | (a0 < p.ramin)
+    ql = np.where(tidy | (ql0 < p.rlmin), 0.0, ql0)
+    qi = np.where(tidy | (qi0 < p.rlmin), 0.0, qi0)
+    t1 = f.inp["pt"] + TSPHY * f.inp["ptendency_cml_t"]
+    return ql0, qi0, ql, qi, a0, tidy, t1
+
+
+def _closure(lhs, terms, scale_extra):
+    rhs = TSPHY * sum(terms)
+    scale = np.abs(lhs) + scale_extra + TSPHY * sum(np.abs(t) for t in terms)
+    return np.abs(lhs - rhs) <= 1e-11 * scale + 2e-14
+
+
+def test_liquid_closure(case):
+    """Above RTHOMO (no homogeneous freezing) and without truncation of the liquid sinks (the explicit net change
+    keeps ZQX >= 0), QXN_L - ZQX_L = dt * (detrainment + subsidence + subsidence evaporation + erosion +
+    deposition + autoconversion + accretion + riming); rain-only fields (16-18) are not part of it."""
+    p, inp, out, f = case
+    ql0, qi0, ql, qi, a0, tidy, t1 = _initial_state(p, f)
+    L = lambda n: f("L", n)
+    expl = L(7) + L(8) + L(9) + L(10) + L(11)  # field 8 holds the implicit L6 <= 0 too: a sufficient test
+    ok = f.lev[None, :] & (t1 > p.rthomo) & (ql + TSPHY * expl >= 0)
+    terms = [L(7), L(8), L(9), L(10), L(11), L(14), L(15), L(19)]
+    good = _closure(L(22) - ql, terms, ql)
+    assert ok.mean() > 0.25
+    assert np.all(good[ok]), np.argwhere(ok & ~good)[:5]
+    # every term is exercised on the points the closure covers
+    for n in (7, 8, 9, 10, 11, 14, 15, 19):
+        assert np.count_nonzero(L(n)[ok]) > 100, n
+    np.testing.assert_array_equal(L(12), 0.0)
+    np.testing.assert_array_equal(L(13), 0.0)
+
+
+@pytest.fixture(scope="module")
+def case_norain(params137):
+    """The same atmosphere without rain in the inputs: ZRAINCLD (from the input rain of the level and the one
+    above) is zero, so ice accreting rain (PIACR, recorded only in the vertical integral) cannot act."""
+    p, ngptot, nproma, klev = params137, 2048, 64, 137
+    inp = budget_inputs(p, ngptot, nproma, klev, 5)
+    inp["pclv"][:, 2] = 0.0
+    inp["ptendency_cml_cld"][:, 2] = 0.0
+    out = run_oracle("dp", p, inp, ngptot, nproma, klev, ALL)
+    return p, inp, out, Fields(p, inp, out, ALL, nproma, ngptot)
+
+
+def test_ice_closure(case_norain):
+    """Above RTHOMO without rain, QXN_I - ZQX_I = dt * (detrainment + subsidence + evaporation + erosion +
+    deposition + sedimentation + autoconversion to snow - melting), the melting of ice being the liquid budget's
+    field 16 (ZBUDL(17) = -ZBUDI(15)).  Excluded: truncation of the ice sinks or of the liquid sinks (which
+    scales the deposition gain, cloudsc.F90:3531-3532) and ice below RLMIN in the first guess, which is
+    evaporated without a budget entry (:3402-3412)."""
+    p, inp, out, f = case_norain
+    ql0, qi0, ql, qi, a0, tidy, t1 = _initial_state(p, f)
+    I, L = (lambda n: f("I", n)), (lambda n: f("L", n))
+    expl = I(7) + I(8) + I(9) + I(10) + I(11) + I(12) - L(16)  # fields 8 and 12 hold implicit sinks <= 0 too
+    lexpl = L(7) + L(8) + L(9) + L(10) + L(11)
+    first_guess = qi + TSPHY * (I(11) + I(12) - L(16))  # a lower bound of ZQXFG(NCLDQI)
+    ok = (f.lev[None, :] & (t1 > p.rthomo) & (qi + TSPHY * expl >= 0) & (ql + TSPHY * lexpl >= 0)
+          & (first_guess >= p.rlmin))
+    terms = [I(7), I(8), I(9), I(10), I(11), I(12), I(13), -L(16)]
+    good = _closure(I(20) - qi, terms, qi)
+    assert ok.mean() > 0.1
+    assert np.all(good[ok]), np.argwhere(ok & ~good)[:5]
+    for n in (7, 8, 9, 10, 11, 12, 13):
+        assert np.count_nonzero(I(n)[ok]) > 1000, n
+    assert np.count_nonzero(L(16)[ok]) > 50
+    # above RTT field 14 (melting of ice and snow) is <= 0 and holds the ice melting
+    warm = ok & (t1 > p.rtt)
+    assert np.all(I(14)[warm] <= -L(16)[warm] + 1e-30)
+
+
+def test_process_regimes(case, case_norain):
+    """Fields of processes that share a closure sum are told apart by where they can act: deposition, riming,
+    freezing and snow autoconversion only at or below RTT, melting only above it, accretion only with rain in
+    the inputs (the no-rain case keeps autoconversion and loses accretion)."""
+    p, inp, out, f = case
+    t1 = _initial_state(p, f)[6]
+    L, I = (lambda n: f("L", n)), (lambda n: f("I", n))
+    warm, cold = f.lev[None, :] & (t1 > p.rtt), f.lev[None, :] & (t1 <= p.rtt)
+    for key, n in (("L", 11), ("L", 17), ("L", 19), ("I", 11), ("I", 13), ("I", 15)):
+        assert np.all(f(key, n)[warm] == 0), (key, n)
+        assert np.count_nonzero(f(key, n)[cold]) > 100, (key, n)
+    for key, n in (("L", 16), ("I", 14)):
+        assert np.all(f(key, n)[cold] == 0), (key, n)
+        assert np.count_nonzero(f(key, n)[warm]) > 50, (key, n)
+    p2, inp2, out2, f2 = case_norain
+    assert np.all(f2("L", 15) == 0) and np.count_nonzero(f2("L", 14)) > 1000
+    assert np.count_nonzero(L(15)) > 1000 and np.count_nonzero(L(14)) > 1000
+    # ice sedimentation acts at all temperatures
+    assert np.count_nonzero(I(12)[warm]) > 100 and np.count_nonzero(I(12)[cold]) > 100
+
+
+def test_cloud_cover_closure(case):
+    """Where the tidy-up leaves ZA = ZAORIG in [0,1] and the new cover is not clamped, the cover tendency less the
+    supersaturation term is the sum of the detrainment, subsidence (source, evaporation, sink) and erosion
+    fields (cloudsc.F90:3429-3438)."""
+    p, inp, out, f = case
+    ql0, qi0, ql, qi, a0, tidy, t1 = _initial_state(p, f)
+    Cf = lambda n: f("C", n)
+    ta = f.out["ptendency_loc_a"]
+    anew = a0 + TSPHY * (ta - Cf(4))
+    ok = f.lev[None, :] & ~tidy & (a0 >= 0) & (a0 <= 1) & (anew > p.ramin) & (anew < 1 - 1e-9)
+    lhs = TSPHY * (ta - Cf(4))
+    good = _closure(lhs, [Cf(6), Cf(7), Cf(8), Cf(9)], np.abs(a0))
+    assert ok.mean() > 0.2
+    assert np.all(good[ok]), np.argwhere(ok & ~good)[:5]
+    for n in (6, 7, 9):
+        assert np.count_nonzero(Cf(n)[ok]) > 100, n
+    np.testing.assert_array_equal(Cf(11), f.col(inp["pvfa"]) * f.lev[None, :])
+    np.testing.assert_array_equal(Cf(12), f.col(inp["pdyna"]) * f.lev[None, :])
+
+
+def test_condensate_fields_reproduce_tendencies(case):
+    """PTENDENCY_LOC_CLD = (QXN - ZQX0)/dt + the supersaturation adjustment (fields L5, I5) wherever the final
+    negative-condensate fix (cloudsc.F90:3795-3808) did not act."""
+    p, inp, out, f = case
+    ql0, qi0, ql, qi, a0, tidy, t1 = _initial_state(p, f)
+    tc = f.out["ptendency_loc_cld"]
+    for j, q0, key in ((0, ql0, "L"), (1, qi0, "I")):
+        fin, sup = f(key, 22 if key == "L" else 20), f(key, 5)
+        ok = f.lev[None, :] & (q0 + TSPHY * tc[:, j] > 1e-12)
+        rhs = (fin - q0) / TSPHY + sup
+        err = np.abs(tc[:, j] - rhs)
+        assert np.all(err[ok] <= 1e-12 * (np.abs(fin) + np.abs(q0))[ok] / TSPHY + 1e-12 * np.abs(sup[ok]) + 1e-25)
+        assert np.count_nonzero(sup[ok]) > 100
+    # cloud cover: field C4 is the supersaturation check's cover change
+    assert np.count_nonzero(f("C", 4)) > 100
+
+
+def test_temperature_budget_matches_liquid_and_ice(case):
+    """Each temperature field is a latent heat times fields of the liquid and ice budgets (cloudsc.F90:4052-4101):
+    a wrong field index in either block breaks one of these."""
+    p, inp, out, f = case
+    T, L, I = (lambda n: f("T", n)), (lambda n: f("L", n)), (lambda n: f("I", n))
+    lv, ls, lf = p.ralvdcp, p.ralsdcp, p.ralfdcp
+    pairs = [(3, (L(5) - I(11)) * lv), (4, (I(5) + I(11)) * ls), (5, I(11) * lf), (6, (L(10) + L(9)) * lv),
+             (7, (I(10) + I(9)) * ls), (8, L(18) * lv), (9, I(16) * ls), (10, -L(16) * lf),
+             (11, (I(14) + L(16)) * lf), (12, -L(17) * lf), (13, L(19) * lf)]
+    for n, ref in pairs:
+        scale = np.max(np.abs(ref)) + 1e-300
+        assert np.max(np.abs(T(n) - ref)) <= 1e-13 * scale, n
+        assert np.count_nonzero(ref) > 50, n
+    np.testing.assert_array_equal(T(1), f.col(inp["phrsw"]) * f.lev[None, :])
+    np.testing.assert_array_equal(T(2), f.col(inp["phrlw"]) * f.lev[None, :])
+
+
+def test_vertical_integrals(case):
+    """The vertical-integral field (cloudsc.F90:3849-3943) against the 3D budgets integrated over the levels
+    with dz = dp/(rho g), rho = PAP/(RD T); the counter slots against their closed forms."""
+    p, inp, out, f = case
+    ql0, qi0, ql, qi, a0, tidy, t1 = _initial_state(p, f)
+    V = f.col(f.px[:, f.off["V"]])  # (columns, slots)
+    paph = f.inp["paph"]
+    dz = (paph[:, 1:] - paph[:, :-1]) / (f.inp["pap"] / (p.rd * t1) * p.rg) * f.lev[None, :]
+    integ = lambda x: (x * dz).sum(axis=1)
+    Cf, L, I = (lambda n: f("C", n)), (lambda n: f("L", n)), (lambda n: f("I", n))
+    pairs = {4: Cf(4), 6: Cf(6), 7: Cf(7), 8: Cf(8), 9: Cf(9), 11: Cf(11), 12: Cf(12),
+             19: L(5), 21: L(7), 22: L(8), 23: L(9), 24: L(10), 25: L(11), 26: L(14), 27: L(15), 28: L(12),
+             29: L(13), 30: L(16), 31: L(17), 32: L(18), 33: L(19), 38: L(20), 39: L(21),
+             44: I(5), 46: I(7), 47: I(8), 48: I(9), 49: I(10), 50: I(11), 51: I(12), 52: I(13),
+             53: I(14) + L(16), 54: I(16), 55: I(15), 56: I(17), 57: I(19)}
+    for s, x in pairs.items():
+        ref = integ(x)
+        scale = np.sum(np.abs(x) * dz, axis=1) + 1e-300
+        assert np.all(np.abs(V[:, s - 1] - ref) <= 1e-12 * scale), s
+    np.testing.assert_allclose(V[:, 57], integ(f.col(inp["pdyns"])), rtol=1e-12, atol=0)
+    # ZPRACS and ZPRACI are zero on the default path; ZPSACR and ZPIACR are accretion amounts >= 0
+    assert np.all(V[:, 33] == 0) and np.all(V[:, 35] == 0)
+    assert np.all(V[:, 34] >= 0) and np.all(V[:, 36] >= 0)
+    assert np.count_nonzero(V[:, 34]) > 10 and np.count_nonzero(V[:, 36]) > 10
+    # untouched slots
+    used = set(pairs) | {10, 34, 35, 36, 37, 58, 60} | set(range(70, 84))
+    for s in range(1, 138):
+        if s not in used:
+            assert np.all(V[:, s - 1] == 0), s
+    # counters: one increment per level NCLDTOP..KLEV
+    nlev = f.lev.sum()
+    kt, kp = f.col(inp["ktype"]), f.col(inp["kpbltype"])
+    eis = f.out["peis"]
+    np.testing.assert_array_equal(V[:, 59], f.out["prainfrac_toprfz"])
+    for s, cond in ((70, kt == 0), (71, kt == 1), (72, kt == 2), (73, kt == 3), (74, kp == 0), (75, kp == 1),
+                    (76, kp == 2), (77, kp == 3), (79, eis > 6), (80, eis > 8), (81, eis > 10),
+                    (82, (kt >= 2) & (eis < p.reisthsc))):
+        np.testing.assert_array_equal(V[:, s - 1], nlev * cond, err_msg=str(s))
+    np.testing.assert_allclose(V[:, 77], nlev * eis, rtol=1e-14)
+    np.testing.assert_array_equal(V[:, 82], np.where(kt > 0, f.col(inp["kctop"]), 0.0))
+
+
+def test_vertical_integral_needs_83_levels(params137):
+    p = params137
+    q = with_flags(p, C.BUD_VERTINT)
+    nproma, klev = 8, 82
+    arr = {n: np.zeros(C.blocked_shape(k, nproma, klev, 1), dtype=np.int32 if k == "1di" else np.float64)
+           for n, k, _ in C.BLOCKED_ARRAYS}
+    for n, k in C.BUDGET_ARRAYS:
+        arr[n] = np.zeros(C.blocked_shape(k, nproma, klev, 1), dtype=np.int32 if k == "1di" else np.float64)
+    arr["pextra"] = np.zeros(C.extra_shape(nproma, klev, 1, 1))
+    q.ncldtop = 2
+    assert oracle.cloudsc_blocked_budget("dp", q, nproma, nproma, klev, TSPHY, arr) == -1
+
+
+def test_timeint_lbud23_and_accumulation(case, params137):
+    """LCLDBUD_TIMEINT multiplies every rate by PTSPHY (not the final condensate, the heating rates or the
+    counters); LBUD23 moves everything 26 fields up, leaves the first 26 alone and makes the KTYPE/KPBLTYPE == 1
+    counters compare with IS+1 = 27 as the reference does; PEXTRA accumulates."""
+    p, inp, out, f = case
+    ngptot, nproma, klev = 2048, 64, 137
+    rng = np.random.default_rng(3)
+    # TIMEINT
+    o2 = run_oracle("dp", p, inp, ngptot, nproma, klev, ALL | C.BUD_TIMEINT)
+    f2 = Fields(p, inp, o2, ALL | C.BUD_TIMEINT, nproma, ngptot)
+    keep = {("L", 22), ("I", 20), ("T", 1), ("T", 2)}
+    for key, n in [("C", k) for k in range(1, 13)] + [("L", k) for k in range(1, 23)] + \
+                  [("I", k) for k in range(1, 21)] + [("T", k) for k in range(1, 14)]:
+        want = f(key, n) if (key, n) in keep else f(key, n) * TSPHY
+        np.testing.assert_allclose(f2(key, n), want, rtol=1e-15, atol=0, err_msg=f"{key}{n}")
+    V1, V2 = f.col(f.px[:, 0]), f2.col(f2.px[:, 0])
+    rates = [s for s in range(1, 60) if s not in (58,)]
+    a, b = V2[:, [s - 1 for s in rates]], V1[:, [s - 1 for s in rates]] * TSPHY
+    assert np.all(np.abs(a - b) <= 1e-12 * np.abs(b).max(axis=0))  # sums over levels of cancelling terms
+    np.testing.assert_array_equal(V2[:, 59:], V1[:, 59:])
+    # LBUD23 on top of a random PEXTRA
+    kf = C.budget_fields(ALL | C.BUD_LBUD23)
+    assert kf == 26 + 68
+    ngpblks = ngptot // nproma
+    init = rng.normal(0, 1, C.extra_shape(nproma, klev, kf, ngpblks))
+    o3 = run_oracle("dp", p, inp, ngptot, nproma, klev, ALL | C.BUD_LBUD23, pextra0=init)
+    np.testing.assert_array_equal(o3["pextra"][:, :26], init[:, :26])
+    d = o3["pextra"][:, 26:] - init[:, 26:]
+    base = out["pextra"]
+    put = np.zeros(d.shape[1:], dtype=bool)
+    put[0, 59, :] = put[0, 82, :] = True  # slots assigned, not accumulated
+    put[0, 70, :] = put[0, 74, :] = True  # the KTYPE/KPBLTYPE == IS+1 counters, checked below
+    m = ~np.broadcast_to(put, d.shape)
+    np.testing.assert_allclose(d[m], base[m], rtol=1e-12, atol=1e-14)
+    np.testing.assert_array_equal(o3["pextra"][:, 26, 59], base[:, 0, 59])
+    V3 = f.col(o3["pextra"][:, 26] - init[:, 26])
+    assert np.all(np.abs(V3[:, 70]) < 1e-14) and np.all(np.abs(V3[:, 74]) < 1e-14)  # KTYPE == 27: never
+    assert np.any(V1[:, 70] > 0) and np.any(V1[:, 74] > 0)
+    # the tendencies do not depend on the budgets
+    o0 = run_oracle("dp", p, inp, ngptot, nproma, klev, 0, kfldx=0)
+    for n in TENDENCIES:
+        for o in (out, o2, o3):
+            np.testing.assert_array_equal(o[n], o0[n], err_msg=n)
+
+
+def test_budget_argument_errors(params137):
+    """KFLDX too small, NULL budget arrays and PEXTRA-writing switches on entries without PEXTRA are refused on the
+    host; the oracle applies the same rules."""
+    lib = C.lib()
+    p = with_flags(params137, C.BUD_L | C.BUD_I)
+    nproma, klev, ngptot = 32, 137, 64
+    ngpblks = 2
+    arr = {n: np.zeros(C.blocked_shape(k, nproma, klev, ngpblks), dtype=np.int32 if k == "1di" else np.float32)
+           for n, k, _ in C.BLOCKED_ARRAYS}
+    bud = {n: np.zeros(C.blocked_shape(k, nproma, klev, ngpblks), dtype=np.int32 if k == "1di" else np.float32)
+           for n, k in C.BUDGET_ARRAYS}
+    px = np.zeros(C.extra_shape(nproma, klev, 42, ngpblks), dtype=np.float32)
+
+    def call(q, kfldx, drop=None):
+        args = [ctypes.byref(q), ngptot, nproma, klev, ngpblks, ctypes.c_float(900.0)]
+        args += [ctypes.c_void_p(arr[n].ctypes.data) for n, _, _ in C.BLOCKED_ARRAYS]
+        args += [ctypes.c_void_p(0 if n == drop else bud[n].ctypes.data) for n, _ in C.BUDGET_ARRAYS]
+        args += [ctypes.c_void_p(0 if drop == "pextra" else px.ctypes.data), kfldx, ctypes.c_void_p(0)]
+        return lib.cloudsc_blocked_budget_sp_c(*args)
+
+    assert call(p, 41) == -1  # L + I need 42 fields
+    assert b"KFLDX" in lib.cloudsc_last_error()
+    for n in ("pvfa", "kctop", "pextra"):
+        assert call(p, 42, drop=n) == -3
+    q = with_flags(params137, ALL | C.BUD_LBUD23)
+    assert call(q, 93) == -1
+    q = with_flags(params137, C.BUD_VERTINT)
+    klev_small = 82
+    assert lib.cloudsc_blocked_budget_sp_c(ctypes.byref(q), ngptot, nproma, klev_small, ngpblks,
+                                           ctypes.c_float(900.0),
+                                           *[ctypes.c_void_p(arr[n].ctypes.data) for n, _, _ in C.BLOCKED_ARRAYS],
+                                           *[ctypes.c_void_p(bud[n].ctypes.data) for n, _ in C.BUDGET_ARRAYS],
+                                           ctypes.c_void_p(px.ctypes.data), 42, ctypes.c_void_p(0)) == -1
+    assert b"83" in lib.cloudsc_last_error()
+    # the oracle's rules are the same
+    ar = {k: v.astype(np.float64) if v.dtype.kind == "f" else v for k, v in {**arr, **bud}.items()}
+    ar["pextra"] = px.astype(np.float64)
+    assert oracle.cloudsc_blocked_budget("dp", with_flags(params137, C.BUD_L | C.BUD_I | C.BUD_T), ngptot, nproma,
+                                         klev, TSPHY, ar) == -1
+    # the per-block entry takes PEXTRA: KFLDX is checked there too
+    a = C.allocate_block(8, 137, "sp", kfldx=20)
+    with pytest.raises(C.CloudscError) as e:
+        C.CLOUDSC(with_flags(params137, C.BUD_L), 1, 8, 8, 137, 900.0, a, precision="sp", kfldx=20)
+    assert e.value.code == -1
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+def _gpu_budget_run(p, prec, ngptot, nproma, klev, seed, flags, pextra0=None):
+    import torch
+
+    f = C.BlockedFields(ngptot, nproma, klev, prec, "cuda", fill=float("nan"))
+    C.synth_device(f, p, seed)
+    inp = {n: f[n].cpu().numpy().copy() for n in C.INPUT_NAMES}
+    extra = budget_inputs(p, ngptot, nproma, klev, seed)
+    dt = torch.float64 if prec == "dp" else torch.float32
+    bud = {n: torch.from_numpy(extra[n]).to("cuda", dt if extra[n].dtype.kind == "f" else torch.int32)
+           for n, _ in C.BUDGET_ARRAYS}
+    kfldx = C.budget_fields(flags)
+    ngpblks = f.ngpblks
+    if pextra0 is None:
+        pextra0 = np.random.default_rng(seed).normal(0, 1e-3, C.extra_shape(nproma, klev, kfldx, ngpblks))
+    bud["pextra"] = torch.from_numpy(pextra0).to("cuda", dt)
+    C.cloudsc_blocked_budget(f, bud, with_flags(p, flags), TSPHY)
+    torch.cuda.synchronize()
+    out = {n: f[n].cpu().numpy() for n in C.OUTPUT_NAMES}
+    out["pextra"] = bud["pextra"].cpu().numpy()
+    for n, _ in C.BUDGET_ARRAYS:
+        inp[n] = bud[n].cpu().numpy()
+    return inp, out, pextra0
+
+
+def _pextra_fields(px, nproma, ngptot):
+    return [columns(px[:, j].astype(np.float64), nproma)[:ngptot] for j in range(px.shape[1])]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("flags", [ALL, THREE_D | C.BUD_TIMEINT, ALL | C.BUD_LBUD23 | C.BUD_TIMEINT, C.BUD_L])
+def test_gpu_budget_fp64_matches_oracle(params137, flags):
+    p, ngptot, nproma, klev = params137, 1500, 64, 137
+    print(f"budget fp64 flags={flags}", flush=True)
+    inp, out, px0 = _gpu_budget_run(p, "dp", ngptot, nproma, klev, 17, flags)
+    ref = run_oracle("dp", p, inp, ngptot, nproma, klev, flags, pextra0=px0)
+    for n in TENDENCIES:
+        g, r = (columns(x[n], nproma)[:ngptot] for x in (out, ref))
+        assert relrms(g, r) <= 1e-12, n
+    for j, (g, r) in enumerate(zip(_pextra_fields(out["pextra"], nproma, ngptot),
+                                   _pextra_fields(ref["pextra"], nproma, ngptot))):
+        assert relrms(g - columns(px0[:, j], nproma)[:ngptot], r - columns(px0[:, j], nproma)[:ngptot]) <= 1e-12, j
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["sp", "mixed"])
+def test_gpu_budget_fp32_as_accurate_as_fp32_oracle(params137, prec):
+    p, ngptot, nproma, klev = params137, 4096, 128, 137
+    print(f"budget {prec}", flush=True)
+    zeros = np.zeros(C.extra_shape(nproma, klev, C.budget_fields(ALL), ngptot // nproma))
+    inp, out, _ = _gpu_budget_run(p, prec, ngptot, nproma, klev, 23, ALL, pextra0=zeros)
+    ref64 = run_oracle("dp", p, inp, ngptot, nproma, klev, ALL)
+    ref32 = run_oracle(prec, p, inp, ngptot, nproma, klev, ALL)
+    gs, r64s, r32s = (_pextra_fields(x["pextra"], nproma, ngptot) for x in (out, ref64, ref32))
+    worst = []
+    for j, (g, r64, r32) in enumerate(zip(gs, r64s, r32s)):
+        e_gpu, e_cpu = relrms(g, r64), relrms(r32, r64)
+        worst.append((e_gpu - 2 * e_cpu, j, e_gpu, e_cpu))
+        assert e_gpu <= 2.0 * e_cpu + 1e-5, f"field {j + 1}: GPU {e_gpu:.3e} vs FP32 oracle {e_cpu:.3e}"
+    print(sorted(worst)[-3:])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["dp", "sp"])
+def test_gpu_per_block_budget_equals_blocked(params137, prec):
+    """cloudsc_*_c with PEXTRA/KFLDX (host arrays, one block) reproduces the blocked budget entry bit for bit."""
+    p, nproma, klev = params137, 32, 137
+    flags = ALL | C.BUD_TIMEINT
+    kf = C.budget_fields(flags)
+    inp, out, px0 = _gpu_budget_run(p, prec, 3 * nproma, nproma, klev, 29, flags)
+    rp = "dp" if prec == "dp" else "sp"
+    for b in (0, 2):
+        arrs = C.allocate_block(nproma, klev, rp, kfldx=kf)
+        for n, kind, role in C.BLOCKED_ARRAYS:
+            if role != "o":
+                arrs[n][...] = inp[n][b].T
+        for n, _ in C.BUDGET_ARRAYS:
+            arrs[n][...] = inp[n][b].T
+        arrs["pextra"][...] = px0[b].astype(arrs["pextra"].dtype).transpose(2, 1, 0)
+        C.CLOUDSC(with_flags(p, flags), 1, nproma, nproma, klev, TSPHY, arrs, precision=prec, kfldx=kf)
+        for n, kind, role in C.BLOCKED_ARRAYS:
+            if role != "i":
+                np.testing.assert_array_equal(arrs[n], out[n][b].T, err_msg=f"{prec} block {b} {n}")
+        np.testing.assert_array_equal(arrs["pextra"], out["pextra"][b].transpose(2, 1, 0))
