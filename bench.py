@@ -21,6 +21,8 @@ Reported besides the throughput:
   satadj        CLOUD_SATADJ (the saturation adjustment ahead of CLOUDSC), FP32
                 and mixed, on the same grid: kernel time, algorithmic GB/s and
                 fraction of HBM peak, error against the FP64 oracle, CPU oracle
+  budget        CLOUDSC with the LCLDBUD budget diagnostics into PEXTRA (68
+                fields) against the plain launch on the same grid
 """
 import argparse
 import json
@@ -318,6 +320,64 @@ def fused_bench(params, nproma, steps):
     return out
 
 
+def budget_bytes_per_column(klev, ncldtop, word=4):
+    """Algorithmic HBM bytes per column of the budget launch (C, L, I, T and vertical-integral budgets): the plain
+    CLOUDSC bytes, the seven budget inputs on the levels the scheme solves and KPBLTYPE/KCTOP, and one read and
+    one write of every PEXTRA word the budgets touch: 52 fields on the solved levels (C 7, L 17, I 15, T 13) and
+    57 slots of the vertical-integral field."""
+    nlev = klev - ncldtop + 1
+    return bytes_per_column(klev, word) + word * (7 * nlev + 2 + 2 * (52 * nlev + 57))
+
+
+def budget_bench(params, nproma, steps):
+    """The budget-instrumented CLOUDSC launch (LCLDBUDC/L/I/T and LCLDBUD_VERTINT into PEXTRA, 68 fields) against
+    the plain launch, FP32, on the same 655,360 x L137 workload; PLUDE restored outside the timed events."""
+    import numpy as np
+    import torch
+
+    from openifs_amd import cloudsc as C
+
+    f = C.BlockedFields(NGPTOT, nproma, KLEV, "sp", "cuda:0")
+    C.synth_device(f, params, 20261205)
+    plude0 = f["plude"].clone()
+    flags = C.BUD_C | C.BUD_L | C.BUD_I | C.BUD_T | C.BUD_VERTINT
+    q = type(params).from_buffer_copy(params)
+    q.lcldbud = flags
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(5)
+    sh2, sh1 = (f.ngpblks, KLEV, nproma), (f.ngpblks, nproma)
+    bud = {n: torch.randn(sh2, generator=g, device="cuda:0") * 1e-5 for n, k in C.BUDGET_ARRAYS if k == "2d"}
+    bud["kpbltype"] = torch.randint(0, 4, sh1, generator=g, device="cuda:0", dtype=torch.int32)
+    bud["kctop"] = torch.randint(1, KLEV + 1, sh1, generator=g, device="cuda:0", dtype=torch.int32)
+    bud["pextra"] = torch.zeros(C.extra_shape(nproma, KLEV, C.budget_fields(flags), f.ngpblks), device="cuda:0")
+    stream = torch.cuda.current_stream()
+    out = {}
+    for name, run in (("plain_ms", lambda: C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)),
+                      ("budget_ms", lambda: C.cloudsc_blocked_budget(f, bud, q, 900.0, stream=stream.cuda_stream))):
+        ts = []
+        for i in range(steps + 2):
+            f["plude"].copy_(plude0, non_blocking=True)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            run()
+            b.record(stream)
+            torch.cuda.synchronize()
+            if i >= 2:
+                ts.append(a.elapsed_time(b))
+        out[name] = float(np.mean(ts))
+    bpc = budget_bytes_per_column(KLEV, params.ncldtop)
+    gbs = bpc * NGPTOT / (out["budget_ms"] / 1e3) / 1e9
+    out.update({"overhead": out["budget_ms"] / out["plain_ms"], "columns_per_s": NGPTOT / (out["budget_ms"] / 1e3),
+                "algorithmic_bytes_per_column": bpc,
+                "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+                "workload": f"CLOUDSC FP32 with the C, L, I, T and vertical-integral budgets (KFLDX = 68), "
+                            f"NLEV={KLEV}, {NGPTOT} columns, NPROMA={nproma}"})
+    del f, bud, plude0
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,6 +390,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--no-satadj", action="store_true")
+    ap.add_argument("--no-budget", action="store_true")
     ap.add_argument("--train-cols", type=int, default=20_000_000)
     ap.add_argument("--hybrid-cols", type=int, default=6_599_680)
     ap.add_argument("--rerun-fraction", type=float, default=0.01)
@@ -420,6 +481,10 @@ def main():
         satadj = satadj_bench(params, args.nproma, max(5, min(args.steps, 20)), 2,
                               args.cpu_threads or min(16, os.cpu_count() or 1), 0.0 if args.no_cpu else 3.0)
 
+    budget = None
+    if not args.no_budget:
+        budget = budget_bench(params, args.nproma, max(5, min(args.steps, 10)))
+
     cpu = None
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -452,6 +517,7 @@ def main():
         "err_vs_fp64": err,
         "hybrid": hybrid,
         "satadj": satadj,
+        "budget": budget,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

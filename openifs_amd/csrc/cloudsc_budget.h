@@ -32,18 +32,22 @@ template <typename R, bool ON>
 struct Bud;
 template <typename R>
 struct Bud<R, false> {
+  __device__ __forceinline__ Bud(R*, uint32_t) {}
   __device__ __forceinline__ void set(int, R) {}
   __device__ __forceinline__ R get(int) const { return R(0); }
 };
+// The entries live in LDS (this lane's NBUD words at stride `n`, conflict-free across a wave): the process sites
+// store into them and section 7 reads them back, so they do not hold registers across the level.
 template <typename R>
 struct Bud<R, true> {
-  R v[NBUD];
-  __device__ __forceinline__ Bud() {
+  R* s;
+  uint32_t n;
+  __device__ __forceinline__ Bud(R* lds, uint32_t stride) : s(lds), n(stride) {
 #pragma unroll
-    for (int i = 0; i < NBUD; ++i) v[i] = R(0);
+    for (int i = 0; i < NBUD; ++i) s[i * n] = R(0);
   }
-  __device__ __forceinline__ void set(int i, R x) { v[i] = x; }
-  __device__ __forceinline__ R get(int i) const { return v[i]; }
+  __device__ __forceinline__ void set(int i, R x) { s[i * n] = x; }
+  __device__ __forceinline__ R get(int i) const { return s[i * n]; }
 };
 
 // Device pointers of the budget inputs and PEXTRA, blocked layout; PEXTRA is (NPROMA, KLEV, KFLDX, NGPBLKS).
@@ -56,148 +60,222 @@ struct KBudPtrs {
   R reisthsc, ralfdcp;
 };
 
-// Section 7 of CLOUDSC for one column and level (:3831-4101).  ext(f, k) is the byte offset of PEXTRA(JL,k,f)
-// (1-based f and k); lev = JK (1-based); zdz = layer depth, as ZDP/(ZRHO*RG).
+// The vertical-integral field (IS+1) adds into slots 4-12, 19-39, 44-58 (process terms) and 70-82 (counters and
+// PEIS) at every level of the column and assigns slots 60 and 83.  The 42 process sums live in LDS for the whole
+// column: loaded from PEXTRA before the level loop, added to in the reference's order, stored after it, so each
+// level adds old + x exactly as PEXTRA(JL,IK,IS+1) = PEXTRA(JL,IK,IS+1) + x does.  The counter slots add a column
+// constant (1 or PEIS) at every level: vertint_end adds it once per level, in sequence, to the loaded value.
+constexpr int NVI = 42;
+__host__ __device__ constexpr int vi_slot(int j) {  // slot of accumulator j
+  return j < 8 ? (j == 0 ? 4 : j + 5) : j < 28 ? (j == 8 ? 19 : j + 12) : (j == 28 ? 44 : j + 17);
+}
+__host__ __device__ constexpr int vi_index(int slot) {  // accumulator of a slot
+  return slot == 4 ? 0 : slot <= 12 ? slot - 5 : slot == 19 ? 8 : slot <= 39 ? slot - 12 : slot == 44 ? 28
+       : slot - 17;
+}
+static_assert(vi_slot(vi_index(4)) == 4 && vi_slot(vi_index(39)) == 39 && vi_slot(vi_index(46)) == 46 &&
+              vi_slot(vi_index(58)) == 58 && vi_index(58) == NVI - 1, "vertical-integral slot map");
+
+template <typename R>
+struct ViAcc {
+  R* s;  // this lane's NVI sums at stride n
+  uint32_t n;
+  __device__ __forceinline__ void add(int slot, R x) const {
+    R& a = s[vi_index(slot) * n];
+    a = a + x;
+  }
+};
+
+// Load (before the level loop) and store (after it) the accumulating slots; ext(f, k) as for budget_write.
 template <typename R, typename EXT>
-__device__ __forceinline__ void budget_write(const Bud<R, true>& b, const KBudPtrs<R>& p, EXT ext, uint32_t lev,
-                                             uint32_t o2, uint32_t o1, R zqtmst, R ptsphy, R zdz, R qxn_l,
-                                             R qxn_i, R prainfrac, int32_t ktype, R peis, R ralvdcp, R ralsdcp,
-                                             R ralfdcp) {
+__device__ __forceinline__ void vertint_begin(const ViAcc<R>& acc, const KBudPtrs<R>& p, EXT ext) {
+  const int F = ((p.flags & BUD_LBUD23) ? 26 : 0) + 1;
+  const char* base = reinterpret_cast<const char*>(p.pextra);
+#pragma unroll
+  for (int j = 0; j < NVI; ++j) acc.s[j * acc.n] = *reinterpret_cast<const R*>(base + ext(F, vi_slot(j)));
+}
+// nlev: the levels the scheme ran (NCLDTOP..KLEV), each of which added the counters once (:3924-3939).
+template <typename R, typename EXT>
+__device__ __forceinline__ void vertint_end(const ViAcc<R>& acc, const KBudPtrs<R>& p, EXT ext, R prainfrac,
+                                            int32_t ktype, R peis, uint32_t o1, int nlev) {
+  const int is = (p.flags & BUD_LBUD23) ? 26 : 0, F = is + 1;
+  char* base = reinterpret_cast<char*>(p.pextra);
+#pragma unroll
+  for (int j = 0; j < NVI; ++j) *reinterpret_cast<R*>(base + ext(F, vi_slot(j))) = acc.s[j * acc.n];
+  // slot 60 holds PRAINFRAC_TOPRFZ as of the last level, slot 83 KCTOP where KTYPE > 0 (:3922, :3939)
+  *reinterpret_cast<R*>(base + ext(F, 60)) = prainfrac;
+  if (ktype > 0) *reinterpret_cast<R*>(base + ext(F, 83)) = R(LD(p.kctop, o1, 0u));
+  const int32_t kpbl = LD(p.kpbltype, o1, 0u);
+  const bool on[13] = {ktype == 0, ktype == is + 1,  // the reference compares KTYPE and KPBLTYPE with IS+1 (:3926)
+                       ktype == 2, ktype == 3, kpbl == 0, kpbl == is + 1, kpbl == 2, kpbl == 3, true,
+                       peis > R(6.0), peis > R(8.0), peis > R(10.0), ktype >= 2 && peis < p.reisthsc};
+  R old[13];
+#pragma unroll
+  for (int j = 0; j < 13; ++j) old[j] = *reinterpret_cast<const R*>(base + ext(F, 70 + j));
+#pragma unroll
+  for (int j = 0; j < 13; ++j) {
+    if (!on[j]) continue;
+    const R x = j == 8 ? peis : R(1);
+    R a = old[j];
+    for (int l = 0; l < nlev; ++l) a = a + x;
+    *reinterpret_cast<R*>(base + ext(F, 70 + j)) = a;
+  }
+}
+
+// The 3D budgets: IS of each block for the switches in force, and the fields each block updates, in the order
+// budget_write produces their values.
+struct BudBlocks {
+  int c, l, i, t;
+};
+__device__ __forceinline__ BudBlocks bud_blocks(int32_t f) {
+  BudBlocks o;
+  int is = ((f & BUD_LBUD23) ? 26 : 0) + ((f & BUD_VERTINT) ? 1 : 0);
+  o.c = is;
+  is += (f & BUD_C) ? 12 : 0;
+  o.l = is;
+  is += (f & BUD_L) ? 22 : 0;
+  o.i = is;
+  is += (f & BUD_I) ? 20 : 0;
+  o.t = is;
+  return o;
+}
+constexpr int NC3 = 7, NL3 = 17, NI3 = 15, NT3 = 13;
+__host__ __device__ constexpr int c_field(int j) { return j == 0 ? 4 : j < 5 ? j + 5 : j + 6; }  // 4, 6-9, 11, 12
+__host__ __device__ constexpr int l_field(int j) { return j == 0 ? 5 : j + 6; }                 // 5, 7-22
+__host__ __device__ constexpr int i_field(int j) { return j == 0 ? 5 : j + 6; }                 // 5, 7-20
+__host__ __device__ constexpr int t_field(int j) { return j + 1; }                              // 1-13
+static_assert(c_field(NC3 - 1) == 12 && l_field(NL3 - 1) == 22 && i_field(NI3 - 1) == 20, "3D budget fields");
+
+// The PEXTRA words of this level that the 3D budgets update, loaded at the start of the level so that the loads
+// overlap the level's computation; budget_write adds to these values and stores.  No other write touches these
+// words in between (the vertical integrals are in field IS+1, the 3D fields are distinct per level).
+template <typename R>
+struct Pre3D {
+  R c[NC3], l[NL3], i[NI3], t[NT3];
+};
+template <typename R, typename EXT>
+__device__ __forceinline__ void budget_prefetch(Pre3D<R>& pre, const KBudPtrs<R>& p, EXT ext, uint32_t lev) {
   const int32_t f = p.flags;
-  int is = (f & BUD_LBUD23) ? 26 : 0;
+  const BudBlocks o = bud_blocks(f);
+  const char* base = reinterpret_cast<const char*>(p.pextra);
+  auto ld = [&](int fld) { return *reinterpret_cast<const R*>(base + ext(fld, lev)); };
+  if (f & BUD_C) {
+#pragma unroll
+    for (int j = 0; j < NC3; ++j) pre.c[j] = ld(o.c + c_field(j));
+  }
+  if (f & BUD_L) {
+#pragma unroll
+    for (int j = 0; j < NL3; ++j) pre.l[j] = ld(o.l + l_field(j));
+  }
+  if (f & BUD_I) {
+#pragma unroll
+    for (int j = 0; j < NI3; ++j) pre.i[j] = ld(o.i + i_field(j));
+  }
+  if (f & BUD_T) {
+#pragma unroll
+    for (int j = 0; j < NT3; ++j) pre.t[j] = ld(o.t + t_field(j));
+  }
+}
+
+// Section 7 of CLOUDSC for one column and level (:3831-4101).  ext(f, k) is the byte offset of PEXTRA(JL,k,f)
+// (1-based f and k); lev = JK (1-based); zdz = layer depth, as ZDP/(ZRHO*RG); acc: the vertical-integral sums;
+// pre: this level's 3D PEXTRA words as loaded by budget_prefetch.  Rates the default path never sets (ZBUDCC 1,
+// 10-12; ZBUDL 1, 8-10, 12-13, 16, 22; ZBUDI 1, 8-10, 18; ZPRACS, ZPRACI, ZPIEVAP) are zero and enter as such, so
+// every word the reference touches is touched here with the same value.
+template <typename R, typename EXT>
+__device__ __forceinline__ void budget_write(const Bud<R, true>& b, const ViAcc<R>& acc, const Pre3D<R>& pre,
+                                             const KBudPtrs<R>& p, EXT ext, uint32_t lev, uint32_t o2, R zqtmst,
+                                             R ptsphy, R zdz, R qxn_l, R qxn_i, R ralvdcp, R ralsdcp, R ralfdcp) {
+  const int32_t f = p.flags;
   const R zdt = (f & BUD_TIMEINT) ? ptsphy : R(1);
-  auto add = [&](int fld, uint32_t k, R x) {
-    R* a = reinterpret_cast<R*>(reinterpret_cast<char*>(p.pextra) + ext(fld, k));
-    *a = *a + x;
-  };
-  auto put = [&](int fld, uint32_t k, R x) { *reinterpret_cast<R*>(reinterpret_cast<char*>(p.pextra) + ext(fld, k)) = x; };
   const R pvfa = LD(p.pvfa, o2, 0u), pdyna = LD(p.pdyna, o2, 0u), pvfl = LD(p.pvfl, o2, 0u);
   const R pdynl = LD(p.pdynl, o2, 0u), pvfi = LD(p.pvfi, o2, 0u), pdyni = LD(p.pdyni, o2, 0u);
   if (f & BUD_VERTINT) {  // vertical integrals into field IS+1, "levels" used as slots (:3849-3943)
-    const int F = is + 1;
     auto vi = [&](R x) { return x * zdz * zdt; };  // X*ZDZ*ZDT, in the reference's order
-    add(F, 4, vi(b.get(CC2)));
-    add(F, 6, vi(b.get(CC3)));
-    add(F, 7, vi((b.get(CC4) + b.get(CC6))));
-    add(F, 8, vi(b.get(CC5)));
-    add(F, 9, vi(b.get(CC7)));
-    add(F, 10, vi(R(0)));  // ZBUDCC(JL,12), the tidy-up, is never set
-    add(F, 11, vi(pvfa));
-    add(F, 12, vi(pdyna));
-    add(F, 19, vi(b.get(L2)));
-    add(F, 21, vi(b.get(L3)));
-    add(F, 22, vi((b.get(L4) + b.get(L6))));
-    add(F, 23, vi(b.get(L5)));
-    add(F, 24, vi(b.get(L7)));
-    add(F, 25, vi(b.get(L11)));
-    add(F, 26, vi(b.get(L14)));
-    add(F, 27, vi(b.get(L15)));
-    add(F, 28, vi(R(0)));
-    add(F, 29, vi(R(0)));
-    add(F, 30, vi(b.get(L17)));
-    add(F, 31, vi((b.get(L18) + b.get(L19))));
-    add(F, 32, vi(b.get(L20)));
-    add(F, 33, vi(b.get(L21)));
-    add(F, 34, vi(R(0)) * zqtmst);  // ZPRACS: rain accreting snow is off on the default path
-    add(F, 35, vi(b.get(PSACR)) * zqtmst);
-    add(F, 36, vi(R(0)) * zqtmst);  // ZPRACI: off on the default path
-    add(F, 37, vi(b.get(PIACR)) * zqtmst);
-    add(F, 38, vi(pvfl));
-    add(F, 39, vi(pdynl));
-    add(F, 44, vi(b.get(I2)));
-    add(F, 46, vi(b.get(I3)));
-    add(F, 47, vi((b.get(I4) + b.get(I6))));
-    add(F, 48, vi(b.get(I5)));
-    add(F, 49, vi(b.get(I7)));
-    add(F, 50, vi(b.get(I11)));
-    add(F, 51, vi((b.get(I12) + b.get(I13))));
-    add(F, 52, vi(b.get(I14)));
-    add(F, 53, vi(b.get(I16)));
-    add(F, 54, vi(b.get(I17)));
-    add(F, 55, vi(b.get(PSDEP) * zqtmst));
-    add(F, 56, vi(pvfi));
-    add(F, 57, vi(pdyni));
-    add(F, 58, vi(LD(p.pdyns, o2, 0u)));
-    put(F, 60, prainfrac);
-    const int32_t kpbl = LD(p.kpbltype, o1, 0u);
-    if (ktype == 0) add(F, 70, R(1));
-    if (ktype == is + 1) add(F, 71, R(1));  // the reference compares KTYPE with IS+1 here (:3926)
-    if (ktype == 2) add(F, 72, R(1));
-    if (ktype == 3) add(F, 73, R(1));
-    if (kpbl == 0) add(F, 74, R(1));
-    if (kpbl == is + 1) add(F, 75, R(1));  // and KPBLTYPE with IS+1 (:3930)
-    if (kpbl == 2) add(F, 76, R(1));
-    if (kpbl == 3) add(F, 77, R(1));
-    add(F, 78, peis);
-    if (peis > R(6.0)) add(F, 79, R(1));
-    if (peis > R(8.0)) add(F, 80, R(1));
-    if (peis > R(10.0)) add(F, 81, R(1));
-    if (ktype >= 2 && peis < p.reisthsc) add(F, 82, R(1));
-    if (ktype > 0) put(F, 83, R(LD(p.kctop, o1, 0u)));
-    is = is + 1;
+    acc.add(4, vi(b.get(CC2)));
+    acc.add(6, vi(b.get(CC3)));
+    acc.add(7, vi((b.get(CC4) + b.get(CC6))));
+    acc.add(8, vi(b.get(CC5)));
+    acc.add(9, vi(b.get(CC7)));
+    acc.add(10, vi(R(0)));  // ZBUDCC(JL,12), the tidy-up, is never set
+    acc.add(11, vi(pvfa));
+    acc.add(12, vi(pdyna));
+    acc.add(19, vi(b.get(L2)));
+    acc.add(21, vi(b.get(L3)));
+    acc.add(22, vi((b.get(L4) + b.get(L6))));
+    acc.add(23, vi(b.get(L5)));
+    acc.add(24, vi(b.get(L7)));
+    acc.add(25, vi(b.get(L11)));
+    acc.add(26, vi(b.get(L14)));
+    acc.add(27, vi(b.get(L15)));
+    acc.add(28, vi(R(0)));
+    acc.add(29, vi(R(0)));
+    acc.add(30, vi(b.get(L17)));
+    acc.add(31, vi((b.get(L18) + b.get(L19))));
+    acc.add(32, vi(b.get(L20)));
+    acc.add(33, vi(b.get(L21)));
+    acc.add(34, vi(R(0)) * zqtmst);  // ZPRACS: rain accreting snow is off on the default path
+    acc.add(35, vi(b.get(PSACR)) * zqtmst);
+    acc.add(36, vi(R(0)) * zqtmst);  // ZPRACI: off on the default path
+    acc.add(37, vi(b.get(PIACR)) * zqtmst);
+    acc.add(38, vi(pvfl));
+    acc.add(39, vi(pdynl));
+    acc.add(44, vi(b.get(I2)));
+    acc.add(46, vi(b.get(I3)));
+    acc.add(47, vi((b.get(I4) + b.get(I6))));
+    acc.add(48, vi(b.get(I5)));
+    acc.add(49, vi(b.get(I7)));
+    acc.add(50, vi(b.get(I11)));
+    acc.add(51, vi((b.get(I12) + b.get(I13))));
+    acc.add(52, vi(b.get(I14)));
+    acc.add(53, vi(b.get(I16)));
+    acc.add(54, vi(b.get(I17)));
+    acc.add(55, vi(b.get(PSDEP) * zqtmst));
+    acc.add(56, vi(pvfi));
+    acc.add(57, vi(pdyni));
+    acc.add(58, vi(LD(p.pdyns, o2, 0u)));
   }
+  const BudBlocks o = bud_blocks(f);
+  char* base = reinterpret_cast<char*>(p.pextra);
+  auto st = [&](int fld, R x) { *reinterpret_cast<R*>(base + ext(fld, lev)) = x; };
   if (f & BUD_C) {  // cloud fraction (:3948-3966)
-    add(is + 4, lev, b.get(CC2) * zdt);
-    add(is + 6, lev, b.get(CC3) * zdt);
-    add(is + 7, lev, (b.get(CC4) + b.get(CC6)) * zdt);
-    add(is + 8, lev, b.get(CC5) * zdt);
-    add(is + 9, lev, b.get(CC7) * zdt);
-    add(is + 11, lev, pvfa * zdt);
-    add(is + 12, lev, pdyna * zdt);
-    is = is + 12;
+    const R v[NC3] = {b.get(CC2) * zdt, b.get(CC3) * zdt, (b.get(CC4) + b.get(CC6)) * zdt, b.get(CC5) * zdt,
+                      b.get(CC7) * zdt, pvfa * zdt, pdyna * zdt};
+#pragma unroll
+    for (int j = 0; j < NC3; ++j) st(o.c + c_field(j), pre.c[j] + v[j]);
   }
   if (f & BUD_L) {  // cloud liquid (:3971-4003)
-    add(is + 5, lev, b.get(L2) * zdt);
-    add(is + 7, lev, b.get(L3) * zdt);
-    add(is + 8, lev, (b.get(L4) + b.get(L6)) * zdt);
-    add(is + 9, lev, b.get(L5) * zdt);
-    add(is + 10, lev, b.get(L7) * zdt);
-    add(is + 11, lev, b.get(L11) * zdt);
-    add(is + 12, lev, R(0) * zdt);
-    add(is + 13, lev, R(0) * zdt);
-    add(is + 14, lev, b.get(L14) * zdt);
-    add(is + 15, lev, b.get(L15) * zdt);
-    add(is + 16, lev, b.get(L17) * zdt);
-    add(is + 17, lev, (b.get(L18) + b.get(L19)) * zdt);
-    add(is + 18, lev, b.get(L20) * zdt);
-    add(is + 19, lev, b.get(L21) * zdt);
-    add(is + 20, lev, pvfl * zdt);
-    add(is + 21, lev, pdynl * zdt);
-    add(is + 22, lev, qxn_l);
-    is = is + 22;
+    const R v[NL3] = {b.get(L2) * zdt,  b.get(L3) * zdt,  (b.get(L4) + b.get(L6)) * zdt, b.get(L5) * zdt,
+                      b.get(L7) * zdt,  b.get(L11) * zdt, R(0) * zdt,  R(0) * zdt,  b.get(L14) * zdt,
+                      b.get(L15) * zdt, b.get(L17) * zdt, (b.get(L18) + b.get(L19)) * zdt, b.get(L20) * zdt,
+                      b.get(L21) * zdt, pvfl * zdt,       pdynl * zdt, qxn_l};
+#pragma unroll
+    for (int j = 0; j < NL3; ++j) st(o.l + l_field(j), pre.l[j] + v[j]);
   }
-  if (f & BUD_I) {  // cloud ice (:4008-4041)
-    add(is + 5, lev, b.get(I2) * zdt);
-    add(is + 7, lev, b.get(I3) * zdt);
-    add(is + 8, lev, (b.get(I4) + b.get(I6)) * zdt);
-    add(is + 9, lev, b.get(I5) * zdt);
-    add(is + 10, lev, b.get(I7) * zdt);
-    add(is + 11, lev, b.get(I11) * zdt);
-    add(is + 12, lev, (b.get(I12) + b.get(I13)) * zdt);
-    add(is + 13, lev, b.get(I14) * zdt);
-    add(is + 14, lev, (b.get(I15) + b.get(I16)) * zdt);
-    add(is + 15, lev, b.get(PSDEP) * zqtmst * zdt);
-    add(is + 16, lev, b.get(I17) * zdt);
-    add(is + 17, lev, pvfi * zdt);
-    add(is + 18, lev, R(0) * zqtmst * zdt);  // ZPIEVAP: ice sublimation (ISUBLICE) is off on the default path
-    add(is + 19, lev, pdyni * zdt);
-    add(is + 20, lev, qxn_i);
-    is = is + 20;
+  if (f & BUD_I) {  // cloud ice (:4008-4041); field 18 is ZPIEVAP: ice sublimation (ISUBLICE) is off by default
+    const R v[NI3] = {b.get(I2) * zdt,  b.get(I3) * zdt,  (b.get(I4) + b.get(I6)) * zdt, b.get(I5) * zdt,
+                      b.get(I7) * zdt,  b.get(I11) * zdt, (b.get(I12) + b.get(I13)) * zdt, b.get(I14) * zdt,
+                      (b.get(I15) + b.get(I16)) * zdt, b.get(PSDEP) * zqtmst * zdt, b.get(I17) * zdt,
+                      pvfi * zdt, R(0) * zqtmst * zdt, pdyni * zdt, qxn_i};
+#pragma unroll
+    for (int j = 0; j < NI3; ++j) st(o.i + i_field(j), pre.i[j] + v[j]);
   }
   if (f & BUD_T) {  // temperature (:4052-4101)
-    add(is + 1, lev, LD(p.phrsw, o2, 0u));
-    add(is + 2, lev, LD(p.phrlw, o2, 0u));
-    add(is + 3, lev, (b.get(L2) * ralvdcp - b.get(I11) * ralvdcp) * zdt);
-    add(is + 4, lev, (b.get(I2) + b.get(I11)) * ralsdcp * zdt);
-    add(is + 5, lev, b.get(I11) * ralfdcp * zdt);
-    add(is + 6, lev, (b.get(L7) + b.get(L5)) * ralvdcp * zdt);
-    add(is + 7, lev, (b.get(I7) + b.get(I5)) * ralsdcp * zdt);
-    add(is + 8, lev, b.get(L20) * ralvdcp * zdt);
-    add(is + 9, lev, b.get(I17) * ralsdcp * zdt);
-    add(is + 10, lev, b.get(I15) * ralfdcp * zdt);
-    add(is + 11, lev, b.get(I16) * ralfdcp * zdt);
-    add(is + 12, lev, R(0) - (b.get(L18) + b.get(L19)) * ralfdcp * zdt);
-    add(is + 13, lev, b.get(L21) * ralfdcp * zdt);
+    const R v[NT3] = {LD(p.phrsw, o2, 0u),
+                      LD(p.phrlw, o2, 0u),
+                      (b.get(L2) * ralvdcp - b.get(I11) * ralvdcp) * zdt,
+                      (b.get(I2) + b.get(I11)) * ralsdcp * zdt,
+                      b.get(I11) * ralfdcp * zdt,
+                      (b.get(L7) + b.get(L5)) * ralvdcp * zdt,
+                      (b.get(I7) + b.get(I5)) * ralsdcp * zdt,
+                      b.get(L20) * ralvdcp * zdt,
+                      b.get(I17) * ralsdcp * zdt,
+                      b.get(I15) * ralfdcp * zdt,
+                      b.get(I16) * ralfdcp * zdt,
+                      R(0) - (b.get(L18) + b.get(L19)) * ralfdcp * zdt,
+                      b.get(L21) * ralfdcp * zdt};
+#pragma unroll
+    for (int j = 0; j < NT3; ++j) st(o.t + t_field(j), pre.t[j] + v[j]);
   }
 }
 

@@ -58,6 +58,18 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
 
   const R ptsphy = c.ptsphy, zqtmst = c.zqtmst, zepsec = R(1.e-14);
   const R rlmin = c.rlmin;
+  R* bud_lds = nullptr;  // this lane's budget entries and vertical-integral sums (BUD only)
+  ViAcc<R> vacc{nullptr, BLOCK};
+  const uint32_t kfx = (uint32_t)bp.kfldx;
+  auto ext = [&](int fld, uint32_t kk) {  // byte offset of PEXTRA(JL,kk,fld) of this block
+    return (((b * kfx + (uint32_t)(fld - 1)) * klev + (kk - 1)) * nproma) * E + jo;
+  };
+  if constexpr (BUD) {
+    __shared__ R s_bud[(NBUD + NVI) * BLOCK];
+    bud_lds = s_bud + threadIdx.x;
+    vacc.s = s_bud + NBUD * BLOCK + threadIdx.x;
+    if (bp.flags & BUD_VERTINT) vertint_begin(vacc, bp, ext);
+  }
 
   // Per-column inputs
   const R plsm = LD(p.plsm, u1, jo);
@@ -227,7 +239,11 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
 
     if ((int)k >= ncldtop0) {
       const bool above_top = (int)k > ncldtop0;  // JK > NCLDTOP
-      Bud<R, BUD> bud;  // ZBUDCC/ZBUDL/ZBUDI of this level (:1271-1273)
+      Bud<R, BUD> bud(bud_lds, BLOCK);  // ZBUDCC/ZBUDL/ZBUDI of this level, zeroed (:1271-1273)
+      // FP32: the 3D budget words are loaded here and arrive during the level; FP64 loads them at section 7
+      // (holding them across the level would spill)
+      Pre3D<R> bud_pre;
+      if constexpr (BUD && sizeof(R) == 4) budget_prefetch(bud_pre, bp, ext, k + 1);
       R zrainaut = R(0), zrainacc = R(0), zsnowaut = R(0), zsnowrime = R(0);  // (:1239-1242)
       const R plude_in = LD(p.plude, i2, jo);
       // derived variables (:1281-1324)
@@ -819,13 +835,10 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       }
       ST(p.pcovptot, i2, jo, zcovptot);  // (:3814-3816)
       if constexpr (BUD) {  // section 7 (:3826-4101)
-        const uint32_t kfx = (uint32_t)bp.kfldx;
-        auto ext = [&](int fld, uint32_t kk) {
-          return (((b * kfx + (uint32_t)(fld - 1)) * klev + (kk - 1)) * nproma) * E + jo;
-        };
         const R zdz = m_div(zdp, zrho * c.rg);
-        budget_write<R>(bud, bp, ext, k + 1, i2 + jo, u1i + joi, zqtmst, ptsphy, zdz, qxn_l, qxn_i, prainfrac, ktype,
-                        peis, c.ralvdcp, c.ralsdcp, bp.ralfdcp);
+        if constexpr (sizeof(R) == 8) budget_prefetch(bud_pre, bp, ext, k + 1);
+        budget_write<R>(bud, vacc, bud_pre, bp, ext, k + 1, i2 + jo, zqtmst, ptsphy, zdz, qxn_l, qxn_i, c.ralvdcp,
+                        c.ralsdcp, bp.ralfdcp);
       }
     }
 
@@ -872,6 +885,9 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
     paph_k = paph_k1;
     pmfu_k = pmfu_n;
     pmfd_k = pmfd_n;
+  }
+  if constexpr (BUD) {
+    if (bp.flags & BUD_VERTINT) vertint_end(vacc, bp, ext, prainfrac, ktype, peis, u1i + joi, (int)klev - ncldtop0);
   }
   // precipitation fluxes at the surface half level
   {
@@ -1054,9 +1070,9 @@ template <typename R, typename RD, typename KC, typename KCD>
 hipError_t launch_either(bool sat, int block, bool uni, unsigned ncols, const KC& c, const KCD& cd,
                          const KPtrs<R>& p, const KDims& d, int ssc_off, const KSatPtrs<R>& s,
                          const KBudPtrs<R>* bud, hipStream_t stream) {
-  if (bud) {  // budget-instrumented kernel: one instantiation per variant (diagnostic runs)
-    hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, false, false, true>), dim3((ncols + 255) / 256),
-                       dim3(256), 0, stream, c, cd, p, d, ssc_off, s, *bud);
+  if (bud) {  // budget-instrumented kernel: one instantiation per variant (diagnostic runs); one wave per group
+    hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 64, false, false, true>), dim3((ncols + 63) / 64),
+                       dim3(64), 0, stream, c, cd, p, d, ssc_off, s, *bud);
     return hipGetLastError();
   }
   return sat ? launch_variant<true, R, RD>(block, uni, ncols, c, cd, p, d, ssc_off, s, stream)
