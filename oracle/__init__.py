@@ -19,6 +19,10 @@ Parity status of each oracle function (what pins it; see tests/test_oracle_*.py)
                                  parity unpinned: rain evaporation, ice and snow deposition, ice->snow
                                  autoconversion, melting, rain freezing, riming, PSACR/PIACR, snow
                                  sublimation, erosion, accretion (no reference run or golden vector exists)
+  budget diagnostics (LCLDBUD*)  pinned (tests/test_budget.py): liquid, ice and cloud-cover closure, the
+                                 temperature budget against the liquid/ice fields, vertical integrals against the
+                                 integrated 3D fields, counter closed forms, process regimes, TIMEINT/LBUD23/
+                                 accumulation; the rain-only rates are pinned only through those identities
 """
 import ctypes
 import os
