@@ -353,7 +353,7 @@ def budget_bench(params, nproma, steps):
     stream = torch.cuda.current_stream()
     out = {}
     for name, run in (("plain_ms", lambda: C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)),
-                      ("budget_ms", lambda: C.cloudsc_blocked_budget(f, bud, q, 900.0, stream=stream.cuda_stream))):
+                      ("budget_ms", lambda: C.cloudsc_blocked_diag(f, bud, q, 900.0, stream=stream.cuda_stream))):
         ts = []
         for i in range(steps + 2):
             f["plude"].copy_(plude0, non_blocking=True)

@@ -20,8 +20,9 @@
  *     aerosol-only, SPP-only, inhomogeneity-only: PVFA, PDYN*, PHRSW, PHRLW,
  *     PVERVEL, PGEOM, PGAW, KCTOP, KPBLTYPE, PGP2DSPP, PLCRIT_AER, PICRIT_AER,
  *     PRE_ICE, PCCN, PNICE, PEXTRA) are kept in the list so the call is a
- *     positional drop-in; they may be NULL.  PFSD is written only when
- *     LRAD_CLOUD_INHOMOG is set, which this build rejects, so it may be NULL.
+ *     positional drop-in; they may be NULL unless the budget switches
+ *     (PVFA..PHRLW, KPBLTYPE, KCTOP, PEXTRA) or LRAD_CLOUD_INHOMOG (PGAW) read
+ *     them.  PFSD is written only when LRAD_CLOUD_INHOMOG is set.
  *   - Write set is the reference's: PTENDENCY_LOC_CLD(:,:,5) is not written;
  *     PCOVPTOT is written only for JK >= NCLDTOP; PRAINFRAC_TOPRFZ is zeroed
  *     for all KLON columns; PLUDE is rescaled in place for NCLDTOP<=JK<KLEV
@@ -102,7 +103,7 @@ typedef struct cloudsc_params_t {
   int32_t ncldtop;          /* first level the cloud scheme treats (sucldp.F90:631-634) */
   int32_t lmfdsnow;         /* TECUMF: convective snow/rain detrainment */
   int32_t lcloud_inhomog;   /* TECLDP: must be 0 (regime-dependent FSD not built) */
-  int32_t lrad_cloud_inhomog; /* TEPHY: must be 0 (ice FSD / PFSD not built) */
+  int32_t lrad_cloud_inhomog; /* TEPHY: condensate FSD into PFSD (per-block and cloudsc_blocked_diag_*_c only) */
   int32_t laeraware;        /* any of LAERLIQAUTOLSP/LAERLIQCOLL/LAERICESED/LAERICEAUTO: must be 0 */
   int32_t lspp;             /* TSPP_CONFIG%LSPP: must be 0 */
   int32_t lcldbud;          /* budget diagnostics, CLOUDSC_BUD_* bits below (0 = all off, the default) */
@@ -159,7 +160,7 @@ int cloudsc_mixed_c(CLOUDSC_ARGS(float));
 /* Bits of cloudsc_params_t::lcldbud: the LCLDBUD* switches of YDECLDP (cloudsc.F90:706-713), LBUD23 of YDEPHY
  * (PEXTRA fields start after 26, :3832-3836) and LCLDBUDGET (its check, :1032-1082 and :4146-4212, has no output,
  * so the bit is accepted and changes nothing).  The C, L, I, T and VERTINT budgets write PEXTRA: they are honoured
- * by the per-block entries (PEXTRA and KFLDX are in the argument list) and by cloudsc_blocked_budget_*_c, and
+ * by the per-block entries (PEXTRA and KFLDX are in the argument list) and by cloudsc_blocked_diag_*_c, and
  * rejected with CLOUDSC_EUNSUPPORTED by the other entries.  VERTINT needs KLEV >= 83 (the slots the reference
  * writes) and all of them need KFLDX >= the number of fields they fill. */
 #define CLOUDSC_BUD_C 1         /* LCLDBUDC: cloud fraction budget, 12 fields */
@@ -186,11 +187,14 @@ int cloudsc_blocked_dp_c(CLOUDSC_BLOCKED_ARGS(double));
 int cloudsc_blocked_sp_c(CLOUDSC_BLOCKED_ARGS(float));
 int cloudsc_blocked_mixed_c(CLOUDSC_BLOCKED_ARGS(float));
 
-/* Blocked entry with the budget diagnostics (params->lcldbud): the arguments of cloudsc_blocked_*_c, then the
- * inputs only the budgets read (PVFA, PDYNA, PDYNL, PDYNI, PDYNS, PHRSW, PHRLW as (NPROMA,KLEV,NGPBLKS);
- * KPBLTYPE, KCTOP as (NPROMA,NGPBLKS)) and PEXTRA (NPROMA,KLEV,KFLDX,NGPBLKS), INOUT: the budgets accumulate into
- * it as the reference does.  Replaces the CLOUDSC call of CLOUD_LAYER in diagnostic runs. */
-#define CLOUDSC_BLOCKED_BUDGET_ARGS(R)                                                                  \
+/* Blocked entry with the diagnostics the plain entries refuse: the arguments of cloudsc_blocked_*_c, then the
+ * inputs only the budgets read (PVFA, PDYNA, PDYNL, PDYNI, PDYNS, PHRSW, PHRLW as (NPROMA,KLEV,NGPBLKS); KPBLTYPE,
+ * KCTOP as (NPROMA,NGPBLKS)), PEXTRA (NPROMA,KLEV,KFLDX,NGPBLKS) and KFLDX, then PGAW (NPROMA,NGPBLKS) and PFSD
+ * (NPROMA,KLEV,NGPBLKS).  params->lcldbud selects the budgets, which accumulate into PEXTRA as the reference does
+ * (cloudsc.F90:3826-4101); params->lrad_cloud_inhomog writes the condensate FSD for the radiation into PFSD on
+ * every level (:1937-1971, :3818-3822, :4112-4126, :4307-4359).  Arrays of a diagnostic that is off may be NULL.
+ * Replaces the CLOUDSC call of CLOUD_LAYER in runs with those switches. */
+#define CLOUDSC_BLOCKED_DIAG_ARGS(R)                                                                    \
   const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,            \
       const R *pt, const R *pq, const R *ptendency_cml_t, const R *ptendency_cml_q,                     \
       const R *ptendency_cml_a, const R *ptendency_cml_cld, R *ptendency_loc_t, R *ptendency_loc_q,     \
@@ -201,11 +205,11 @@ int cloudsc_blocked_mixed_c(CLOUDSC_BLOCKED_ARGS(float));
       R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur,   \
       R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, const R *pvfa, const R *pdyna, const R *pdynl,        \
       const R *pdyni, const R *pdyns, const R *phrsw, const R *phrlw, const int32_t *kpbltype,          \
-      const int32_t *kctop, R *pextra, int kfldx, void *stream
+      const int32_t *kctop, R *pextra, int kfldx, const R *pgaw, R *pfsd, void *stream
 
-int cloudsc_blocked_budget_dp_c(CLOUDSC_BLOCKED_BUDGET_ARGS(double));
-int cloudsc_blocked_budget_sp_c(CLOUDSC_BLOCKED_BUDGET_ARGS(float));
-int cloudsc_blocked_budget_mixed_c(CLOUDSC_BLOCKED_BUDGET_ARGS(float));
+int cloudsc_blocked_diag_dp_c(CLOUDSC_BLOCKED_DIAG_ARGS(double));
+int cloudsc_blocked_diag_sp_c(CLOUDSC_BLOCKED_DIAG_ARGS(float));
+int cloudsc_blocked_diag_mixed_c(CLOUDSC_BLOCKED_DIAG_ARGS(float));
 
 /* Last HIP error text of the calling thread (empty string if none). */
 const char *cloudsc_last_error(void);

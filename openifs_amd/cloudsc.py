@@ -378,7 +378,8 @@ def cloudsc_blocked(fields, params, ptsphy, stream=None):
 # ---------------------------------------------------------------------------
 # Bits of cloudsc_params_t.lcldbud: the LCLDBUD* switches (cloudsc.F90:706-713), LBUD23 and LCLDBUDGET.
 BUD_C, BUD_L, BUD_I, BUD_T, BUD_VERTINT, BUD_TIMEINT, BUD_LBUD23, BUD_CHECK = 1, 2, 4, 8, 16, 32, 64, 128
-# Arrays only the budgets read, in the argument order of cloudsc_blocked_budget_*_c (PEXTRA and KFLDX follow).
+# Arrays only the budgets read, in the argument order of cloudsc_blocked_diag_*_c (PEXTRA and KFLDX follow, then
+# PGAW and PFSD).
 BUDGET_ARRAYS = [("pvfa", "2d"), ("pdyna", "2d"), ("pdynl", "2d"), ("pdyni", "2d"), ("pdyns", "2d"),
                  ("phrsw", "2d"), ("phrlw", "2d"), ("kpbltype", "1di"), ("kctop", "1di")]
 
@@ -397,20 +398,26 @@ def extra_shape(nproma, klev, kfldx, ngpblks):
     return (ngpblks, kfldx, klev, nproma)
 
 
-def cloudsc_blocked_budget(fields, budget, params, ptsphy, stream=None):
-    """One launch of cloudsc_blocked_budget_{dp,sp,mixed}_c: ``fields`` a BlockedFields, ``budget`` a dict of
-    device tensors with the BUDGET_ARRAYS and ``pextra`` (extra_shape), which the budgets accumulate into."""
+def cloudsc_blocked_diag(fields, diag, params, ptsphy, stream=None):
+    """One launch of cloudsc_blocked_diag_{dp,sp,mixed}_c: ``fields`` a BlockedFields, ``diag`` a dict of device
+    tensors: the BUDGET_ARRAYS and ``pextra`` (extra_shape) for the budgets, which accumulate into it, and
+    ``pgaw`` (NGPBLKS, NPROMA) and ``pfsd`` (NGPBLKS, KLEV, NPROMA) for LRAD_CLOUD_INHOMOG.  Arrays of a diagnostic
+    that is off may be left out."""
     import torch
 
     if stream is None:
         stream = torch.cuda.current_stream(fields.device).cuda_stream
-    fn = f"cloudsc_blocked_budget_{fields.precision}_c"
-    px = budget["pextra"]
-    if px.dim() != 4 or (px.shape[0], px.shape[2], px.shape[3]) != (fields.ngpblks, fields.klev, fields.nproma):
-        raise ValueError(f"pextra: need shape (NGPBLKS, KFLDX, KLEV, NPROMA), got {tuple(px.shape)}")
+    fn = f"cloudsc_blocked_diag_{fields.precision}_c"
+    px = diag.get("pextra")
+    kfldx = 0
+    if px is not None:
+        if px.dim() != 4 or (px.shape[0], px.shape[2], px.shape[3]) != (fields.ngpblks, fields.klev, fields.nproma):
+            raise ValueError(f"pextra: need shape (NGPBLKS, KFLDX, KLEV, NPROMA), got {tuple(px.shape)}")
+        kfldx = px.shape[1]
+    ptr = lambda n: ctypes.c_void_p(diag[n].data_ptr() if diag.get(n) is not None else 0)
     args = [ctypes.byref(params), ctypes.c_int(fields.ngptot), ctypes.c_int(fields.nproma), ctypes.c_int(fields.klev),
             ctypes.c_int(fields.ngpblks), _ctype_real(fields.precision)(ptsphy)]
     args += blocked_args(fields.t, lambda x: x.data_ptr())
-    args += [ctypes.c_void_p(budget[n].data_ptr()) for n, _ in BUDGET_ARRAYS]
-    args += [ctypes.c_void_p(px.data_ptr()), ctypes.c_int(px.shape[1]), ctypes.c_void_p(stream)]
+    args += [ptr(n) for n, _ in BUDGET_ARRAYS]
+    args += [ptr("pextra"), ctypes.c_int(kfldx), ptr("pgaw"), ptr("pfsd"), ctypes.c_void_p(stream)]
     _check(fn, getattr(lib(), fn)(*args))

@@ -11,7 +11,9 @@ int fail(int code, const char* what, hipError_t e = hipSuccess);
 // CLOUDSC_OK if the current device is a gfx950, CLOUDSC_ENODEVICE otherwise (result cached per device).
 int check_device();
 // The switches of the parameter block that the kernels implement: CLOUDSC_EUNSUPPORTED for LCLOUD_INHOMOG,
-// LRAD_CLOUD_INHOMOG, aerosol, SPP, unknown lcldbud bits and, unless `budget` (the entry takes PEXTRA), the
-// budget switches that write PEXTRA.
-int check_config(const cloudsc_params_t* p, bool budget = false);
+// aerosol, SPP, unknown lcldbud bits and, unless `diag` (the entry takes PEXTRA and PFSD), the budget switches
+// that write PEXTRA and LRAD_CLOUD_INHOMOG, which writes PFSD.
+int check_config(const cloudsc_params_t* p, bool diag = false);
+// CLOUDSC_ENULL if LRAD_CLOUD_INHOMOG is on and PGAW or PFSD is NULL.
+int check_fsd(const cloudsc_params_t* p, const void* pgaw, const void* pfsd);
 }  // namespace cloudsc_capi

@@ -14,7 +14,8 @@ namespace cloudsc_gpu {
 
 // Switch bits of cloudsc_params_t::lcldbud (include/cloudsc_hip.h).
 enum : int32_t {
-  BUD_C = 1, BUD_L = 2, BUD_I = 4, BUD_T = 8, BUD_VERTINT = 16, BUD_TIMEINT = 32, BUD_LBUD23 = 64, BUD_CHECK = 128
+  BUD_C = 1, BUD_L = 2, BUD_I = 4, BUD_T = 8, BUD_VERTINT = 16, BUD_TIMEINT = 32, BUD_LBUD23 = 64, BUD_CHECK = 128,
+  BUD_OUT = BUD_C | BUD_L | BUD_I | BUD_T | BUD_VERTINT  // the switches that write PEXTRA
 };
 
 // Budget entries the default path sets (names: array and reference index).  L12/L13 (rain freezing from
@@ -50,7 +51,9 @@ struct Bud<R, true> {
   __device__ __forceinline__ R get(int i) const { return s[i * n]; }
 };
 
-// Device pointers of the budget inputs and PEXTRA, blocked layout; PEXTRA is (NPROMA, KLEV, KFLDX, NGPBLKS).
+// Device pointers of the diagnostics, blocked layout: the budget inputs and PEXTRA (NPROMA, KLEV, KFLDX, NGPBLKS)
+// when flags has output bits; PGAW (NPROMA, NGPBLKS) and PFSD (NPROMA, KLEV, NGPBLKS) when fsd
+// (LRAD_CLOUD_INHOMOG).
 template <typename R>
 struct KBudPtrs {
   const R *pvfa, *pdyna, *pdynl, *pdyni, *pdyns, *phrsw, *phrlw, *pvfl, *pvfi;
@@ -58,7 +61,57 @@ struct KBudPtrs {
   R* pextra;
   int32_t kfldx, flags;
   R reisthsc, ralfdcp;
+  const R* pgaw;
+  R* pfsd;
+  int32_t fsd;
+  R ra, rpi, rcloud_frac_std;
 };
+
+// Cloud condensate FSD for the radiation (LRAD_CLOUD_INHOMOG; the ZLFSD/ZIFSD/ZFSD of cloudsc.F90).
+// Representative grid-box length in km from the normalised Gaussian weight PGAW (:1944, :4313).
+template <typename R>
+__device__ __forceinline__ R fsd_gridlen(R ra, R rpi, R pgaw) {
+  return R(2) * ra * m_sqrt(rpi * pgaw) * R(0.001);
+}
+// Liquid FSD where the in-cloud liquid exceeds ZEPSEC (:1937-1967): Boutle et al. (2013) with the IFS parameters,
+// from the tidied vapour and liquid and the tidied cloud fraction.
+template <typename R>
+__device__ __forceinline__ R liquid_fsd(R qv, R ql, R za, R zgridlen) {
+  const R zqtot = m_min((qv + ql) * R(1000.0), R(30.0));
+  const R zphip1 = R(0.2) + R(0.01) * zqtot + R(0.0027) * (zqtot * zqtot) - R(0.00008) * (zqtot * zqtot * zqtot);
+  const R zphip2 = zphip1 - R(0.2);
+  const R zphip3 = R(0.123) * m_exp(-(zphip1 * R(0.55)));
+  const R zanew2 = m_max(za, R(0.1));
+  const R zphic = m_pow(zgridlen * zanew2, R(1) / R(3)) *
+                  m_pow(m_pow(zphip3 * zgridlen * zanew2, R(1.5)) + R(3) * zphip3, R(-0.17));
+  R zfracsdc = (zphip1 - zphip2 * zanew2) * zphic;
+  if (za > R(0.95)) zfracsdc = R(0.17) * zphic;
+  return R(1.3) * zfracsdc;  // ZR12, the 1D-to-2D enhancement
+}
+// PFSD of one level (:4307-4359).  qv, qi: tidied state; zanewp: ZANEWP (:3818-3822); zratfsd: ZRATFSD
+// (:4112-4121); zicef, zliqf: final ice and liquid (:4123-4124); zlfsd: the liquid FSD of the level.  The
+// literals .5, .001, .24, .11, 10. and .03 of :4320-4322 and 0.00001 of :4315 are default real (FP32) in the
+// reference and are kept so.
+template <typename R>
+__device__ __forceinline__ R cloud_fsd(R rcloud_frac_std, R zgridlen, R qv, R qi, R zdp, R zrho, R rg, R zanewp,
+                                       R zratfsd, R zicef, R zliqf, R zlfsd) {
+  const R zifsdback = R(0.446585) * m_pow(zgridlen, R(1) / R(3)) *
+                      m_pow(m_pow(R(0.308061) * zgridlen, R(0.395736)) + R(1), R(-0.744527));
+  const R zqtot = m_max(m_min((qv + qi) * R(1000.0), R(30.0)), R(0.00001f));
+  R zdelz = m_div(R(1), rg * zrho) * zdp * R(0.001);
+  zdelz = m_max(m_min(zdelz, R(0.5f)), R(0.001f));
+  const R zphi = m_pow(m_div(zdelz, R(0.24f)), R(0.11f)) * m_pow(m_div(zqtot, R(10.0f)), R(0.03f));
+  R zifsd = rcloud_frac_std;
+  if (zanewp > R(0.95)) {  // overcast
+    zifsd = R(1.3) * (zifsdback * zphi + zratfsd * R(1.5));
+  } else if (zanewp > R(0.001) && zanewp <= R(0.95)) {
+    zifsd = R(1.3) * (zifsdback * zphi * R(1.5) + zratfsd * R(1.5));
+  }
+  R zfsd = rcloud_frac_std;
+  if (zicef > R(1.e-14)) zfsd = zifsd;
+  if (zliqf > R(0) && zicef <= R(1.e-14)) zfsd = zlfsd;
+  return m_min(m_max(R(0.1), zfsd), R(3.575));
+}
 
 // The vertical-integral field (IS+1) adds into slots 4-12, 19-39, 44-58 (process terms) and 70-82 (counters and
 // PEIS) at every level of the column and assigns slots 60 and 83.  The 42 process sums live in LDS for the whole

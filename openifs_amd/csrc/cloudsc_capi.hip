@@ -39,15 +39,25 @@ int fail(int code, const char* what, hipError_t e) {
   return code;
 }
 
-int check_config(const cloudsc_params_t* p, bool budget) {
+int check_config(const cloudsc_params_t* p, bool diag) {
   if (!p) return fail(CLOUDSC_ENULL, "cloudsc: NULL parameter block");
-  if (p->lcloud_inhomog || p->lrad_cloud_inhomog || p->laeraware || p->lspp)
-    return fail(CLOUDSC_EUNSUPPORTED, "cloudsc: LCLOUD_INHOMOG/LRAD_CLOUD_INHOMOG/aerosol/SPP are not implemented");
+  if (p->lcloud_inhomog || p->laeraware || p->lspp)
+    return fail(CLOUDSC_EUNSUPPORTED, "cloudsc: LCLOUD_INHOMOG/aerosol/SPP are not implemented");
   const int32_t out = CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T | CLOUDSC_BUD_VERTINT;
-  if ((p->lcldbud & out) && !budget)
+  if ((p->lcldbud & out) && !diag)
     return fail(CLOUDSC_EUNSUPPORTED,
-                "cloudsc: the budget diagnostics write PEXTRA: use the per-block entry or cloudsc_blocked_budget_*_c");
+                "cloudsc: the budget diagnostics write PEXTRA: use the per-block entry or cloudsc_blocked_diag_*_c");
+  if (p->lrad_cloud_inhomog && !diag)
+    return fail(CLOUDSC_EUNSUPPORTED,
+                "cloudsc: LRAD_CLOUD_INHOMOG writes PFSD: use the per-block entry or cloudsc_blocked_diag_*_c");
   if (p->lcldbud & ~255) return fail(CLOUDSC_EUNSUPPORTED, "cloudsc: unknown budget switch bits");
+  return CLOUDSC_OK;
+}
+
+// PGAW and PFSD where LRAD_CLOUD_INHOMOG asks for the condensate FSD.
+int check_fsd(const cloudsc_params_t* p, const void* pgaw, const void* pfsd) {
+  if (p->lrad_cloud_inhomog && (!pgaw || !pfsd))
+    return fail(CLOUDSC_ENULL, "cloudsc: LRAD_CLOUD_INHOMOG needs PGAW and PFSD");
   return CLOUDSC_OK;
 }
 
@@ -90,6 +100,7 @@ namespace {
 using cloudsc_capi::check_device;
 using cloudsc_capi::check_budget;
 using cloudsc_capi::check_config;
+using cloudsc_capi::check_fsd;
 using cloudsc_capi::fail;
 
 template <typename R, typename RD>
@@ -124,12 +135,15 @@ int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngp
                           bud->kpbltype, bud->kctop, bud->pextra};
     rc = check_budget(q, klev, bud->kfldx, breq, 10);
     if (rc) return rc;
+    rc = check_fsd(q, bud->pgaw, bud->pfsd);
+    if (rc) return rc;
   }
+  const bool diag = budgets || (bud && bud->fsd);
   rc = check_device();
   if (rc) return rc;
   KDims d{ngptot, nproma, klev, ngpblks, 1, nproma};
   hipError_t e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, 0, (hipStream_t)stream, sat,
-                                                    budgets ? bud : nullptr);
+                                                    diag ? bud : nullptr);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_blocked: kernel launch", e);
   return CLOUDSC_OK;
 }
@@ -175,7 +189,7 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
               R* prainfrac_toprfz, R* pfsqlf, R* pfsqif, R* pfcqnng, R* pfcqlng, R* pfsqrf, R* pfsqsf, R* pfcqrng,
               R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn, R* pfhpsl, R* pfhpsn, const R* pvfa,
               const R* pdyna, const R* pdynl, const R* pdyni, const R* pdyns, const R* phrsw, const R* phrlw,
-              const int32_t* kpbltype, const int32_t* kctop, R* pextra, int kfldx) {
+              const int32_t* kpbltype, const int32_t* kctop, R* pextra, int kfldx, const R* pgaw, R* pfsd) {
   int rc = check_config(q, true);
   if (rc) return rc;
   if (klon < 1 || klev < 2 || kidia < 1 || kfdia < kidia || kfdia > klon || kidia + kfdia - 1 > klon)
@@ -189,9 +203,12 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
     if (!x) return fail(CLOUDSC_ENULL, "cloudsc: NULL array on the default path");
   const bool budgets = q->lcldbud & (CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T |
                                      CLOUDSC_BUD_VERTINT);
+  const bool fsd = q->lrad_cloud_inhomog != 0;
   {
     const void* breq[] = {pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra};
     rc = check_budget(q, klev, kfldx, breq, 10);
+    if (rc) return rc;
+    rc = check_fsd(q, pgaw, pfsd);
     if (rc) return rc;
   }
   rc = check_device();
@@ -233,7 +250,9 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
                 {phrlw, nullptr, budgets ? n2 : 0, sizeof(R)},
                 {kpbltype, nullptr, budgets ? n1 : 0, sizeof(int32_t)},
                 {kctop, nullptr, budgets ? n1 : 0, sizeof(int32_t)},
-                {pextra, budgets ? pextra : nullptr, budgets ? n2 * (size_t)kfldx : 0, sizeof(R)}};
+                {pextra, budgets ? pextra : nullptr, budgets ? n2 * (size_t)kfldx : 0, sizeof(R)},
+                // condensate FSD (LRAD_CLOUD_INHOMOG)
+                {pgaw, nullptr, fsd ? n1 : 0, sizeof(R)},       {pfsd, fsd ? pfsd : nullptr, fsd ? n2 : 0, sizeof(R)}};
   const int narr = sizeof(arrs) / sizeof(arrs[0]);
   size_t off[sizeof(arrs) / sizeof(arrs[0])];
   size_t total = 0;
@@ -266,11 +285,16 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
   if (budgets) {
     bp.pvfa = D(42); bp.pdyna = D(43); bp.pdynl = D(44); bp.pdyni = D(45); bp.pdyns = D(46); bp.phrsw = D(47);
     bp.phrlw = D(48); bp.kpbltype = (const int32_t*)D(49); bp.kctop = (const int32_t*)D(50); bp.pextra = D(51);
-    bp.pvfl = p.pvfl; bp.pvfi = p.pvfi;
-    bp.kfldx = kfldx; bp.flags = q->lcldbud; bp.reisthsc = R(q->reisthsc); bp.ralfdcp = R(q->ralfdcp);
+    bp.kfldx = kfldx; bp.flags = q->lcldbud;
   }
+  if (fsd) {
+    bp.pgaw = D(52); bp.pfsd = D(53); bp.fsd = 1;
+  }
+  bp.pvfl = p.pvfl; bp.pvfi = p.pvfi;
+  bp.reisthsc = R(q->reisthsc); bp.ralfdcp = R(q->ralfdcp);
+  bp.ra = R(q->ra); bp.rpi = R(q->rpi); bp.rcloud_frac_std = R(q->rcloud_frac_std);
   e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, kidia - 1, w->stream, nullptr,
-                                         budgets ? &bp : nullptr);
+                                         (budgets || fsd) ? &bp : nullptr);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: kernel launch", e);
   for (int i = 0; i < narr && e == hipSuccess; ++i)
     if (arrs[i].host_out && arrs[i].n)
@@ -304,18 +328,21 @@ KPtrs<R> blocked_ptrs(const R* pt, const R* pq, const R* tcml_t, const R* tcml_q
 
 #define PER_BLOCK_CALL(R, RD)                                                                                   \
   (void)lddiag, (void)pdynr, (void)pvervel, (void)pgeom, (void)pgaw, (void)pgp2dspp, (void)ldslphy,              \
-      (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice, (void)pfsd;                   \
+      (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice;                               \
   return per_block<R, RD>(ydparams, kidia, kfdia, klon, klev, ptsphy, pt, pq, ptendency_cml_t, ptendency_cml_q,  \
                           ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a,   \
                           ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis, plu, plude, pludeli, \
                           psnde, pmfu, pmfd, pa, pclv, pcovptot, prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng,        \
                           pfcqlng, pfsqrf, pfsqsf, pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl,   \
-                          pfhpsn, pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx)
+                          pfhpsn, pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx,   \
+                          pgaw, pfsd)
 
-#define BUDGET_CALL(R, RD)                                                                                       \
+#define DIAG_CALL(R, RD)                                                                                         \
+  const cloudsc_params_t* qd = ydparams;                                                                          \
   cloudsc_gpu::KBudPtrs<R> bud{pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, pvfl, pvfi, kpbltype, kctop,        \
-                               pextra, kfldx, ydparams ? ydparams->lcldbud : 0,                                  \
-                               R(ydparams ? ydparams->reisthsc : 0.0), R(ydparams ? ydparams->ralfdcp : 0.0)};    \
+                               pextra, kfldx, qd ? qd->lcldbud : 0, R(qd ? qd->reisthsc : 0.0),                   \
+                               R(qd ? qd->ralfdcp : 0.0), pgaw, pfsd, qd ? qd->lrad_cloud_inhomog : 0,            \
+                               R(qd ? qd->ra : 0.0), R(qd ? qd->rpi : 0.0), R(qd ? qd->rcloud_frac_std : 0.0)};   \
   return blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy,                                          \
                         blocked_ptrs<R>(pt, pq, ptendency_cml_t, ptendency_cml_q, ptendency_cml_a,                \
                                         ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a,     \
@@ -354,9 +381,9 @@ int cloudsc_satadj_cloudsc_blocked_dp_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(doub
 int cloudsc_satadj_cloudsc_blocked_sp_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(float)) { FUSED_CALL(float, float); }
 int cloudsc_satadj_cloudsc_blocked_mixed_c(CLOUDSC_SATADJ_CLOUDSC_BLOCKED_ARGS(float)) { FUSED_CALL(float, double); }
 
-int cloudsc_blocked_budget_dp_c(CLOUDSC_BLOCKED_BUDGET_ARGS(double)) { BUDGET_CALL(double, double); }
-int cloudsc_blocked_budget_sp_c(CLOUDSC_BLOCKED_BUDGET_ARGS(float)) { BUDGET_CALL(float, float); }
-int cloudsc_blocked_budget_mixed_c(CLOUDSC_BLOCKED_BUDGET_ARGS(float)) { BUDGET_CALL(float, double); }
+int cloudsc_blocked_diag_dp_c(CLOUDSC_BLOCKED_DIAG_ARGS(double)) { DIAG_CALL(double, double); }
+int cloudsc_blocked_diag_sp_c(CLOUDSC_BLOCKED_DIAG_ARGS(float)) { DIAG_CALL(float, float); }
+int cloudsc_blocked_diag_mixed_c(CLOUDSC_BLOCKED_DIAG_ARGS(float)) { DIAG_CALL(float, double); }
 
 int cloudsc_dp_c(CLOUDSC_ARGS(double)) { PER_BLOCK_CALL(double, double); }
 int cloudsc_sp_c(CLOUDSC_ARGS(float)) { PER_BLOCK_CALL(float, float); }

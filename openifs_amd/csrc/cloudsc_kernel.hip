@@ -70,6 +70,8 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
     vacc.s = s_bud + NBUD * BLOCK + threadIdx.x;
     if (bp.flags & BUD_VERTINT) vertint_begin(vacc, bp, ext);
   }
+  // LRAD_CLOUD_INHOMOG: grid length of the column (diagnostic instantiation only)
+  const R zgridlen = (BUD && bp.fsd) ? fsd_gridlen(bp.ra, bp.rpi, LD(bp.pgaw, u1, jo)) : R(0);
 
   // Per-column inputs
   const R plsm = LD(p.plsm, u1, jo);
@@ -236,6 +238,9 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       ST(p.pfhpsl, ih0, jo, -c.rlvtt * fl);
       ST(p.pfhpsn, ih0, jo, -c.rlstt * fn);
     }
+
+    // ZLFSD, ZANEWP, ZRATFSD of this level (LRAD_CLOUD_INHOMOG); above NCLDTOP they keep these values (:944-951)
+    R fsd_liq = BUD ? bp.rcloud_frac_std : R(0), fsd_anewp = R(0), fsd_rat = R(0);
 
     if ((int)k >= ncldtop0) {
       const bool above_top = (int)k > ncldtop0;  // JK > NCLDTOP
@@ -426,6 +431,9 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
 
       // 4.3 warm rain, implicit KK2000 (:2050-2254)
       if (zliqcld > zepsec) {
+        if constexpr (BUD) {
+          if (bp.fsd) fsd_liq = liquid_fsd(qv, ql, za, zgridlen);  // (:1937-1971)
+        }
         zrainaut = c.c_aut * m_pow(zliqcld, c.rcl_kkbauq - R(1)) * zconst_pow;
         qbrl = qbrl + zrainaut;
         if (zraincld > zepsec)
@@ -834,11 +842,32 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
         }
       }
       ST(p.pcovptot, i2, jo, zcovptot);  // (:3814-3816)
-      if constexpr (BUD) {  // section 7 (:3826-4101)
-        const R zdz = m_div(zdp, zrho * c.rg);
-        if constexpr (sizeof(R) == 8) budget_prefetch(bud_pre, bp, ext, k + 1);
-        budget_write<R>(bud, vacc, bud_pre, bp, ext, k + 1, i2 + jo, zqtmst, ptsphy, zdz, qxn_l, qxn_i, c.ralvdcp,
-                        c.ralsdcp, bp.ralfdcp);
+      if constexpr (BUD) {
+        if (bp.fsd) {
+          // cloud fraction at the end of the step with the precipitation fraction (:3818-3822); ZDA = ZANEW - ZAORIG
+          const R a_end = pa + (zanewm1 - zaorig);
+          if (a_end > R(0.001)) fsd_anewp = m_max(m_max(a_end, zcovptot), R(0.1));
+          // detrained share of the condensate (:4112-4121), from PLUDE as rescaled above
+          if (ql0 + qi0 + plude > rlmin) fsd_rat = m_max(R(0), m_min(R(1), m_div(plude, plude + ql0 + qi0)));
+        }
+      }
+      // section 7 (:3826-4101), only when a budget writes PEXTRA: otherwise the budget inputs may be NULL (a run
+      // with only LRAD_CLOUD_INHOMOG)
+      if constexpr (BUD) {
+        if (bp.flags & BUD_OUT) {
+          const R zdz = m_div(zdp, zrho * c.rg);
+          if constexpr (sizeof(R) == 8) budget_prefetch(bud_pre, bp, ext, k + 1);
+          budget_write<R>(bud, vacc, bud_pre, bp, ext, k + 1, i2 + jo, zqtmst, ptsphy, zdz, qxn_l, qxn_i, c.ralvdcp,
+                          c.ralsdcp, bp.ralfdcp);
+        }
+      }
+    }
+
+    if constexpr (BUD) {
+      if (bp.fsd) {  // PFSD of every level (:4307-4359); ZQX and ZTP1 are the tidied start-of-step state
+        const R zrho_f = m_div(pap, c.rd * ztp1);
+        ST(bp.pfsd, i2, jo, cloud_fsd(bp.rcloud_frac_std, zgridlen, qv, qi, paph_k1 - paph_k, zrho_f, c.rg,
+                                      fsd_anewp, fsd_rat, qxn_i, qxn_l, fsd_liq));
       }
     }
 
@@ -1095,10 +1124,14 @@ template <typename R>
 KBudPtrs<R> offset_bud_ptrs(const KBudPtrs<R>& p, int64_t b0, int64_t nproma, int64_t klev) {
   const int64_t s2 = nproma * klev;
   KBudPtrs<R> q = p;
-  q.pvfa += b0 * s2; q.pdyna += b0 * s2; q.pdynl += b0 * s2; q.pdyni += b0 * s2; q.pdyns += b0 * s2;
-  q.phrsw += b0 * s2; q.phrlw += b0 * s2; q.pvfl += b0 * s2; q.pvfi += b0 * s2;
-  q.kpbltype += b0 * nproma; q.kctop += b0 * nproma;
-  q.pextra += b0 * s2 * q.kfldx;
+  auto adv = [](auto*& x, int64_t n) {  // arrays of a diagnostic that is off are NULL
+    if (x) x += n;
+  };
+  adv(q.pvfa, b0 * s2); adv(q.pdyna, b0 * s2); adv(q.pdynl, b0 * s2); adv(q.pdyni, b0 * s2);
+  adv(q.pdyns, b0 * s2); adv(q.phrsw, b0 * s2); adv(q.phrlw, b0 * s2); adv(q.pvfl, b0 * s2); adv(q.pvfi, b0 * s2);
+  adv(q.kpbltype, b0 * nproma); adv(q.kctop, b0 * nproma);
+  adv(q.pextra, b0 * s2 * q.kfldx);
+  adv(q.pgaw, b0 * nproma); adv(q.pfsd, b0 * s2);
   return q;
 }
 

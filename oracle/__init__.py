@@ -75,9 +75,10 @@ def cloudsc_blocked(precision, params, ngptot, nproma, klev, ptsphy, arrays, nth
     return arrays
 
 
-def cloudsc_blocked_budget(precision, params, ngptot, nproma, klev, ptsphy, arrays, nthreads=0):
-    """cloudsc_blocked plus the budget diagnostics: ``arrays`` also holds the BUDGET_ARRAYS of
-    openifs_amd.cloudsc and ``pextra`` as (NGPBLKS, KFLDX, KLEV, NPROMA); returns the return code."""
+def cloudsc_blocked_diag(precision, params, ngptot, nproma, klev, ptsphy, arrays, nthreads=0):
+    """cloudsc_blocked plus the diagnostics: ``arrays`` may also hold the BUDGET_ARRAYS of openifs_amd.cloudsc and
+    ``pextra`` as (NGPBLKS, KFLDX, KLEV, NPROMA), and ``pgaw`` (NGPBLKS, NPROMA) and ``pfsd`` (NGPBLKS, KLEV,
+    NPROMA); missing ones are passed as NULL.  Returns the return code."""
     from openifs_amd.cloudsc import BLOCKED_ARRAYS, BUDGET_ARRAYS
 
     ngpblks = (ngptot + nproma - 1) // nproma
@@ -85,17 +86,30 @@ def cloudsc_blocked_budget(precision, params, ngptot, nproma, klev, ptsphy, arra
     rdt = np.float64 if precision == "dp" else np.float32
     args = [ctypes.byref(params), ctypes.c_int(ngptot), ctypes.c_int(nproma), ctypes.c_int(klev),
             ctypes.c_int(ngpblks), real(ptsphy)]
-    for name, kind in [(n, k) for n, k, _ in BLOCKED_ARRAYS] + BUDGET_ARRAYS + [("pextra", "extra")]:
-        a = arrays[name]
+
+    def ptr(name, kind):
+        a = arrays.get(name)
+        if a is None:
+            return ctypes.c_void_p(0)
         want = np.int32 if kind == "1di" else rdt
         if a.dtype != want or not a.flags.c_contiguous:
             raise TypeError(f"{name}: need a C-contiguous {np.dtype(want).name} array")
-        args.append(ctypes.c_void_p(a.ctypes.data))
-    px = arrays["pextra"]
-    if px.shape != (ngpblks, px.shape[1], klev, nproma):
-        raise ValueError("pextra: need (NGPBLKS, KFLDX, KLEV, NPROMA)")
-    args += [ctypes.c_int(px.shape[1]), ctypes.c_int(nthreads)]
-    return getattr(lib(), f"oracle_cloudsc_blocked_budget_{precision}")(*args)
+        return ctypes.c_void_p(a.ctypes.data)
+
+    for name, kind, _ in BLOCKED_ARRAYS:
+        if name not in arrays:
+            raise KeyError(name)
+        args.append(ptr(name, kind))
+    args += [ptr(n, k) for n, k in BUDGET_ARRAYS]
+    px = arrays.get("pextra")
+    kfldx = 0
+    if px is not None:
+        if px.shape != (ngpblks, px.shape[1], klev, nproma):
+            raise ValueError("pextra: need (NGPBLKS, KFLDX, KLEV, NPROMA)")
+        kfldx = px.shape[1]
+    args += [ptr("pextra", "extra"), ctypes.c_int(kfldx), ptr("pgaw", "1d"), ptr("pfsd", "2d"),
+             ctypes.c_int(nthreads)]
+    return getattr(lib(), f"oracle_cloudsc_blocked_diag_{precision}")(*args)
 
 
 def supersatcheck(params, t, q, a, pp, psurf, precision="dp"):

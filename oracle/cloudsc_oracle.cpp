@@ -6,7 +6,8 @@
 //   oracle_cloudsc_{dp,sp,mixed}        one NPROMA block, CLOUDSC's argument list
 //   oracle_cloudsc_blocked_{dp,sp,mixed} all blocks, OpenMP over blocks with
 //       SCHEDULE(DYNAMIC,1) as the reference driver (ec_phys_drv.F90:430-439)
-//   oracle_cloudsc_blocked_budget_{dp,sp,mixed}  the same with the LCLDBUD* budgets into PEXTRA
+//   oracle_cloudsc_blocked_diag_{dp,sp,mixed}  the same with the LCLDBUD* budgets into PEXTRA and the PFSD of
+//       LRAD_CLOUD_INHOMOG
 //   oracle_satadj_{dp,sp,mixed}         one block of CLOUD_SATADJ, the argument list of
 //       include/cloudsc_satadj.h
 //   oracle_satadj_blocked_{dp,sp,mixed}  all blocks, OpenMP over blocks
@@ -24,11 +25,12 @@ namespace {
 
 constexpr int32_t kBudOut = CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T | CLOUDSC_BUD_VERTINT;
 
-// The same rules as the product library: budgets that write PEXTRA only where PEXTRA is an argument.
-int check_config(const cloudsc_params_t* p, bool budget = false) {
+// The same rules as the product library: budgets that write PEXTRA and the PFSD of LRAD_CLOUD_INHOMOG only where
+// PEXTRA and PFSD are arguments.
+int check_config(const cloudsc_params_t* p, bool diag = false) {
   if (!p) return CLOUDSC_ENULL;
-  if (p->lcloud_inhomog || p->lrad_cloud_inhomog || p->laeraware || p->lspp) return CLOUDSC_EUNSUPPORTED;
-  if ((p->lcldbud & kBudOut) && !budget) return CLOUDSC_EUNSUPPORTED;
+  if (p->lcloud_inhomog || p->laeraware || p->lspp) return CLOUDSC_EUNSUPPORTED;
+  if (((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog) && !diag) return CLOUDSC_EUNSUPPORTED;
   if (p->lcldbud & ~255) return CLOUDSC_EUNSUPPORTED;
   return CLOUDSC_OK;
 }
@@ -46,6 +48,7 @@ int budget_fields(int32_t f) {
 
 template <typename B>
 int check_budget(const cloudsc_params_t* p, int klev, const B& b) {
+  if (p->lrad_cloud_inhomog && (!b.pgaw || !b.pfsd)) return CLOUDSC_ENULL;
   if (!(p->lcldbud & kBudOut)) return CLOUDSC_OK;
   const void* req[] = {b.pvfa, b.pdyna, b.pdynl, b.pdyni, b.pdyns, b.phrsw, b.phrlw, b.kpbltype, b.kctop, b.pextra};
   for (const void* x : req)
@@ -83,15 +86,16 @@ int run_block(const cloudsc_params_t* p, int kidia, int kfdia, int klon, int kle
                     ldcum,  ktype,  peis,    plu,     plude,   pludeli,  psnde,   pmfu,    pmfd,    pa,
                     pclv,   pcovptot, prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf,
                     pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn,
-                    (p->lcldbud & kBudOut) ? &bud : nullptr};
+                    ((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog) ? &bud : nullptr};
   oracle::cloudsc<R, RD>(*p, a);
   return CLOUDSC_OK;
 }
 
 #define ORACLE_BLOCK_CALL(R, RD)                                                                              \
   (void)lddiag, (void)pdynr, (void)pvervel, (void)pgeom, (void)pgaw, (void)pgp2dspp, (void)ldslphy,          \
-      (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice, (void)pfsd;                   \
-  const oracle::BudArgs<R> bud{pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx}; \
+      (void)plcrit_aer, (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice;                               \
+  const oracle::BudArgs<R> bud{pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx,  \
+                               pgaw, pfsd};                                                                     \
   return run_block<R, RD>(ydparams, kidia, kfdia, klon, klev, ptsphy, pt, pq, ptendency_cml_t, ptendency_cml_q, \
                           ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a, \
                           ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis, plu, plude,        \
@@ -117,7 +121,7 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
     rc = check_budget(p, klev, *bud);
     if (rc) return rc;
   }
-  const bool budgets = bud && (p->lcldbud & kBudOut);
+  const bool budgets = bud && ((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog);
   const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1), s1 = nproma;
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)
@@ -126,9 +130,13 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
     const size_t o2 = ibl * s2, oh = ibl * sh, o1 = ibl * s1, o5 = ibl * s2 * 5, o4 = ibl * s2 * 4,
                  o2b = ibl * s2 * 2;
     oracle::BudArgs<R> bb{};
-    if (budgets)  // PEXTRA block IBL is (NPROMA,KLEV,KFLDX)
-      bb = {bud->pvfa + o2,  bud->pdyna + o2, bud->pdynl + o2, bud->pdyni + o2, bud->pdyns + o2, bud->phrsw + o2,
-            bud->phrlw + o2, bud->kpbltype + o1, bud->kctop + o1, bud->pextra + o2 * bud->kfldx, bud->kfldx};
+    if (budgets) {  // PEXTRA block IBL is (NPROMA,KLEV,KFLDX); arrays of a diagnostic that is off may be NULL
+      auto at = [](auto* x, size_t n) { return x ? x + n : x; };
+      bb = {at(bud->pvfa, o2),     at(bud->pdyna, o2),    at(bud->pdynl, o2), at(bud->pdyni, o2),
+            at(bud->pdyns, o2),    at(bud->phrsw, o2),    at(bud->phrlw, o2), at(bud->kpbltype, o1),
+            at(bud->kctop, o1),    at(bud->pextra, o2 * bud->kfldx), bud->kfldx, at(bud->pgaw, o1),
+            at(bud->pfsd, o2)};
+    }
     oracle::Args<R> a{1,
                       kfdia,
                       nproma,
@@ -201,9 +209,9 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
       R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur,   \
       R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, int nthreads
 
-// The blocked oracle with the budget diagnostics: the arguments of cloudsc_blocked_budget_*_c in their order
-// (host arrays), then the thread count.
-#define ORACLE_BLOCKED_BUDGET_ARGS(R)                                                                   \
+// The blocked oracle with the diagnostics: the arguments of cloudsc_blocked_diag_*_c in their order (host arrays),
+// then the thread count.
+#define ORACLE_BLOCKED_DIAG_ARGS(R)                                                                   \
   const cloudsc_params_t *ydparams, int ngptot, int nproma, int klev, int ngpblks, R ptsphy,            \
       const R *pt, const R *pq, const R *ptendency_cml_t, const R *ptendency_cml_q,                     \
       const R *ptendency_cml_a, const R *ptendency_cml_cld, R *ptendency_loc_t, R *ptendency_loc_q,     \
@@ -214,10 +222,11 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
       R *pfcqnng, R *pfcqlng, R *pfsqrf, R *pfsqsf, R *pfcqrng, R *pfcqsng, R *pfsqltur, R *pfsqitur,   \
       R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, const R *pvfa, const R *pdyna, const R *pdynl,        \
       const R *pdyni, const R *pdyns, const R *phrsw, const R *phrlw, const int32_t *kpbltype,          \
-      const int32_t *kctop, R *pextra, int kfldx, int nthreads
+      const int32_t *kctop, R *pextra, int kfldx, const R *pgaw, R *pfsd, int nthreads
 
-#define ORACLE_BLOCKED_BUDGET_CALL(R, RD)                                                                       \
-  const oracle::BudArgs<R> bud{pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx};   \
+#define ORACLE_BLOCKED_DIAG_CALL(R, RD)                                                                         \
+  const oracle::BudArgs<R> bud{pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx,    \
+                               pgaw, pfsd};                                                                       \
   return run_blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy, pt, pq, ptendency_cml_t,          \
                             ptendency_cml_q, ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, \
                             ptendency_loc_a, ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm, ldcum, ktype, peis,  \
@@ -327,10 +336,10 @@ int oracle_cloudsc_blocked_dp(ORACLE_BLOCKED_ARGS(double)) { ORACLE_BLOCKED_CALL
 int oracle_cloudsc_blocked_sp(ORACLE_BLOCKED_ARGS(float)) { ORACLE_BLOCKED_CALL(float, float); }
 int oracle_cloudsc_blocked_mixed(ORACLE_BLOCKED_ARGS(float)) { ORACLE_BLOCKED_CALL(float, double); }
 
-int oracle_cloudsc_blocked_budget_dp(ORACLE_BLOCKED_BUDGET_ARGS(double)) { ORACLE_BLOCKED_BUDGET_CALL(double, double); }
-int oracle_cloudsc_blocked_budget_sp(ORACLE_BLOCKED_BUDGET_ARGS(float)) { ORACLE_BLOCKED_BUDGET_CALL(float, float); }
-int oracle_cloudsc_blocked_budget_mixed(ORACLE_BLOCKED_BUDGET_ARGS(float)) {
-  ORACLE_BLOCKED_BUDGET_CALL(float, double);
+int oracle_cloudsc_blocked_diag_dp(ORACLE_BLOCKED_DIAG_ARGS(double)) { ORACLE_BLOCKED_DIAG_CALL(double, double); }
+int oracle_cloudsc_blocked_diag_sp(ORACLE_BLOCKED_DIAG_ARGS(float)) { ORACLE_BLOCKED_DIAG_CALL(float, float); }
+int oracle_cloudsc_blocked_diag_mixed(ORACLE_BLOCKED_DIAG_ARGS(float)) {
+  ORACLE_BLOCKED_DIAG_CALL(float, double);
 }
 
 // CLOUD_SUPERSATCHECK on n points (KIDIA = 1, KFDIA = n), outputs in RD.

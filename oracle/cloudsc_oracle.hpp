@@ -245,14 +245,17 @@ void implicit_solve(const R qa[NCLV][NCLV], const R qb[NCLV][NCLV], const R fall
   for (int m = 0; m < NCLV; ++m) qxn[m] = R(x[m]);
 }
 
-// Inputs and output of the budget diagnostics (LCLDBUD*, cloudsc.F90:3826-4101) for one call, Fortran layout:
-// (KLON,KLEV) arrays, KPBLTYPE/KCTOP (KLON), PEXTRA (KLON,KLEV,KFLDX).
+// Inputs and outputs of the diagnostics for one call, Fortran layout: the budgets (LCLDBUD*,
+// cloudsc.F90:3826-4101) read (KLON,KLEV) arrays and KPBLTYPE/KCTOP (KLON) and accumulate into PEXTRA
+// (KLON,KLEV,KFLDX); LRAD_CLOUD_INHOMOG reads PGAW (KLON) and writes PFSD (KLON,KLEV).
 template <typename R>
 struct BudArgs {
   const R *pvfa, *pdyna, *pdynl, *pdyni, *pdyns, *phrsw, *phrlw;
   const int32_t *kpbltype, *kctop;
   R* pextra;
   int kfldx;
+  const R* pgaw;
+  R* pfsd;
 };
 
 // Arguments of one CLOUDSC call, Fortran layout (JL fastest).
@@ -270,7 +273,7 @@ struct Args {
   R *pcovptot, *prainfrac_toprfz;
   R *pfsqlf, *pfsqif, *pfcqnng, *pfcqlng, *pfsqrf, *pfsqsf, *pfcqrng, *pfcqsng, *pfsqltur, *pfsqitur;
   R *pfplsl, *pfplsn, *pfhpsl, *pfhpsn;
-  const BudArgs<R>* bud = nullptr;  // budget diagnostics: switched by params.lcldbud, NULL = none
+  const BudArgs<R>* bud = nullptr;  // diagnostics: switched by params.lcldbud and lrad_cloud_inhomog, NULL = none
 };
 
 // ---------------------------------------------------------------------------
@@ -294,6 +297,8 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
   const R RLVTT = R(p.rlvtt), RPI = R(p.rpi);
   const R RALVDCP = R(p.ralvdcp), RALSDCP = R(p.ralsdcp), R5LES = R(p.r5les), R5IES = R(p.r5ies);
   const R RALFDCP = R(p.ralfdcp);  // budget diagnostics only
+  const R RA = R(p.ra), RCLOUD_FRAC_STD = R(p.rcloud_frac_std);  // LRAD_CLOUD_INHOMOG only
+  const R ZR12 = R(1.3), ZU1 = R(0.446585), ZU2 = R(0.308061), ZU3 = R(0.395736), ZU4 = R(-0.744527);  // :587-592
   const R R4LES = R(p.r4les), R4IES = R(p.r4ies);
   const R RAMID = R(p.ramid), RAMIN = R(p.ramin), RLMIN = R(p.rlmin), RCLDIFF = R(p.rcldiff);
   const R RCLDIFF_CONVI = R(p.rcldiff_convi);  // ZLCRIT (RCLCRIT_*) is not used by IWARMRAIN=3
@@ -473,6 +478,17 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
   auto BCC = [&](int jl, int n) -> R& { return zbudcc[(size_t)(n - 1) * KL + (jl - 1)]; };
   auto BL = [&](int jl, int n) -> R& { return zbudl[(size_t)(n - 1) * KL + (jl - 1)]; };
   auto BI = [&](int jl, int n) -> R& { return zbudi[(size_t)(n - 1) * KL + (jl - 1)]; };
+  // LRAD_CLOUD_INHOMOG arrays, initialised for all levels (:944-951)
+  const bool lfsd = a.bud && p.lrad_cloud_inhomog;
+  std::vector<R> zifsd, zlfsd, zliqf, zicef, zratfsd, zanewp;
+  if (lfsd) {
+    zifsd.assign(n2, RCLOUD_FRAC_STD);
+    zlfsd.assign(n2, RCLOUD_FRAC_STD);
+    zliqf.assign(n2, R(0));
+    zicef.assign(n2, R(0));
+    zratfsd.assign(n2, R(0));
+    zanewp.assign(n2, R(0));
+  }
 
   // ======================= level loop (:1219-4128) =======================
   for (int jk = ncldtop; jk <= klev; ++jk) {
@@ -709,6 +725,20 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
     for (int jl = kidia; jl <= kfdia; ++jl) {
       const int i = jl - 1;
       if (zliqcld[i] > ZEPSEC) {
+        if (lfsd) {  // inhomogeneity for the radiation (:1937-1971)
+          const size_t k = i2(jl, jk);
+          const R zqtot = std::min((zqx[i3(jl, jk, QV + 1)] + zqx[i3(jl, jk, QL + 1)]) * R(1000.), R(30.));
+          const R zgridlen = R(2) * RA * std::sqrt(RPI * a.bud->pgaw[i]) * R(0.001);
+          const R zphip1 = R(.2) + R(.01) * zqtot + R(.0027) * (zqtot * zqtot) - R(.00008) * (zqtot * zqtot * zqtot);
+          const R zphip2 = zphip1 - R(0.2);
+          const R zphip3 = R(0.123) * std::exp(-(zphip1 * R(0.55)));
+          const R zanew2 = std::max(za[k], R(0.1));
+          const R zphic = std::pow(zgridlen * zanew2, R(1) / R(3)) *
+                          std::pow(std::pow(zphip3 * zgridlen * zanew2, R(1.5)) + R(3) * zphip3, R(-0.17));
+          R zfracsdc = (zphip1 - zphip2 * zanew2) * zphic;
+          if (za[k] > R(0.95)) zfracsdc = R(0.17) * zphic;
+          zlfsd[k] = ZR12 * zfracsdc;
+        }
         R zconst;
         if (a.plsm[i] > R(0.5)) {
           zconst = RCL_KK_CLOUD_NUM_LAND;
@@ -1156,6 +1186,11 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
       }
     }
     for (int jl = kidia; jl <= kfdia; ++jl) a.pcovptot[i2(jl, jk)] = zcovptot[jl - 1];  // (:3814-3816)
+    if (lfsd)  // cloud fraction/precipitation fraction at the end of the step (:3818-3822)
+      for (int jl = kidia; jl <= kfdia; ++jl) {
+        const size_t k = i2(jl, jk);
+        if (a.pa[k] + zda[jl - 1] > R(0.001)) zanewp[k] = std::max(std::max(a.pa[k] + zda[jl - 1], a.pcovptot[k]), R(0.1));
+      }
 
     // ---- 7. cloud budget diagnostics (:3826-4101) ----
     if (lcldbud) {
@@ -1320,6 +1355,17 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         }
       }
     }
+    if (lfsd)  // for the ice FSD (:4112-4126)
+      for (int jl = kidia; jl <= kfdia; ++jl) {
+        const size_t k = i2(jl, jk);
+        const R q0 = zqx0[i3(jl, jk, QL + 1)] + zqx0[i3(jl, jk, QI + 1)];
+        if (q0 + a.plude[k] > RLMIN) {
+          zratfsd[k] = a.plude[k] / (a.plude[k] + zqx0[i3(jl, jk, QL + 1)] + zqx0[i3(jl, jk, QI + 1)]);
+          zratfsd[k] = std::max(R(0), std::min(R(1), zratfsd[k]));
+        }
+        zicef[k] = V(zqxn, jl, QI);
+        zliqf[k] = V(zqxn, jl, QL);
+      }
   }  // level loop
 
   // ---- 8. fluxes (:4218-4303) ----
@@ -1366,6 +1412,36 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
       a.pfhpsl[i2(jl, jk)] = -RLVTT * a.pfplsl[i2(jl, jk)];
       a.pfhpsn[i2(jl, jk)] = -RLSTT * a.pfplsn[i2(jl, jk)];
     }
+
+  // Ice FSD and PFSD (:4307-4359).  The literals .5, .001, .24, .11, 10., .03 and 0.00001 are default real in the
+  // reference: FP32 values, widened.
+  if (lfsd) {
+    for (int jk = 1; jk <= klev; ++jk)
+      for (int jl = kidia; jl <= kfdia; ++jl) {
+        const int i = jl - 1;
+        const size_t k = i2(jl, jk);
+        const R zgridlen = R(2) * RA * std::sqrt(RPI * a.bud->pgaw[i]) * R(0.001);
+        const R zifsdback = ZU1 * std::pow(zgridlen, R(1) / R(3)) * std::pow(std::pow(ZU2 * zgridlen, ZU3) + R(1), ZU4);
+        const R zqtot = std::max(std::min((zqx[i3(jl, jk, QV + 1)] + zqx[i3(jl, jk, QI + 1)]) * R(1000.), R(30.)),
+                                 R(0.00001f));
+        const R zdp_k = a.paph[i2(jl, jk + 1)] - a.paph[k];
+        const R zrho_k = a.pap[k] / (RD_ * ztp1[k]);
+        R zdelz = R(1) / (RG * zrho_k) * zdp_k * R(0.001);
+        zdelz = std::max(std::min(zdelz, R(.5f)), R(.001f));
+        const R zphi = std::pow(zdelz / R(.24f), R(.11f)) * std::pow(zqtot / R(10.f), R(.03f));
+        if (zanewp[k] > R(0.95)) {
+          zifsd[k] = zifsdback * zphi;
+          zifsd[k] = ZR12 * (zifsd[k] + zratfsd[k] * R(1.5));
+        } else if (zanewp[k] > R(0.001) && zanewp[k] <= R(0.95)) {
+          zifsd[k] = zifsdback * zphi * R(1.5);
+          zifsd[k] = ZR12 * (zifsd[k] + zratfsd[k] * R(1.5));
+        }
+        R zfsd = RCLOUD_FRAC_STD;
+        if (zicef[k] > ZEPSEC) zfsd = zifsd[k];
+        if (zliqf[k] > R(0) && zicef[k] <= ZEPSEC) zfsd = zlfsd[k];
+        a.bud->pfsd[k] = std::min(std::max(R(0.1), zfsd), R(3.575));
+      }
+  }
 }
 
 }  // namespace oracle

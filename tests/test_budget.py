@@ -69,7 +69,7 @@ def run_oracle(prec, p, inp, ngptot, nproma, klev, flags, kfldx=None, pextra0=No
             arr[name] = np.full(C.blocked_shape(kind, nproma, klev, ngpblks), np.nan, dtype=dt)
     shape = C.extra_shape(nproma, klev, max(kfldx, 1), ngpblks)
     arr["pextra"] = np.zeros(shape, dtype=dt) if pextra0 is None else pextra0.astype(dt).copy()
-    rc = oracle.cloudsc_blocked_budget(prec, with_flags(p, flags), ngptot, nproma, klev, TSPHY, arr, nthreads=8)
+    rc = oracle.cloudsc_blocked_diag(prec, with_flags(p, flags), ngptot, nproma, klev, TSPHY, arr, nthreads=8)
     assert rc == 0, rc
     return arr
 
@@ -302,7 +302,7 @@ def test_vertical_integral_needs_83_levels(params137):
         arr[n] = np.zeros(C.blocked_shape(k, nproma, klev, 1), dtype=np.int32 if k == "1di" else np.float64)
     arr["pextra"] = np.zeros(C.extra_shape(nproma, klev, 1, 1))
     q.ncldtop = 2
-    assert oracle.cloudsc_blocked_budget("dp", q, nproma, nproma, klev, TSPHY, arr) == -1
+    assert oracle.cloudsc_blocked_diag("dp", q, nproma, nproma, klev, TSPHY, arr) == -1
 
 
 def test_timeint_lbud23_and_accumulation(case, params137):
@@ -363,12 +363,17 @@ def test_budget_argument_errors(params137):
            for n, k in C.BUDGET_ARRAYS}
     px = np.zeros(C.extra_shape(nproma, klev, 42, ngpblks), dtype=np.float32)
 
-    def call(q, kfldx, drop=None):
-        args = [ctypes.byref(q), ngptot, nproma, klev, ngpblks, ctypes.c_float(900.0)]
+    gaw = np.zeros((ngpblks, nproma), dtype=np.float32)
+    fsd = np.zeros((ngpblks, klev, nproma), dtype=np.float32)
+
+    def call(q, kfldx, drop=None, klev_=klev):
+        args = [ctypes.byref(q), ngptot, nproma, klev_, ngpblks, ctypes.c_float(900.0)]
         args += [ctypes.c_void_p(arr[n].ctypes.data) for n, _, _ in C.BLOCKED_ARRAYS]
         args += [ctypes.c_void_p(0 if n == drop else bud[n].ctypes.data) for n, _ in C.BUDGET_ARRAYS]
-        args += [ctypes.c_void_p(0 if drop == "pextra" else px.ctypes.data), kfldx, ctypes.c_void_p(0)]
-        return lib.cloudsc_blocked_budget_sp_c(*args)
+        args += [ctypes.c_void_p(0 if drop == "pextra" else px.ctypes.data), kfldx,
+                 ctypes.c_void_p(0 if drop == "pgaw" else gaw.ctypes.data),
+                 ctypes.c_void_p(0 if drop == "pfsd" else fsd.ctypes.data), ctypes.c_void_p(0)]
+        return lib.cloudsc_blocked_diag_sp_c(*args)
 
     assert call(p, 41) == -1  # L + I need 42 fields
     assert b"KFLDX" in lib.cloudsc_last_error()
@@ -377,17 +382,21 @@ def test_budget_argument_errors(params137):
     q = with_flags(params137, ALL | C.BUD_LBUD23)
     assert call(q, 93) == -1
     q = with_flags(params137, C.BUD_VERTINT)
-    klev_small = 82
-    assert lib.cloudsc_blocked_budget_sp_c(ctypes.byref(q), ngptot, nproma, klev_small, ngpblks,
-                                           ctypes.c_float(900.0),
-                                           *[ctypes.c_void_p(arr[n].ctypes.data) for n, _, _ in C.BLOCKED_ARRAYS],
-                                           *[ctypes.c_void_p(bud[n].ctypes.data) for n, _ in C.BUDGET_ARRAYS],
-                                           ctypes.c_void_p(px.ctypes.data), 42, ctypes.c_void_p(0)) == -1
+    assert call(q, 42, klev_=82) == -1
     assert b"83" in lib.cloudsc_last_error()
+    # LRAD_CLOUD_INHOMOG needs PGAW and PFSD; the entries without PFSD refuse it
+    q = with_flags(params137, 0)
+    q.lrad_cloud_inhomog = 1
+    for n in ("pgaw", "pfsd"):
+        assert call(q, 0, drop=n) == -3
+    args = [ctypes.byref(q), ngptot, nproma, klev, ngpblks, ctypes.c_float(900.0)]
+    args += [ctypes.c_void_p(arr[n].ctypes.data) for n, _, _ in C.BLOCKED_ARRAYS] + [ctypes.c_void_p(0)]
+    assert lib.cloudsc_blocked_sp_c(*args) == -2
+    assert b"PFSD" in lib.cloudsc_last_error()
     # the oracle's rules are the same
     ar = {k: v.astype(np.float64) if v.dtype.kind == "f" else v for k, v in {**arr, **bud}.items()}
     ar["pextra"] = px.astype(np.float64)
-    assert oracle.cloudsc_blocked_budget("dp", with_flags(params137, C.BUD_L | C.BUD_I | C.BUD_T), ngptot, nproma,
+    assert oracle.cloudsc_blocked_diag("dp", with_flags(params137, C.BUD_L | C.BUD_I | C.BUD_T), ngptot, nproma,
                                          klev, TSPHY, ar) == -1
     # the per-block entry takes PEXTRA: KFLDX is checked there too
     a = C.allocate_block(8, 137, "sp", kfldx=20)
@@ -414,7 +423,7 @@ def _gpu_budget_run(p, prec, ngptot, nproma, klev, seed, flags, pextra0=None):
     if pextra0 is None:
         pextra0 = np.random.default_rng(seed).normal(0, 1e-3, C.extra_shape(nproma, klev, kfldx, ngpblks))
     bud["pextra"] = torch.from_numpy(pextra0).to("cuda", dt)
-    C.cloudsc_blocked_budget(f, bud, with_flags(p, flags), TSPHY)
+    C.cloudsc_blocked_diag(f, bud, with_flags(p, flags), TSPHY)
     torch.cuda.synchronize()
     out = {n: f[n].cpu().numpy() for n in C.OUTPUT_NAMES}
     out["pextra"] = bud["pextra"].cpu().numpy()
