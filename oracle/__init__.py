@@ -23,6 +23,10 @@ Parity status of each oracle function (what pins it; see tests/test_oracle_*.py)
                                  temperature budget against the liquid/ice fields, vertical integrals against the
                                  integrated 3D fields, counter closed forms, process regimes, TIMEINT/LBUD23/
                                  accumulation; the rain-only rates are pinned only through those identities
+  condensate FSD (PFSD)          ice FSD pinned (tests/test_fsd.py): structure recovered from two PGAW values
+                                 (detrainment ratio, layer-depth/total-water factor, overcast/broken branch),
+                                 defaults and bounds; liquid FSD (Boutle et al. 2013, IFS parameters): parity
+                                 unpinned
 """
 import ctypes
 import os
