@@ -37,7 +37,7 @@ HIP_FLAGS = [
     "-I", os.path.join(ROOT, "include"),
 ]
 SOURCES = ["cloudsc_kernel.hip", "cloudsc_capi.hip", "synth.hip", "cloudsc_params.cpp", "mlp.hip", "surrogate.hip",
-           "satadj.hip", "field.hip", "supersat.hip"]
+           "satadj.hip", "field.hip", "supersat.hip", "cloudsc2.hip"]
 
 
 def _run(cmd, cwd=None):

@@ -1,0 +1,527 @@
+// CLOUDSC2 for gfx950: the simplified prognostic cloud scheme of the linearised physics
+// (ifs-source/arpifs/phys_ec/cloudsc2.F90:10-709) with its final saturation adjustment (CUADJTQ / CUADJTQS,
+// KCALL = 0).  One column per lane: the precipitation fluxes and the precipitation fraction are carried from level
+// to level in registers, like CLOUDSC's sweep.  C-ABI in include/cloudsc2.h.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/cloudsc2.h"
+#include "capi_util.h"
+#include "cloudsc_device.h"
+
+namespace cloudsc2_gpu {
+
+using cloudsc_capi::fail;
+using cloudsc_gpu::m_div;
+using cloudsc_gpu::m_exp;
+using cloudsc_gpu::m_max;
+using cloudsc_gpu::m_min;
+using cloudsc_gpu::m_pow;
+using cloudsc_gpu::m_sqrt;
+
+// Constants of one launch in the storage precision (cloudsc2.F90:243-253 and the TCST/TTHF/TECLDP/TEPHLI values).
+template <typename R>
+struct C2Const {
+  R rg, rd, rcpd, retv, rtt, rlvtt, rlstt, rlmlt, r2es, r3les, r3ies, r4les, r4ies, r5les, r5ies, r5alvcp, r5alscp;
+  R ralvdcp, ralsdcp, rtwat, rtice, rtwat_rtice_r, rlmin, rpecons, rclcrit, rlptrc, rvtmp2;
+  R ptsphy, zckcodtl, zckcodti, zcons2, zcons3, zmeltp2, zqtmst;
+  int32_t lin;      // LPHYLIN or LDRAIN1D: tanh phase function and CUADJTQS
+  int32_t evap;     // LEVAPLS2 or LDRAIN1D: precipitation evaporation, the 1.9 RCLCRIT / 1e-4 thresholds
+};
+
+template <typename R>
+struct C2Ptrs {
+  const R *ceta, *paphp1, *papp1, *pqm1, *pqs, *ptm1, *pl, *pi, *plude, *plu, *pmfu, *pmfd;
+  R* ptent;
+  const R* pgtent;
+  R* ptenq;
+  const R* pgtenq;
+  R* ptenl;
+  const R* pgtenl;
+  R* pteni;
+  const R* pgteni;
+  R *pclc, *pfplsl, *pfplsn, *pfhpsl, *pfhpsn, *pcovptot;
+};
+
+struct C2Dims {
+  int ngptot, nproma, klev, ngpblks, ktdia, kidia, kfdia;  // 1-based KIDIA/KFDIA bound the columns of each block
+};
+
+// fcttre.func.h statement functions on the mixed phase (FOEALFA, FOEEWM, FOEDEM, FOELDCPM)
+template <typename R>
+__device__ __forceinline__ R foealfa(const C2Const<R>& c, R t) {
+  const R x = (m_max(c.rtice, m_min(c.rtwat, t)) - c.rtice) * c.rtwat_rtice_r;
+  return m_min(R(1), x * x);
+}
+template <typename R>
+__device__ __forceinline__ R foeewm(const C2Const<R>& c, R t, R a) {
+  return c.r2es * (a * m_exp(m_div(c.r3les * (t - c.rtt), t - c.r4les)) +
+                   (R(1) - a) * m_exp(m_div(c.r3ies * (t - c.rtt), t - c.r4ies)));
+}
+
+// CUADJTQ, KCALL = 0 (cuadjtq.F90:256-272), or CUADJTQS, KCALL = 0 (cuadjtqs.F90:123-134, 217-250): two
+// Newton steps of the isobaric saturation adjustment of (pt, pq) at pressure psp.
+template <typename R>
+__device__ __forceinline__ void adjust(const C2Const<R>& c, R psp, R& pt, R& pq) {
+  const R zqp = m_div(R(1), psp);
+  if (!c.lin) {
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const R a = foealfa(c, pt);
+      R zqsat = foeewm(c, pt, a) * zqp;
+      zqsat = m_min(R(0.5), zqsat);
+      const R zcor = m_div(R(1), R(1) - c.retv * zqsat);
+      zqsat = zqsat * zcor;
+      const R dl = pt - c.r4les, di = pt - c.r4ies;
+      const R foedem = a * c.r5alvcp * m_div(R(1), dl * dl) + (R(1) - a) * c.r5alscp * m_div(R(1), di * di);
+      const R zcond1 = m_div(pq - zqsat, R(1) + zqsat * zcor * foedem);
+      pt = pt + (a * c.ralvdcp + (R(1) - a) * c.ralsdcp) * zcond1;
+      pq = pq - zcond1;
+    }
+  } else {
+    const bool liq = pt > c.rtt;  // phase from the entry temperature, kept for both steps
+    const R z3es = liq ? c.r3les : c.r3ies, z4es = liq ? c.r4les : c.r4ies;
+    const R z5alcp = liq ? c.r5alvcp : c.r5alscp, zaldcp = liq ? c.ralvdcp : c.ralsdcp;
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      const R ztarg = pt;
+      R zqsat = zqp * (c.r2es * m_exp(m_div(z3es * (ztarg - c.rtt), ztarg - z4es)));
+      if (zqsat > R(0.5)) zqsat = R(0.5);
+      const R zcor = m_div(R(1), R(1) - c.retv * zqsat);
+      zqsat = zqsat * zcor;
+      const R d = ztarg - z4es;
+      const R z2s = m_div(z5alcp, d * d);
+      const R zcond1 = m_div(pq - zqsat, R(1) + zqsat * zcor * z2s);
+      pt = pt + zaldcp * zcond1;
+      pq = pq - zcond1;
+    }
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(128) void cloudsc2_kernel(const C2Const<R> c, const C2Ptrs<R> p, const C2Dims d) {
+  const int64_t g = (int64_t)blockIdx.x * 128 + threadIdx.x;
+  const int nproma = d.nproma, klev = d.klev;
+  if (g >= (int64_t)d.ngpblks * nproma) return;
+  const int64_t b = g / nproma;
+  const int jl = (int)(g - b * nproma);
+  const int kfdia = (int)std::min<int64_t>(d.kfdia, (int64_t)d.ngptot - b * nproma);
+  if (jl < d.kidia - 1 || jl >= kfdia) return;
+  const int64_t s2 = (int64_t)nproma * klev;
+  const int64_t o2 = b * s2 + jl, oh = b * (s2 + nproma) + jl;  // level 0 of (NPROMA,KLEV) / (NPROMA,KLEV+1)
+  auto at2 = [&](int k) { return o2 + (int64_t)k * nproma; };
+  auto ath = [&](int k) { return oh + (int64_t)k * nproma; };
+  const R ptsphy = c.ptsphy;
+  const R zeps1 = R(1.e-12), zeps2 = R(1.e-10), zqmax = R(0.5);
+
+  // eta of the tropopause (:323-334): the last level with 0.1 < eta < 0.4 and T above the level below
+  R ztrpaus = R(0.1);
+  for (int k = 0; k < klev - 1; ++k) {
+    const R ceta = p.ceta[k];
+    if (ceta > R(0.1) && ceta < R(0.4)) {
+      const R t0 = p.ptm1[at2(k)] + ptsphy * p.pgtent[at2(k)];
+      const R t1 = p.ptm1[at2(k + 1)] + ptsphy * p.pgtent[at2(k + 1)];
+      if (t0 > t1) ztrpaus = ceta;
+    }
+  }
+  // outputs the level loop does not reach (:297-320): PCLC, PCOVPTOT zero, PFPLSL/PFPLSN zero at the top
+  for (int k = 0; k < d.ktdia - 1; ++k) {
+    p.pclc[at2(k)] = R(0);
+    p.pcovptot[at2(k)] = R(0);
+  }
+  p.pfplsl[ath(0)] = R(0);
+  p.pfplsn[ath(0)] = R(0);
+  p.pfhpsl[ath(0)] = -R(0) * c.rlvtt;  // (:694-699), as -PFPLSL*RLVTT
+  p.pfhpsn[ath(0)] = -R(0) * c.rlstt;
+  for (int k = 1; k < d.ktdia; ++k) {  // half levels the loop leaves as they are
+    p.pfhpsl[ath(k)] = -p.pfplsl[ath(k)] * c.rlvtt;
+    p.pfhpsn[ath(k)] = -p.pfplsn[ath(k)] * c.rlstt;
+  }
+  const R paph_surf = p.paphp1[ath(klev)];
+
+  R zrfl = R(0), zsfl = R(0), zcovptot = R(0);
+  for (int k = d.ktdia - 1; k < klev; ++k) {
+    const int64_t i2 = at2(k);
+    const R ceta = p.ceta[k];
+    const R zscalm = R(0.9) * m_pow(m_max(ceta - R(0.2), zeps1), R(0.2));  // (:276)
+    R ztp1 = p.ptm1[i2] + ptsphy * p.pgtent[i2];
+    R zqp1 = p.pqm1[i2] + ptsphy * p.pgtenq[i2];
+    const R zl = p.pl[i2] + ptsphy * p.pgtenl[i2];
+    const R zi = p.pi[i2] + ptsphy * p.pgteni[i2];
+    const R paph0 = p.paphp1[ath(k)], paph1 = p.paphp1[ath(k + 1)];
+    const R zdp = paph1 - paph0;
+    const R zzz = m_div(R(1), c.rcpd + c.rcpd * c.rvtmp2 * zqp1);
+    const R zlfdcp = c.rlmlt * zzz, zlsdcp = c.rlstt * zzz, zlvdcp = c.rlvtt * zzz;
+    const R pqs = p.pqs[i2], papp1 = p.papp1[i2], plude = p.plude[i2];
+
+    // 3.1 (:352-416)
+    R zfwat, zfoeew, zesdp;
+    if (c.lin) {
+      const R zoealfaw = R(0.545) * (tanh(R(0.17) * (ztp1 - c.rlptrc)) + R(1.0));
+      const bool cold = ztp1 < c.rtt;
+      zfwat = cold ? zoealfaw : R(1.0);
+      const R z3es = cold ? c.r3ies : c.r3les, z4es = cold ? c.r4ies : c.r4les;
+      zfoeew = c.r2es * m_exp(m_div(z3es * (ztp1 - c.rtt), ztp1 - z4es));
+      zesdp = m_div(zfoeew, papp1);
+      if (zesdp > zqmax) zesdp = zqmax;
+    } else {
+      zfwat = foealfa(c, ztp1);
+      zfoeew = foeewm(c, ztp1, zfwat);
+      zesdp = m_div(zfoeew, papp1);
+    }
+    const R dl = ztp1 - c.r4les, di = ztp1 - c.r4ies;
+    const R zfac = zfwat * m_div(c.r5les, dl * dl) + (R(1.0) - zfwat) * m_div(c.r5ies, di * di);
+    const R zcor = m_div(R(1.0), R(1.0) - c.retv * zesdp);
+    const R zdqsdtemp = zfac * zcor * pqs;
+    const R zcorqs = R(1.0) + c.zcons3 * zdqsdtemp;
+    const R zqlim = zqp1 > pqs ? pqs : zqp1;
+    // critical relative humidity (:386-407)
+    const R zeta3 = ztrpaus;
+    const R sr = m_div(zeta3 - R(0.25), R(0.15));
+    const R zrh2 = R(0.35) + R(0.14) * (sr * sr) + m_div(R(0.04) * m_min(zeta3 - R(0.25), R(0.0)), R(0.15));
+    const R zdeta2 = R(0.3);
+    const R zdeta1 = R(0.09) + m_div(R(0.16) * (R(0.4) - zeta3), R(0.3));
+    R zcrh2 = R(0);
+    if (ceta < zeta3) {
+      zcrh2 = R(1.0);
+    } else if (ceta >= zeta3 && ceta < (zeta3 + zdeta2)) {
+      zcrh2 = R(1.0) + (zrh2 - R(1.0)) * m_div(ceta - zeta3, zdeta2);
+    } else if (ceta >= (zeta3 + zdeta2) && ceta < (R(1.0) - zdeta1)) {
+      zcrh2 = zrh2;
+    } else if (ceta >= (R(1.0) - zdeta1)) {
+      zcrh2 = R(1.0) + (zrh2 - R(1.0)) * m_sqrt(m_div(R(1.0) - ceta, zdeta1));
+    }
+    const R zsupsat = ztp1 < c.rtice ? R(1.8) - R(3.E-03) * ztp1 : R(1.0);
+    const R zqsat = pqs * zsupsat;
+    const R zqcrit = zcrh2 * zqsat;
+
+    // uniform total-water distribution (:419-436)
+    R pclc, zqc;
+    const R zqt = zqp1 + zl + zi;
+    if (zqt <= zqcrit) {
+      pclc = R(0.0);
+      zqc = R(0.0);
+    } else if (zqt >= zqsat) {
+      pclc = R(1.0);
+      zqc = (R(1.0) - zscalm) * (zqsat - zqcrit);
+    } else {
+      const R zqpd = zqsat - zqt, zqcd = zqsat - zqcrit;
+      pclc = R(1.0) - m_sqrt(m_div(zqpd, zqcd - zscalm * (zqt - zqcrit)));
+      zqc = (zscalm * zqpd + (R(1.0) - zscalm) * zqcd) * (pclc * pclc);
+    }
+    // convective component (:440-453)
+    const R zgdp = m_div(c.rg, paph1 - paph0);
+    const R zlude = plude * ptsphy * zgdp;
+    if (k < klev - 1) {
+      const R plu_n = p.plu[at2(k + 1)];
+      if (zlude >= c.rlmin && plu_n >= zeps2) {
+        pclc = pclc + (R(1.0) - pclc) * (R(1.0) - m_exp(-m_div(zlude, plu_n)));
+        zqc = zqc + zlude;
+      }
+    }
+    // compensating subsidence (:457-471)
+    {
+      const R zfac1 = m_div(R(1.0), c.rd * ztp1);
+      const R zrho = papp1 * zfac1;
+      const R zfac2 = m_div(R(1.0), papp1 - c.retv * zfoeew);
+      const R zrodqsdp = -zrho * pqs * zfac2;
+      const R zldcp = zfwat * zlvdcp + (R(1.0) - zfwat) * zlsdcp;
+      const R zfac3 = m_div(R(1.0), R(1.0) + zldcp * zdqsdtemp);
+      const R dtdzmo = c.rg * (m_div(R(1.0), c.rcpd) - zldcp * zrodqsdp) * zfac3;
+      const R zdqsdz = zdqsdtemp * dtdzmo - c.rg * zrodqsdp;
+      const R zfac4 = m_div(R(1.0), zrho);
+      const R zdqc = m_min(zdqsdz * (p.pmfu[i2] + p.pmfd[i2]) * ptsphy * zfac4, zqc);
+      zqc = zqc - zdqc;
+    }
+    // new cloud contents and condensation (:475-481)
+    R zqlwc = zqc * zfwat, zqiwc = zqc * (R(1.0) - zfwat);
+    R zcondl = (zqlwc - zl) * c.zqtmst, zcondi = (zqiwc - zi) * c.zqtmst;
+    // maximum overlap of precipitation (:487-493)
+    if (pclc > zcovptot) zcovptot = pclc;
+    const R zcovpclr = m_max(zcovptot - pclc, R(0.0));
+    // melting of incoming snow (:499-511)
+    R zrfln, zsfln;
+    if (zsfl != R(0.0)) {
+      const R zcons = m_div(c.zcons2 * zdp, zlfdcp);
+      const R zsnmlt = m_min(zsfl, zcons * m_max(R(0.0), ztp1 - c.zmeltp2));
+      zrfln = zrfl + zsnmlt;
+      zsfln = zsfl - zsnmlt;
+      ztp1 = ztp1 - m_div(zsnmlt, zcons);
+    } else {
+      zrfln = zrfl;
+      zsfln = zsfl;
+    }
+    // rain and snow production (:513-545)
+    R zprr = R(0), zprs = R(0);
+    if (pclc > zeps2) {
+      const R zlcrit = c.evap ? R(1.9) * c.rclcrit : c.rclcrit * R(2.);
+      const R zcldl = m_div(zqlwc, pclc);
+      const R r = m_div(zcldl, zlcrit);
+      const R zd = c.zckcodtl * (R(1.0) - m_exp(-(r * r)));
+      const R zlnew = pclc * zcldl * m_exp(-zd);
+      zprr = zqlwc - zlnew;
+      zqlwc = zqlwc - zprr;
+      const R zlcriti = c.evap ? R(1.E-04) : c.rclcrit * R(2.);
+      const R zcldi = m_div(zqiwc, pclc);
+      const R ri = m_div(zcldi, zlcriti);
+      const R zdi = c.zckcodti * m_exp(R(0.025) * (ztp1 - c.rtt)) * (R(1.0) - m_exp(-(ri * ri)));
+      const R zinew = pclc * zcldi * m_exp(-zdi);
+      zprs = zqiwc - zinew;
+      zqiwc = zqiwc - zprs;
+    }
+    const R zdr = c.zcons2 * zdp * (zprr + zprs);
+    R zrfreeze = R(0), zfwatr;
+    if (ztp1 < c.rtt) {
+      zrfreeze = c.zcons2 * zdp * zprr;
+      zfwatr = R(0.0);
+    } else {
+      zfwatr = R(1.0);
+    }
+    zrfln = zrfln + zfwatr * zdr;
+    zsfln = zsfln + (R(1.0) - zfwatr) * zdr;
+    // precipitation evaporation (:568-597)
+    R zevapr = R(0), zevaps = R(0), pcovptot = R(0);
+    const R zprtot = zrfln + zsfln;
+    if (zprtot > zeps2 && zcovpclr > zeps2 && c.evap) {
+      R zpreclr = m_div(zprtot * zcovpclr, zcovptot);
+      const R om = R(1.0) - pclc;
+      const R zqe = pqs - m_div((pqs - zqlim) * zcovpclr, om * om);
+      const R zbeta = c.rg * c.rpecons *
+                      m_pow(m_div(m_div(m_sqrt(m_div(papp1, paph_surf)), R(5.09E-3)) * zpreclr, zcovpclr), R(0.5777));
+      const R zb = m_div(ptsphy * zbeta * (pqs - zqe), R(1.0) + zbeta * ptsphy * zcorqs);
+      const R zdtgdp = m_div(ptsphy * c.rg, paph1 - paph0);
+      R zdpr = m_div(zcovpclr * zb, zdtgdp);
+      zdpr = m_min(zdpr, zpreclr);
+      zpreclr = zpreclr - zdpr;
+      if (zpreclr <= R(0.0)) zcovptot = pclc;
+      pcovptot = zcovptot;
+      zevapr = m_div(zdpr * zrfln, zprtot);
+      zrfln = zrfln - zevapr;
+      zevaps = m_div(zdpr * zsfln, zprtot);
+      zsfln = zsfln - zevaps;
+    }
+    // tendencies, first guess and the final adjustment (:608-665)
+    auto tendency_t = [&](R condl, R condi, R rfreeze) {
+      return zlvdcp * condl + zlsdcp * condi -
+             (zlvdcp * zevapr + zlsdcp * zevaps + plude * (zfwat * zlvdcp + (R(1.0) - zfwat) * zlsdcp) -
+              (zlsdcp - zlvdcp) * rfreeze) *
+                 zgdp;
+    };
+    {
+      const R zdqdt = -(zcondl + zcondi) + (plude + zevapr + zevaps) * zgdp;
+      const R zdtdt = tendency_t(zcondl, zcondi, zrfreeze);
+      ztp1 = ztp1 + ptsphy * zdtdt;
+      zqp1 = zqp1 + ptsphy * zdqdt;
+    }
+    const R zqold = zqp1;
+    adjust(c, papp1, ztp1, zqp1);
+    const R zdq = m_max(R(0.0), zqold - zqp1);
+    const R zdr2 = c.zcons2 * zdp * zdq;
+    R zrfreeze2;
+    if (ztp1 < c.rtt) {
+      zrfreeze2 = zfwat * zdr2;
+      zfwatr = R(0.0);
+    } else {
+      zrfreeze2 = R(0.0);
+      zfwatr = R(1.0);
+    }
+    zcondl = zcondl + zfwatr * zdq * c.zqtmst;
+    zcondi = zcondi + (R(1.0) - zfwatr) * zdq * c.zqtmst;
+    zrfln = zrfln + zfwatr * zdr2;
+    zsfln = zsfln + (R(1.0) - zfwatr) * zdr2;
+    zrfreeze = zrfreeze + zrfreeze2;
+
+    p.ptenq[i2] = -(zcondl + zcondi) + (plude + zevapr + zevaps) * zgdp;
+    p.ptent[i2] = tendency_t(zcondl, zcondi, zrfreeze);
+    p.ptenl[i2] = (zqlwc - zl) * c.zqtmst;
+    p.pteni[i2] = (zqiwc - zi) * c.zqtmst;
+    p.pclc[i2] = pclc;
+    p.pcovptot[i2] = pcovptot;
+    const int64_t ih = ath(k + 1);
+    p.pfplsl[ih] = zrfln;
+    p.pfplsn[ih] = zsfln;
+    p.pfhpsl[ih] = -zrfln * c.rlvtt;
+    p.pfhpsn[ih] = -zsfln * c.rlstt;
+    zrfl = zrfln;
+    zsfl = zsfln;
+  }
+}
+
+template <typename R>
+C2Const<R> make_const(const cloudsc_params_t& q, const cloudsc2_params_t& q2, double ptsphy, bool ldrain1d) {
+  C2Const<R> c{};
+  c.rg = R(q.rg); c.rd = R(q.rd); c.rcpd = R(q.rcpd); c.retv = R(q.retv); c.rtt = R(q.rtt); c.rlvtt = R(q.rlvtt);
+  c.rlstt = R(q.rlstt); c.rlmlt = R(q.rlmlt); c.r2es = R(q.r2es); c.r3les = R(q.r3les); c.r3ies = R(q.r3ies);
+  c.r4les = R(q.r4les); c.r4ies = R(q.r4ies); c.r5les = R(q.r5les); c.r5ies = R(q.r5ies);
+  c.r5alvcp = R(q.r5alvcp); c.r5alscp = R(q.r5alscp); c.ralvdcp = R(q.ralvdcp); c.ralsdcp = R(q.ralsdcp);
+  c.rtwat = R(q.rtwat); c.rtice = R(q.rtice); c.rtwat_rtice_r = R(q.rtwat_rtice_r); c.rlmin = R(q.rlmin);
+  c.rpecons = R(q.rpecons); c.rclcrit = R(q2.rclcrit); c.rlptrc = R(q2.rlptrc); c.rvtmp2 = R(q2.rvtmp2);
+  // (:243-249), evaluated in R as the reference does in JPRB
+  const R ts = R(ptsphy), rkconv = R(q.rkconv);
+  c.ptsphy = ts;
+  c.zckcodtl = R(2.0) * rkconv * ts;
+  c.zckcodti = R(5.0) * rkconv * ts;
+  c.zcons2 = R(1) / (ts * c.rg);
+  c.zcons3 = c.rlvtt / c.rcpd;
+  c.zmeltp2 = c.rtt + R(2.0);
+  c.zqtmst = R(1) / ts;
+  c.lin = (q2.lphylin || ldrain1d) ? 1 : 0;
+  c.evap = (q2.levapls2 || ldrain1d) ? 1 : 0;
+  return c;
+}
+
+int check(const cloudsc_params_t* q, const cloudsc2_params_t* q2, int ktdia, int klev) {
+  if (!q || !q2) return fail(CLOUDSC_ENULL, "cloudsc2: NULL parameter block");
+  if (klev < 2 || ktdia < 1 || ktdia > klev) return fail(CLOUDSC_EDIMS, "cloudsc2: need KLEV >= 2, 1 <= KTDIA <= KLEV");
+  return CLOUDSC_OK;
+}
+
+template <typename R>
+hipError_t launch(const cloudsc_params_t& q, const cloudsc2_params_t& q2, bool ldrain1d, double ptsphy,
+                  const C2Ptrs<R>& p, const C2Dims& d, hipStream_t stream) {
+  const C2Const<R> c = make_const<R>(q, q2, ptsphy, ldrain1d);
+  const int64_t ncols = (int64_t)d.ngpblks * d.nproma;
+  hipLaunchKernelGGL(cloudsc2_kernel<R>, dim3((unsigned)((ncols + 127) / 128)), dim3(128), 0, stream, c, p, d);
+  return hipGetLastError();
+}
+
+// Per-block drop-in: a per-thread device workspace and stream, as the other per-block entries.
+struct Workspace {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  void* dev = nullptr;
+  size_t bytes = 0;
+};
+thread_local Workspace g_ws;
+
+template <typename R>
+int per_block(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* ceta, int kidia, int kfdia, int klon,
+              int ktdia, int klev, int ldrain1d, R ptsphy, const R* paphp1, const R* papp1, const R* pqm1,
+              const R* pqs, const R* ptm1, const R* pl, const R* pi, const R* plude, const R* plu, const R* pmfu,
+              const R* pmfd, R* ptent, const R* pgtent, R* ptenq, const R* pgtenq, R* ptenl, const R* pgtenl,
+              R* pteni, const R* pgteni, R* pclc, R* pfplsl, R* pfplsn, R* pfhpsl, R* pfhpsn, R* pcovptot) {
+  int rc = check(q, q2, ktdia, klev);
+  if (rc) return rc;
+  if (klon < 1 || kidia < 1 || kfdia < kidia || kfdia > klon)
+    return fail(CLOUDSC_EDIMS, "cloudsc2: need 1 <= KIDIA <= KFDIA <= KLON");
+  const void* req[] = {ceta,  paphp1, papp1, pqm1,  pqs,    ptm1,   pl,    pi,     plude, plu,   pmfu,  pmfd,
+                       ptent, pgtent, ptenq, pgtenq, ptenl, pgtenl, pteni, pgteni, pclc,  pfplsl, pfplsn, pfhpsl,
+                       pfhpsn, pcovptot};
+  for (const void* x : req)
+    if (!x) return fail(CLOUDSC_ENULL, "cloudsc2: NULL array");
+  rc = cloudsc_capi::check_device();
+  if (rc) return rc;
+  const size_t n2 = (size_t)klon * klev, nh = (size_t)klon * (klev + 1);
+  struct Arr {
+    const void* host;
+    void* out;
+    size_t n;
+  };
+  // outputs are copied in too, so that what CLOUDSC2 leaves untouched comes back unchanged
+  Arr arrs[] = {{ceta, nullptr, (size_t)klev}, {paphp1, nullptr, nh}, {papp1, nullptr, n2}, {pqm1, nullptr, n2},
+                {pqs, nullptr, n2},   {ptm1, nullptr, n2},  {pl, nullptr, n2},     {pi, nullptr, n2},
+                {plude, nullptr, n2}, {plu, nullptr, n2},   {pmfu, nullptr, n2},   {pmfd, nullptr, n2},
+                {ptent, ptent, n2},   {pgtent, nullptr, n2}, {ptenq, ptenq, n2},   {pgtenq, nullptr, n2},
+                {ptenl, ptenl, n2},   {pgtenl, nullptr, n2}, {pteni, pteni, n2},   {pgteni, nullptr, n2},
+                {pclc, pclc, n2},     {pfplsl, pfplsl, nh},  {pfplsn, pfplsn, nh}, {pfhpsl, pfhpsl, nh},
+                {pfhpsn, pfhpsn, nh}, {pcovptot, pcovptot, n2}};
+  const int narr = sizeof(arrs) / sizeof(arrs[0]);
+  size_t off[sizeof(arrs) / sizeof(arrs[0])], total = 0;
+  for (int i = 0; i < narr; ++i) {
+    off[i] = total;
+    total += (arrs[i].n * sizeof(R) + 255) & ~(size_t)255;
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: hipGetDevice", e);
+  Workspace& w = g_ws;
+  if (w.device != dev) {
+    w = Workspace{};
+    w.device = dev;
+    e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: hipStreamCreate", e);
+  }
+  if (w.bytes < total) {
+    if (w.dev) (void)hipFree(w.dev);
+    w.dev = nullptr;
+    w.bytes = 0;
+    e = hipMalloc(&w.dev, total);
+    if (e != hipSuccess) return fail(CLOUDSC_ENOMEM, "cloudsc2: hipMalloc workspace", e);
+    w.bytes = total;
+  }
+  char* base = (char*)w.dev;
+  for (int i = 0; i < narr && e == hipSuccess; ++i)
+    e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * sizeof(R), hipMemcpyHostToDevice, w.stream);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: copy to device", e);
+  auto D = [&](int i) { return (R*)(base + off[i]); };
+  C2Ptrs<R> p{D(0),  D(1),  D(2),  D(3),  D(4),  D(5),  D(6),  D(7),  D(8),  D(9),  D(10), D(11), D(12),
+              D(13), D(14), D(15), D(16), D(17), D(18), D(19), D(20), D(21), D(22), D(23), D(24), D(25)};
+  C2Dims d{kfdia, klon, klev, 1, ktdia, kidia, kfdia};
+  e = launch<R>(*q, *q2, ldrain1d != 0, (double)ptsphy, p, d, w.stream);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: kernel launch", e);
+  for (int i = 0; i < narr && e == hipSuccess; ++i)
+    if (arrs[i].out)
+      e = hipMemcpyAsync(arrs[i].out, base + off[i], arrs[i].n * sizeof(R), hipMemcpyDeviceToHost, w.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: copy to host", e);
+  return CLOUDSC_OK;
+}
+
+template <typename R>
+int blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* ceta, int ngptot, int nproma, int ktdia,
+            int klev, int ngpblks, int ldrain1d, R ptsphy, const C2Ptrs<R>& p, void* stream) {
+  int rc = check(q, q2, ktdia, klev);
+  if (rc) return rc;
+  if (nproma < 1 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma)
+    return fail(CLOUDSC_EDIMS, "cloudsc2_blocked: need NGPBLKS = ceil(NGPTOT/NPROMA)");
+  const void* req[] = {p.ceta,  p.paphp1, p.papp1, p.pqm1,   p.pqs,   p.ptm1,   p.pl,    p.pi,     p.plude,
+                       p.plu,   p.pmfu,   p.pmfd,  p.ptent,  p.pgtent, p.ptenq, p.pgtenq, p.ptenl, p.pgtenl,
+                       p.pteni, p.pgteni, p.pclc,  p.pfplsl, p.pfplsn, p.pfhpsl, p.pfhpsn, p.pcovptot};
+  for (const void* x : req)
+    if (!x) return fail(CLOUDSC_ENULL, "cloudsc2_blocked: NULL array");
+  rc = cloudsc_capi::check_device();
+  if (rc) return rc;
+  C2Dims d{ngptot, nproma, klev, ngpblks, ktdia, 1, nproma};
+  hipError_t e = launch<R>(*q, *q2, ldrain1d != 0, (double)ptsphy, p, d, (hipStream_t)stream);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2_blocked: kernel launch", e);
+  return CLOUDSC_OK;
+}
+
+}  // namespace cloudsc2_gpu
+
+#define CLOUDSC2_PER_BLOCK(R)                                                                                     \
+  return cloudsc2_gpu::per_block<R>(ydparams, ydparams2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d, ptsphy, \
+                                    paphp1, papp1, pqm1, pqs, ptm1, pl, pi, plude, plu, pmfu, pmfd, ptent, pgtent, \
+                                    ptenq, pgtenq, ptenl, pgtenl, pteni, pgteni, pclc, pfplsl, pfplsn, pfhpsl,     \
+                                    pfhpsn, pcovptot)
+
+#define CLOUDSC2_BLOCKED(R)                                                                                        \
+  const cloudsc2_gpu::C2Ptrs<R> p{ceta,  paphp1, papp1, pqm1,   pqs,   ptm1,   pl,    pi,     plude,  plu,     \
+                                  pmfu,  pmfd,   ptent, pgtent, ptenq, pgtenq, ptenl, pgtenl, pteni,  pgteni,  \
+                                  pclc,  pfplsl, pfplsn, pfhpsl, pfhpsn, pcovptot};                              \
+  return cloudsc2_gpu::blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d,     \
+                                  ptsphy, p, stream)
+
+extern "C" {
+
+int cloudsc2_params_default(cloudsc2_params_t* p2, const cloudsc_params_t* p) {
+  if (!p2 || !p) return cloudsc_capi::fail(CLOUDSC_ENULL, "cloudsc2_params_default: NULL");
+  p2->rclcrit = 4.0E-4;                                    // sucldp.F90:264
+  p2->rlptrc = p->rtice + (p->rtwat - p->rtice) / std::sqrt(2.0);  // suphli.F90:125
+  p2->rlpal1 = 0.15;                                        // suphli.F90:126
+  p2->rlpal2 = 20.0;                                        // suphli.F90:127
+  p2->rvtmp2 = 0.0;                                         // suphec.F90:229
+  p2->lphylin = 0;                                          // suphli.F90:107
+  p2->levapls2 = 0;                                         // su0phy.F90:1361
+  return CLOUDSC_OK;
+}
+
+int cloudsc2_dp_c(CLOUDSC2_ARGS(double)) { CLOUDSC2_PER_BLOCK(double); }
+int cloudsc2_sp_c(CLOUDSC2_ARGS(float)) { CLOUDSC2_PER_BLOCK(float); }
+int cloudsc2_blocked_dp_c(CLOUDSC2_BLOCKED_ARGS(double)) { CLOUDSC2_BLOCKED(double); }
+int cloudsc2_blocked_sp_c(CLOUDSC2_BLOCKED_ARGS(float)) { CLOUDSC2_BLOCKED(float); }
+
+}  // extern "C"

@@ -27,6 +27,11 @@ Parity status of each oracle function (what pins it; see tests/test_oracle_*.py)
                                  (detrainment ratio, layer-depth/total-water factor, overcast/broken branch),
                                  defaults and bounds; liquid FSD (Boutle et al. 2013, IFS parameters): parity
                                  unpinned
+  CLOUDSC2 (cloudsc2_oracle.hpp) pinned (tests/test_cloudsc2.py): per-level water closure of the tendencies,
+                                 the precipitation fluxes and the detrainment, the enthalpy fluxes, the dry column,
+                                 no supersaturation left after CUADJTQ/CUADJTQS, cloud cover 0/1 above the
+                                 tropopause and monotone in total water, melting of snow, the write set and KTDIA;
+                                 the autoconversion and evaporation rates only through the water closure
 """
 import ctypes
 import os
@@ -230,4 +235,42 @@ def sltend_supersat_blocked(precision, params, ngptot, nproma, klev, tsphy, arra
     rc = getattr(lib(), f"oracle_sltend_supersat_blocked_{precision}")(*args)
     if rc != 0:
         raise RuntimeError(f"oracle_sltend_supersat_blocked_{precision} returned {rc}")
+    return arrays
+
+
+def cloudsc2_block(precision, params, params2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d, ptsphy, arrays):
+    """One CLOUDSC2 block on the CPU, same argument list as the product's cloudsc2_*_c (Fortran-ordered arrays)."""
+    from openifs_amd.cloudsc2 import block_args
+
+    ceta = np.ascontiguousarray(ceta, dtype=np.float64 if precision == "dp" else np.float32)
+    args = block_args(params, params2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d, ptsphy, arrays, precision)
+    rc = getattr(lib(), f"oracle_cloudsc2_{precision}")(*args)
+    if rc != 0:
+        raise RuntimeError(f"oracle_cloudsc2_{precision} returned {rc}")
+    return arrays
+
+
+def cloudsc2_blocked(precision, params, params2, ceta, ngptot, nproma, ktdia, klev, ldrain1d, ptsphy, arrays,
+                     nthreads=0):
+    """CLOUDSC2 over all blocks of blocked host arrays (C-order (NGPBLKS, KLEV[+1], NPROMA), the argument names of
+    openifs_amd.cloudsc2.CLOUDSC2_ARRAYS); outputs written in place."""
+    from openifs_amd.cloudsc2 import CLOUDSC2_ARRAYS
+
+    ngpblks = (ngptot + nproma - 1) // nproma
+    real = ctypes.c_double if precision == "dp" else ctypes.c_float
+    rdt = np.float64 if precision == "dp" else np.float32
+    ceta = np.ascontiguousarray(ceta, dtype=rdt)
+    args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.ctypes.data), ctypes.c_int(ngptot),
+            ctypes.c_int(nproma), ctypes.c_int(ktdia), ctypes.c_int(klev), ctypes.c_int(ngpblks),
+            ctypes.c_int(int(bool(ldrain1d))), real(ptsphy)]
+    for name, kind, _ in CLOUDSC2_ARRAYS:
+        a = arrays[name]
+        want = (ngpblks, klev + 1 if kind == "h" else klev, nproma)
+        if a.dtype != rdt or not a.flags.c_contiguous or a.shape != want:
+            raise TypeError(f"{name}: need a C-contiguous {np.dtype(rdt).name} array of shape {want}")
+        args.append(ctypes.c_void_p(a.ctypes.data))
+    args.append(ctypes.c_int(nthreads))
+    rc = getattr(lib(), f"oracle_cloudsc2_blocked_{precision}")(*args)
+    if rc != 0:
+        raise RuntimeError(f"oracle_cloudsc2_blocked_{precision} returned {rc}")
     return arrays

@@ -12,12 +12,14 @@
 //       include/cloudsc_satadj.h
 //   oracle_satadj_blocked_{dp,sp,mixed}  all blocks, OpenMP over blocks
 //   oracle_sltend_supersat_blocked_{dp,sp,mixed}  SLTEND's supersaturation adjustment, all blocks
+//   oracle_cloudsc2_{dp,sp}, oracle_cloudsc2_blocked_{dp,sp}  CLOUDSC2 (cloudsc2_oracle.hpp), one block / all blocks
 //   oracle_supersatcheck_*, oracle_supersatcheck_k_dp (with KFTLIQICE), oracle_lu5_dp,
 //   oracle_truncate_sinks_dp, oracle_thermo_dp, oracle_qsatmixadj_dp  unit-level entry points for the tests
 #include <omp.h>
 
 #include "../include/cloudsc_satadj.h"
 #include "cloudsc_oracle.hpp"
+#include "cloudsc2_oracle.hpp"
 #include "satadj_oracle.hpp"
 #include "sltend_oracle.hpp"
 
@@ -324,9 +326,77 @@ int sltend_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, 
   return sltend_blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, tsphy, pt, pq, pa, prsf1, prs1, pcent, \
                                pcenq, pcena, ptent, ptenq, ptena, ptencld, padja, padjl, padji, nthreads)
 
+// ---- CLOUDSC2 ----
+template <typename R>
+int cloudsc2_check(const cloudsc_params_t* p, const cloudsc2_params_t* p2, int kidia, int kfdia, int klon, int ktdia,
+                   int klev) {
+  if (!p || !p2) return CLOUDSC_ENULL;
+  if (klon < 1 || klev < 2 || kidia < 1 || kfdia < kidia || kfdia > klon || ktdia < 1 || ktdia > klev)
+    return CLOUDSC_EDIMS;
+  return CLOUDSC_OK;
+}
+
+#define ORACLE_CLOUDSC2_CALL(R)                                                                                   \
+  int rc = cloudsc2_check<R>(ydparams, ydparams2, kidia, kfdia, klon, ktdia, klev);                               \
+  if (rc) return rc;                                                                                              \
+  oracle::Cloudsc2Args<R> a{kidia,  kfdia, klon,  ktdia, klev,   ldrain1d != 0, ptsphy, ceta,   paphp1, papp1,   \
+                            pqm1,   pqs,   ptm1,  pl,    pi,     plude,         plu,    pmfu,   pmfd,   ptent,   \
+                            pgtent, ptenq, pgtenq, ptenl, pgtenl, pteni,        pgteni, pclc,   pfplsl, pfplsn,  \
+                            pfhpsl, pfhpsn, pcovptot};                                                             \
+  oracle::cloudsc2<R>(*ydparams, *ydparams2, a);                                                                  \
+  return CLOUDSC_OK;
+
+// All blocks of blocked host arrays (NGPBLKS outermost), OpenMP over blocks; the CLOUDSC2_BLOCKED_ARGS order
+// without the stream, then the thread count.
+template <typename R>
+int cloudsc2_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2, const R* ceta, int ngptot, int nproma,
+                     int ktdia, int klev, int ngpblks, int ldrain1d, R ptsphy, const R* paphp1, const R* papp1,
+                     const R* pqm1, const R* pqs, const R* ptm1, const R* pl, const R* pi, const R* plude,
+                     const R* plu, const R* pmfu, const R* pmfd, R* ptent, const R* pgtent, R* ptenq,
+                     const R* pgtenq, R* ptenl, const R* pgtenl, R* pteni, const R* pgteni, R* pclc, R* pfplsl,
+                     R* pfplsn, R* pfhpsl, R* pfhpsn, R* pcovptot, int nthreads) {
+  int rc = cloudsc2_check<R>(p, p2, 1, nproma, nproma, ktdia, klev);
+  if (rc) return rc;
+  if (ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma) return CLOUDSC_EDIMS;
+  const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int ibl = 0; ibl < ngpblks; ++ibl) {
+    const int kfdia = std::min(nproma, ngptot - ibl * nproma);
+    const size_t o2 = ibl * s2, oh = ibl * sh;
+    oracle::Cloudsc2Args<R> a{1,           kfdia,       nproma,      ktdia,       klev,         ldrain1d != 0,
+                              ptsphy,      ceta,        paphp1 + oh, papp1 + o2,  pqm1 + o2,    pqs + o2,
+                              ptm1 + o2,   pl + o2,     pi + o2,     plude + o2,  plu + o2,     pmfu + o2,
+                              pmfd + o2,   ptent + o2,  pgtent + o2, ptenq + o2,  pgtenq + o2,  ptenl + o2,
+                              pgtenl + o2, pteni + o2,  pgteni + o2, pclc + o2,   pfplsl + oh,  pfplsn + oh,
+                              pfhpsl + oh, pfhpsn + oh, pcovptot + o2};
+    oracle::cloudsc2<R>(*p, *p2, a);
+  }
+  return CLOUDSC_OK;
+}
+
+#define ORACLE_CLOUDSC2_BLOCKED_ARGS(R)                                                                 \
+  const cloudsc_params_t *ydparams, const cloudsc2_params_t *ydparams2, const R *ceta, int ngptot,       \
+      int nproma, int ktdia, int klev, int ngpblks, int ldrain1d, R ptsphy, const R *paphp1,            \
+      const R *papp1, const R *pqm1, const R *pqs, const R *ptm1, const R *pl, const R *pi,               \
+      const R *plude, const R *plu, const R *pmfu, const R *pmfd, R *ptent, const R *pgtent, R *ptenq,   \
+      const R *pgtenq, R *ptenl, const R *pgtenl, R *pteni, const R *pgteni, R *pclc, R *pfplsl,         \
+      R *pfplsn, R *pfhpsl, R *pfhpsn, R *pcovptot, int nthreads
+
+#define ORACLE_CLOUDSC2_BLOCKED_CALL(R)                                                                        \
+  return cloudsc2_blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d, ptsphy, \
+                             paphp1, papp1, pqm1, pqs, ptm1, pl, pi, plude, plu, pmfu, pmfd, ptent, pgtent,     \
+                             ptenq, pgtenq, ptenl, pgtenl, pteni, pgteni, pclc, pfplsl, pfplsn, pfhpsl, pfhpsn, \
+                             pcovptot, nthreads)
+
 }  // namespace
 
 extern "C" {
+
+int oracle_cloudsc2_dp(CLOUDSC2_ARGS(double)) { ORACLE_CLOUDSC2_CALL(double) }
+int oracle_cloudsc2_sp(CLOUDSC2_ARGS(float)) { ORACLE_CLOUDSC2_CALL(float) }
+int oracle_cloudsc2_blocked_dp(ORACLE_CLOUDSC2_BLOCKED_ARGS(double)) { ORACLE_CLOUDSC2_BLOCKED_CALL(double); }
+int oracle_cloudsc2_blocked_sp(ORACLE_CLOUDSC2_BLOCKED_ARGS(float)) { ORACLE_CLOUDSC2_BLOCKED_CALL(float); }
 
 int oracle_cloudsc_dp(CLOUDSC_ARGS(double)) { ORACLE_BLOCK_CALL(double, double); }
 int oracle_cloudsc_sp(CLOUDSC_ARGS(float)) { ORACLE_BLOCK_CALL(float, float); }
