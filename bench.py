@@ -378,6 +378,76 @@ def budget_bench(params, nproma, steps):
     return out
 
 
+def cloudsc2_bytes_per_column(klev, word=4):
+    """Algorithmic HBM bytes per column of CLOUDSC2: PAPHP1 and four flux outputs on KLEV+1 half levels, 14 inputs
+    and 6 outputs on KLEV levels, each read or written once."""
+    return word * (20 * klev + 5 * (klev + 1))
+
+
+def cloudsc2_bench(params, nproma, steps, cpu_threads, cpu_seconds):
+    """CLOUDSC2 (the simplified scheme of the linearised physics) in FP32 on 655,360 x L137: kernel time from HIP
+    events on the launch stream, algorithmic bytes, accuracy of the first 2048 columns against the FP64 oracle, and
+    the FP32 oracle on the host cores over those columns."""
+    import numpy as np
+    import torch
+
+    import oracle
+    from openifs_amd import cloudsc2 as C2
+
+    q2 = C2.default_params2(params)
+    f = C2.synth_device(params, 20261206, NGPTOT, nproma, KLEV, "sp", "cuda:0")
+    ceta = torch.from_numpy(C2.reference_eta(KLEV)).to("cuda:0", torch.float32)
+    stream = torch.cuda.current_stream()
+    run = lambda: C2.cloudsc2_blocked(f, params, q2, ceta, NGPTOT, nproma, KLEV, 900.0, "sp",
+                                      stream=stream.cuda_stream)
+    for _ in range(2):
+        run()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        run()
+        b.record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    nb = 2048 // nproma
+    inp = {n: f[n][:nb].cpu().numpy().copy() for n in C2.INPUT_NAMES}
+    got = {n: f[n][:nb].double().cpu().numpy() for n in C2.OUTPUT_NAMES}
+    del f
+    torch.cuda.empty_cache()
+    ref = {n: v.astype(np.float64) for n, v in inp.items()}
+    for n in C2.OUTPUT_NAMES:
+        ref[n] = np.full(got[n].shape, np.nan)
+    oracle.cloudsc2_blocked("dp", params, q2, C2.reference_eta(KLEV), nb * nproma, nproma, 1, KLEV, False, 900.0, ref,
+                            nthreads=cpu_threads)
+    err = {}
+    for n in C2.OUTPUT_NAMES:
+        a, r = got[n], ref[n]
+        err[n] = float(np.sqrt(np.sum((a - r) ** 2) / max(np.sum(r ** 2), 1e-300)))
+    bpc = cloudsc2_bytes_per_column(KLEV)
+    gbs = bpc * NGPTOT / (ms / 1e3) / 1e9
+    res = {"kernel_ms": ms, "columns_per_s": NGPTOT / (ms / 1e3), "algorithmic_bytes_per_column": bpc,
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+           "err_vs_fp64": err,
+           "workload": f"CLOUDSC2 FP32 (CUADJTQ final adjustment), NLEV={KLEV}, {NGPTOT} columns, NPROMA={nproma}, "
+                       f"dt=900 s"}
+    if cpu_seconds > 0:
+        arr = dict(inp)
+        for n in C2.OUTPUT_NAMES:
+            arr[n] = np.zeros(got[n].shape, dtype=np.float32)
+        passes, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            oracle.cloudsc2_blocked("sp", params, q2, C2.reference_eta(KLEV), nb * nproma, nproma, 1, KLEV, False,
+                                    900.0, arr, nthreads=cpu_threads)
+            passes += 1
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": passes * nb * nproma / dt, "unit": "columns/s", "cores": cpu_threads,
+                               "kind": "port", "sample": f"{nb * nproma} columns x {passes} passes ({dt:.1f} s), "
+                                                         "FP32 oracle, OpenMP over NPROMA blocks"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -391,6 +461,7 @@ def main():
     ap.add_argument("--no-hybrid", action="store_true")
     ap.add_argument("--no-satadj", action="store_true")
     ap.add_argument("--no-budget", action="store_true")
+    ap.add_argument("--no-cloudsc2", action="store_true")
     ap.add_argument("--train-cols", type=int, default=20_000_000)
     ap.add_argument("--hybrid-cols", type=int, default=6_599_680)
     ap.add_argument("--rerun-fraction", type=float, default=0.01)
@@ -485,6 +556,11 @@ def main():
     if not args.no_budget:
         budget = budget_bench(params, args.nproma, max(5, min(args.steps, 10)))
 
+    cloudsc2 = None
+    if not args.no_cloudsc2:
+        cloudsc2 = cloudsc2_bench(params, args.nproma, max(5, min(args.steps, 20)),
+                                  args.cpu_threads or min(16, os.cpu_count() or 1), 0.0 if args.no_cpu else 3.0)
+
     cpu = None
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -518,6 +594,7 @@ def main():
         "hybrid": hybrid,
         "satadj": satadj,
         "budget": budget,
+        "cloudsc2": cloudsc2,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
