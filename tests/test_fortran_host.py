@@ -145,3 +145,45 @@ def test_fortran_block_loop_equals_step_on_fields(tmp_path, params137):
             fld = np.fromfile(os.path.join(d, f"{n}.fld"), dtype=np.float32)
             assert np.array_equal(blk, fld, equal_nan=True), n
             assert np.isfinite(fld).any(), n
+
+
+def run_cloudsc2_driver(tmp_path, arrs, prec, klon, klev, kidia, kfdia, ktdia, ldrain1d, lphylin, levapls2):
+    from openifs_amd import cloudsc2 as C2
+
+    d = str(tmp_path)
+    _write_common(d, klon, klev, kidia, kfdia)
+    np.array([ktdia, int(ldrain1d), lphylin, levapls2], dtype=np.int32).tofile(os.path.join(d, "c2.bin"))
+    C2.reference_eta(klev).astype(np.float64).tofile(os.path.join(d, "ceta.bin"))
+    for n, kind, role in C2.CLOUDSC2_ARRAYS:
+        np.asfortranarray(arrs[n]).ravel(order="F").tofile(os.path.join(d, f"{n}.bin"))
+    subprocess.run([DRIVER, "cloudsc2", d, prec], check=True, timeout=120)
+    return {n: np.fromfile(os.path.join(d, f"{n}.out"), dtype=arrs[n].dtype).reshape(arrs[n].shape, order="F")
+            for n in C2.OUTPUT_NAMES}
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["dp", "sp"])
+def test_fortran_host_cloudsc2(tmp_path, params137, prec):
+    """CLOUDSC2 through CLOUDSC2_*_C from Fortran (CLOUDSC2_PARAMS_T filled by CLOUDSC2_PARAMS_DEFAULT, then
+    LPHYLIN/LEVAPLS2 set) equals the Python host bit for bit; the FP64 variant equals the oracle to 1e-12."""
+    import oracle
+    from openifs_amd import cloudsc2 as C2
+
+    klon, klev, ktdia = 40, 137, 3
+    host = C2.synth_host(params137, 41, klon, klon, klev, prec)
+    arrs = {k: np.asfortranarray(v[0].T) for k, v in host.items()}
+    out = run_cloudsc2_driver(tmp_path, arrs, prec, klon, klev, 2, klon - 1, ktdia, False, 0, 1)
+    q2 = C2.default_params2(params137)
+    q2.levapls2 = 1
+    ceta = C2.reference_eta(klev)
+    ref = {k: v.copy(order="F") for k, v in arrs.items()}
+    C2.CLOUDSC2(params137, q2, ceta, 2, klon - 1, klon, ktdia, klev, False, 900.0, ref, prec)
+    for n in out:
+        np.testing.assert_array_equal(out[n], ref[n], err_msg=n)
+    if prec == "dp":
+        orc = {k: v.copy(order="F") for k, v in arrs.items()}
+        oracle.cloudsc2_block("dp", params137, q2, ceta, 2, klon - 1, klon, ktdia, klev, False, 900.0, orc)
+        for n in out:
+            np.testing.assert_array_equal(np.isnan(out[n]), np.isnan(orc[n]), err_msg=n)
+            assert relrms(out[n], orc[n]) <= 1e-12, n
