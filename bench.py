@@ -428,7 +428,8 @@ def cloudsc2_bench(params, nproma, steps, cpu_threads, cpu_seconds):
     gbs = bpc * NGPTOT / (ms / 1e3) / 1e9
     res = {"kernel_ms": ms, "columns_per_s": NGPTOT / (ms / 1e3), "algorithmic_bytes_per_column": bpc,
            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+                        "frac": gbs / HBM_PEAK_GBS,
+                        "traffic": read_traffic(nproma, "cloudsc2_sp_l137_655360_traffic.md")},
            "err_vs_fp64": err,
            "workload": f"CLOUDSC2 FP32 (CUADJTQ final adjustment), NLEV={KLEV}, {NGPTOT} columns, NPROMA={nproma}, "
                        f"dt=900 s"}

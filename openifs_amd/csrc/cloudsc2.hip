@@ -22,6 +22,9 @@ using cloudsc_gpu::m_min;
 using cloudsc_gpu::m_pow;
 using cloudsc_gpu::m_sqrt;
 
+using cloudsc_gpu::LD;
+using cloudsc_gpu::ST;
+
 // Constants of one launch in the storage precision (cloudsc2.F90:243-253 and the TCST/TTHF/TECLDP/TEPHLI values).
 template <typename R>
 struct C2Const {
@@ -101,19 +104,31 @@ __device__ __forceinline__ void adjust(const C2Const<R>& c, R psp, R& pt, R& pq)
   }
 }
 
-template <typename R>
-__global__ __launch_bounds__(128) void cloudsc2_kernel(const C2Const<R> c, const C2Ptrs<R> p, const C2Dims d) {
-  const int64_t g = (int64_t)blockIdx.x * 128 + threadIdx.x;
-  const int nproma = d.nproma, klev = d.klev;
-  if (g >= (int64_t)d.ngpblks * nproma) return;
-  const int64_t b = g / nproma;
-  const int jl = (int)(g - b * nproma);
-  const int kfdia = (int)std::min<int64_t>(d.kfdia, (int64_t)d.ngptot - b * nproma);
-  if (jl < d.kidia - 1 || jl >= kfdia) return;
-  const int64_t s2 = (int64_t)nproma * klev;
-  const int64_t o2 = b * s2 + jl, oh = b * (s2 + nproma) + jl;  // level 0 of (NPROMA,KLEV) / (NPROMA,KLEV+1)
-  auto at2 = [&](int k) { return o2 + (int64_t)k * nproma; };
-  auto ath = [&](int k) { return oh + (int64_t)k * nproma; };
+// Work-group size: 64, 128 and 256 run within 1 % of each other at 655,360 x L137 in FP32.
+constexpr int CLOUDSC2_BLOCK = 128;
+
+// UNI: NPROMA is a multiple of 64, so every wave lies in one block and the block's offsets are wave-uniform.
+// Arrays are addressed as a scalar base plus a 32-bit byte offset (LD/ST); the launcher keeps every array of one
+// launch below 4 GiB.
+template <typename R, bool UNI>
+__global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudsc2_kernel(const C2Const<R> c, const C2Ptrs<R> p,
+                                                                  const C2Dims d) {
+  const uint32_t g = blockIdx.x * CLOUDSC2_BLOCK + threadIdx.x;
+  const uint32_t nproma = (uint32_t)d.nproma;
+  const int klev = d.klev;
+  if (g >= (uint32_t)d.ngpblks * nproma) return;
+  uint32_t b = g / nproma;
+  if (UNI) b = __builtin_amdgcn_readfirstlane(b);
+  const uint32_t jl = g - b * nproma;
+  const int kfdia = std::min(d.kfdia, d.ngptot - (int)(b * nproma));
+  if ((int)jl < d.kidia - 1 || (int)jl >= kfdia) return;
+  constexpr uint32_t E = sizeof(R);
+  const uint32_t jo = jl * E;                          // the lane's byte offset within a level
+  const uint32_t lstep = nproma * E;                   // one level
+  const uint32_t b2 = b * nproma * (uint32_t)klev * E;  // level 0 of a (NPROMA,KLEV) array
+  const uint32_t bh = b2 + b * lstep;                  // level 0 of a (NPROMA,KLEV+1) array
+  auto at2 = [&](int k) { return b2 + (uint32_t)k * lstep; };
+  auto ath = [&](int k) { return bh + (uint32_t)k * lstep; };
   const R ptsphy = c.ptsphy;
   const R zeps1 = R(1.e-12), zeps2 = R(1.e-10), zqmax = R(0.5);
 
@@ -122,40 +137,61 @@ __global__ __launch_bounds__(128) void cloudsc2_kernel(const C2Const<R> c, const
   for (int k = 0; k < klev - 1; ++k) {
     const R ceta = p.ceta[k];
     if (ceta > R(0.1) && ceta < R(0.4)) {
-      const R t0 = p.ptm1[at2(k)] + ptsphy * p.pgtent[at2(k)];
-      const R t1 = p.ptm1[at2(k + 1)] + ptsphy * p.pgtent[at2(k + 1)];
+      const R t0 = LD(p.ptm1, at2(k), jo) + ptsphy * LD(p.pgtent, at2(k), jo);
+      const R t1 = LD(p.ptm1, at2(k + 1), jo) + ptsphy * LD(p.pgtent, at2(k + 1), jo);
       if (t0 > t1) ztrpaus = ceta;
     }
   }
   // outputs the level loop does not reach (:297-320): PCLC, PCOVPTOT zero, PFPLSL/PFPLSN zero at the top
   for (int k = 0; k < d.ktdia - 1; ++k) {
-    p.pclc[at2(k)] = R(0);
-    p.pcovptot[at2(k)] = R(0);
+    ST(p.pclc, at2(k), jo, R(0));
+    ST(p.pcovptot, at2(k), jo, R(0));
   }
-  p.pfplsl[ath(0)] = R(0);
-  p.pfplsn[ath(0)] = R(0);
-  p.pfhpsl[ath(0)] = -R(0) * c.rlvtt;  // (:694-699), as -PFPLSL*RLVTT
-  p.pfhpsn[ath(0)] = -R(0) * c.rlstt;
+  ST(p.pfplsl, ath(0), jo, R(0));
+  ST(p.pfplsn, ath(0), jo, R(0));
+  ST(p.pfhpsl, ath(0), jo, -R(0) * c.rlvtt);  // (:694-699), as -PFPLSL*RLVTT
+  ST(p.pfhpsn, ath(0), jo, -R(0) * c.rlstt);
   for (int k = 1; k < d.ktdia; ++k) {  // half levels the loop leaves as they are
-    p.pfhpsl[ath(k)] = -p.pfplsl[ath(k)] * c.rlvtt;
-    p.pfhpsn[ath(k)] = -p.pfplsn[ath(k)] * c.rlstt;
+    ST(p.pfhpsl, ath(k), jo, -LD(p.pfplsl, ath(k), jo) * c.rlvtt);
+    ST(p.pfhpsn, ath(k), jo, -LD(p.pfplsn, ath(k), jo) * c.rlstt);
   }
-  const R paph_surf = p.paphp1[ath(klev)];
+  const R paph_surf = LD(p.paphp1, ath(klev), jo);
+
+  // The inputs of one level, loaded one level ahead of the sweep so that the loads of level k+1 are in flight
+  // while level k computes.
+  struct Level {
+    R tm1, gtent, qm1, gtenq, l, gtenl, i, gteni, paph1, qs, app1, lude, lu_next, mfu, mfd;
+  };
+  auto load = [&](int k) {
+    const uint32_t j = at2(k);
+    Level v;
+    v.tm1 = LD(p.ptm1, j, jo); v.gtent = LD(p.pgtent, j, jo); v.qm1 = LD(p.pqm1, j, jo);
+    v.gtenq = LD(p.pgtenq, j, jo); v.l = LD(p.pl, j, jo); v.gtenl = LD(p.pgtenl, j, jo); v.i = LD(p.pi, j, jo);
+    v.gteni = LD(p.pgteni, j, jo); v.paph1 = LD(p.paphp1, ath(k + 1), jo); v.qs = LD(p.pqs, j, jo);
+    v.app1 = LD(p.papp1, j, jo); v.lude = LD(p.plude, j, jo);
+    v.lu_next = k < klev - 1 ? LD(p.plu, j + lstep, jo) : R(0);
+    v.mfu = LD(p.pmfu, j, jo); v.mfd = LD(p.pmfd, j, jo);
+    return v;
+  };
 
   R zrfl = R(0), zsfl = R(0), zcovptot = R(0);
+  R paph0 = LD(p.paphp1, ath(d.ktdia - 1), jo);
+  Level nxt = load(d.ktdia - 1);
   for (int k = d.ktdia - 1; k < klev; ++k) {
-    const int64_t i2 = at2(k);
+    const Level cur = nxt;
+    if (k + 1 < klev) nxt = load(k + 1);
+    const uint32_t i2 = at2(k);
     const R ceta = p.ceta[k];
     const R zscalm = R(0.9) * m_pow(m_max(ceta - R(0.2), zeps1), R(0.2));  // (:276)
-    R ztp1 = p.ptm1[i2] + ptsphy * p.pgtent[i2];
-    R zqp1 = p.pqm1[i2] + ptsphy * p.pgtenq[i2];
-    const R zl = p.pl[i2] + ptsphy * p.pgtenl[i2];
-    const R zi = p.pi[i2] + ptsphy * p.pgteni[i2];
-    const R paph0 = p.paphp1[ath(k)], paph1 = p.paphp1[ath(k + 1)];
+    R ztp1 = cur.tm1 + ptsphy * cur.gtent;
+    R zqp1 = cur.qm1 + ptsphy * cur.gtenq;
+    const R zl = cur.l + ptsphy * cur.gtenl;
+    const R zi = cur.i + ptsphy * cur.gteni;
+    const R paph1 = cur.paph1;
     const R zdp = paph1 - paph0;
     const R zzz = m_div(R(1), c.rcpd + c.rcpd * c.rvtmp2 * zqp1);
     const R zlfdcp = c.rlmlt * zzz, zlsdcp = c.rlstt * zzz, zlvdcp = c.rlvtt * zzz;
-    const R pqs = p.pqs[i2], papp1 = p.papp1[i2], plude = p.plude[i2];
+    const R pqs = cur.qs, papp1 = cur.app1, plude = cur.lude;
 
     // 3.1 (:352-416)
     R zfwat, zfoeew, zesdp;
@@ -216,7 +252,7 @@ __global__ __launch_bounds__(128) void cloudsc2_kernel(const C2Const<R> c, const
     const R zgdp = m_div(c.rg, paph1 - paph0);
     const R zlude = plude * ptsphy * zgdp;
     if (k < klev - 1) {
-      const R plu_n = p.plu[at2(k + 1)];
+      const R plu_n = cur.lu_next;
       if (zlude >= c.rlmin && plu_n >= zeps2) {
         pclc = pclc + (R(1.0) - pclc) * (R(1.0) - m_exp(-m_div(zlude, plu_n)));
         zqc = zqc + zlude;
@@ -233,7 +269,7 @@ __global__ __launch_bounds__(128) void cloudsc2_kernel(const C2Const<R> c, const
       const R dtdzmo = c.rg * (m_div(R(1.0), c.rcpd) - zldcp * zrodqsdp) * zfac3;
       const R zdqsdz = zdqsdtemp * dtdzmo - c.rg * zrodqsdp;
       const R zfac4 = m_div(R(1.0), zrho);
-      const R zdqc = m_min(zdqsdz * (p.pmfu[i2] + p.pmfd[i2]) * ptsphy * zfac4, zqc);
+      const R zdqc = m_min(zdqsdz * (cur.mfu + cur.mfd) * ptsphy * zfac4, zqc);
       zqc = zqc - zdqc;
     }
     // new cloud contents and condensation (:475-481)
@@ -334,19 +370,20 @@ __global__ __launch_bounds__(128) void cloudsc2_kernel(const C2Const<R> c, const
     zsfln = zsfln + (R(1.0) - zfwatr) * zdr2;
     zrfreeze = zrfreeze + zrfreeze2;
 
-    p.ptenq[i2] = -(zcondl + zcondi) + (plude + zevapr + zevaps) * zgdp;
-    p.ptent[i2] = tendency_t(zcondl, zcondi, zrfreeze);
-    p.ptenl[i2] = (zqlwc - zl) * c.zqtmst;
-    p.pteni[i2] = (zqiwc - zi) * c.zqtmst;
-    p.pclc[i2] = pclc;
-    p.pcovptot[i2] = pcovptot;
-    const int64_t ih = ath(k + 1);
-    p.pfplsl[ih] = zrfln;
-    p.pfplsn[ih] = zsfln;
-    p.pfhpsl[ih] = -zrfln * c.rlvtt;
-    p.pfhpsn[ih] = -zsfln * c.rlstt;
+    ST(p.ptenq, i2, jo, -(zcondl + zcondi) + (plude + zevapr + zevaps) * zgdp);
+    ST(p.ptent, i2, jo, tendency_t(zcondl, zcondi, zrfreeze));
+    ST(p.ptenl, i2, jo, (zqlwc - zl) * c.zqtmst);
+    ST(p.pteni, i2, jo, (zqiwc - zi) * c.zqtmst);
+    ST(p.pclc, i2, jo, pclc);
+    ST(p.pcovptot, i2, jo, pcovptot);
+    const uint32_t ih = ath(k + 1);
+    ST(p.pfplsl, ih, jo, zrfln);
+    ST(p.pfplsn, ih, jo, zsfln);
+    ST(p.pfhpsl, ih, jo, -zrfln * c.rlvtt);
+    ST(p.pfhpsn, ih, jo, -zsfln * c.rlstt);
     zrfl = zrfln;
     zsfl = zsfln;
+    paph0 = paph1;
   }
 }
 
@@ -379,13 +416,42 @@ int check(const cloudsc_params_t* q, const cloudsc2_params_t* q2, int ktdia, int
   return CLOUDSC_OK;
 }
 
+// The blocked fields of blocks b0.. of one launch: (NPROMA,KLEV) arrays advance by NPROMA*KLEV words per block,
+// (NPROMA,KLEV+1) arrays by NPROMA*(KLEV+1); CETA is shared.
+template <typename R>
+C2Ptrs<R> offset_ptrs(const C2Ptrs<R>& p, int64_t b0, int nproma, int klev) {
+  const int64_t s2 = b0 * nproma * klev, sh = b0 * nproma * (klev + 1);
+  C2Ptrs<R> o = p;
+  o.paphp1 += sh; o.papp1 += s2; o.pqm1 += s2; o.pqs += s2; o.ptm1 += s2; o.pl += s2; o.pi += s2;
+  o.plude += s2; o.plu += s2; o.pmfu += s2; o.pmfd += s2; o.ptent += s2; o.pgtent += s2; o.ptenq += s2;
+  o.pgtenq += s2; o.ptenl += s2; o.pgtenl += s2; o.pteni += s2; o.pgteni += s2; o.pclc += s2; o.pfplsl += sh;
+  o.pfplsn += sh; o.pfhpsl += sh; o.pfhpsn += sh; o.pcovptot += s2;
+  return o;
+}
+
 template <typename R>
 hipError_t launch(const cloudsc_params_t& q, const cloudsc2_params_t& q2, bool ldrain1d, double ptsphy,
                   const C2Ptrs<R>& p, const C2Dims& d, hipStream_t stream) {
   const C2Const<R> c = make_const<R>(q, q2, ptsphy, ldrain1d);
-  const int64_t ncols = (int64_t)d.ngpblks * d.nproma;
-  hipLaunchKernelGGL(cloudsc2_kernel<R>, dim3((unsigned)((ncols + 127) / 128)), dim3(128), 0, stream, c, p, d);
-  return hipGetLastError();
+  const bool uni = d.nproma % 64 == 0;
+  // blocks per launch: every (NPROMA,KLEV+1) array of one launch stays below 4 GiB (32-bit byte offsets)
+  const int64_t per_block = (int64_t)d.nproma * (d.klev + 1) * (int64_t)sizeof(R);
+  const int64_t max_blocks = std::max<int64_t>(1, ((int64_t)1 << 32) / per_block - 1);
+  for (int64_t b0 = 0; b0 < d.ngpblks; b0 += max_blocks) {
+    const int64_t nb = std::min<int64_t>(max_blocks, d.ngpblks - b0);
+    C2Dims dc = d;
+    dc.ngpblks = (int)nb;
+    dc.ngptot = (int)(d.ngptot - b0 * d.nproma);
+    const C2Ptrs<R> pc = offset_ptrs(p, b0, d.nproma, d.klev);
+    const dim3 grid((unsigned)((nb * d.nproma + CLOUDSC2_BLOCK - 1) / CLOUDSC2_BLOCK));
+    if (uni)
+      hipLaunchKernelGGL((cloudsc2_kernel<R, true>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+    else
+      hipLaunchKernelGGL((cloudsc2_kernel<R, false>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // Per-block drop-in: a per-thread device workspace and stream, as the other per-block entries.
