@@ -102,7 +102,7 @@ typedef struct cloudsc_params_t {
   /* integer / logical switches */
   int32_t ncldtop;          /* first level the cloud scheme treats (sucldp.F90:631-634) */
   int32_t lmfdsnow;         /* TECUMF: convective snow/rain detrainment */
-  int32_t lcloud_inhomog;   /* TECLDP: must be 0 (regime-dependent FSD not built) */
+  int32_t lcloud_inhomog;   /* TECLDP: KK2000 enhancement from the condensate FSD (per-block and diag entries only) */
   int32_t lrad_cloud_inhomog; /* TEPHY: condensate FSD into PFSD (per-block and cloudsc_blocked_diag_*_c only) */
   int32_t laeraware;        /* any of LAERLIQAUTOLSP/LAERLIQCOLL/LAERICESED/LAERICEAUTO: must be 0 */
   int32_t lspp;             /* TSPP_CONFIG%LSPP: must be 0 */

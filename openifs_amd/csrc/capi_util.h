@@ -10,10 +10,10 @@ namespace cloudsc_capi {
 int fail(int code, const char* what, hipError_t e = hipSuccess);
 // CLOUDSC_OK if the current device is a gfx950, CLOUDSC_ENODEVICE otherwise (result cached per device).
 int check_device();
-// The switches of the parameter block that the kernels implement: CLOUDSC_EUNSUPPORTED for LCLOUD_INHOMOG,
-// aerosol, SPP, unknown lcldbud bits and, unless `diag` (the entry takes PEXTRA and PFSD), the budget switches
-// that write PEXTRA and LRAD_CLOUD_INHOMOG, which writes PFSD.
+// The switches of the parameter block that the kernels implement: CLOUDSC_EUNSUPPORTED for aerosol, SPP, unknown
+// lcldbud bits and, unless `diag` (the entry takes PEXTRA, PGAW and PFSD), the budget switches that write PEXTRA,
+// LRAD_CLOUD_INHOMOG, which writes PFSD, and LCLOUD_INHOMOG, which reads PGAW.
 int check_config(const cloudsc_params_t* p, bool diag = false);
-// CLOUDSC_ENULL if LRAD_CLOUD_INHOMOG is on and PGAW or PFSD is NULL.
+// CLOUDSC_ENULL if LRAD_CLOUD_INHOMOG is on and PGAW or PFSD is NULL, or LCLOUD_INHOMOG is on and PGAW is NULL.
 int check_fsd(const cloudsc_params_t* p, const void* pgaw, const void* pfsd);
 }  // namespace cloudsc_capi

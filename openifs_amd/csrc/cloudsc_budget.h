@@ -65,6 +65,7 @@ struct KBudPtrs {
   R* pfsd;
   int32_t fsd;
   R ra, rpi, rcloud_frac_std;
+  int32_t inhomog;  // LCLOUD_INHOMOG: autoconversion and accretion enhancement from the condensate FSD
 };
 
 // Cloud condensate FSD for the radiation (LRAD_CLOUD_INHOMOG; the ZLFSD/ZIFSD/ZFSD of cloudsc.F90).
@@ -73,6 +74,30 @@ template <typename R>
 __device__ __forceinline__ R fsd_gridlen(R ra, R rpi, R pgaw) {
   return R(2) * ra * m_sqrt(rpi * pgaw) * R(0.001);
 }
+// LCLOUD_INHOMOG (:2075-2098): enhancement of the KK2000 autoconversion and accretion rates for sub-grid
+// lognormal cloud and rain water (Boutle et al. 2013), E[q_c^b]/E[q_c]^b = (1+f_c^2)^(b(b-1)/2) and, for accretion,
+// the bivariate moment with the cloud-rain correlation 1 - 0.8 a.  The rain FSD is Boutle et al.'s fit in the grid
+// length and cloud fraction.
+template <typename R>
+struct InhomogFactors {
+  R aut, acc;
+};
+template <typename R>
+__device__ __forceinline__ InhomogFactors<R> inhomog_factors(R fsdc, R za, R zgridlen, R kkbauq, R kkbac) {
+  const R fc2 = R(1) + fsdc * fsdc;
+  InhomogFactors<R> e;
+  e.aut = m_pow(fc2, (kkbauq - R(1)) * kkbauq / R(2));
+  const R ga = zgridlen * za;
+  const R zphir = m_pow(ga, R(1) / R(3)) * m_pow(m_pow(R(0.11) * ga, R(1.14)) + R(1), R(-0.22));
+  R zfracsdr = (R(1.1) - R(0.8) * za) * zphir;
+  if (za > R(0.95)) zfracsdr = R(0.3) * zphir;
+  const R fr2 = R(1) + zfracsdr * zfracsdr;
+  const R ex = (kkbac - R(1)) * kkbac / R(2);
+  const R corr = R(1) - R(0.8) * za;  // ZCLDRAINCORR (:1947)
+  e.acc = m_pow(fc2, ex) * m_pow(fr2, ex) * m_exp(corr * kkbac * kkbac * m_sqrt(m_log(fc2) * m_log(fr2)));
+  return e;
+}
+
 // Liquid FSD where the in-cloud liquid exceeds ZEPSEC (:1937-1967): Boutle et al. (2013) with the IFS parameters,
 // from the tidied vapour and liquid and the tidied cloud fraction.
 template <typename R>

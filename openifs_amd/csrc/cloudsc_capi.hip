@@ -41,8 +41,10 @@ int fail(int code, const char* what, hipError_t e) {
 
 int check_config(const cloudsc_params_t* p, bool diag) {
   if (!p) return fail(CLOUDSC_ENULL, "cloudsc: NULL parameter block");
-  if (p->lcloud_inhomog || p->laeraware || p->lspp)
-    return fail(CLOUDSC_EUNSUPPORTED, "cloudsc: LCLOUD_INHOMOG/aerosol/SPP are not implemented");
+  if (p->laeraware || p->lspp) return fail(CLOUDSC_EUNSUPPORTED, "cloudsc: aerosol/SPP are not implemented");
+  if (p->lcloud_inhomog && !diag)
+    return fail(CLOUDSC_EUNSUPPORTED,
+                "cloudsc: LCLOUD_INHOMOG reads PGAW: use the per-block entry or cloudsc_blocked_diag_*_c");
   const int32_t out = CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T | CLOUDSC_BUD_VERTINT;
   if ((p->lcldbud & out) && !diag)
     return fail(CLOUDSC_EUNSUPPORTED,
@@ -58,6 +60,7 @@ int check_config(const cloudsc_params_t* p, bool diag) {
 int check_fsd(const cloudsc_params_t* p, const void* pgaw, const void* pfsd) {
   if (p->lrad_cloud_inhomog && (!pgaw || !pfsd))
     return fail(CLOUDSC_ENULL, "cloudsc: LRAD_CLOUD_INHOMOG needs PGAW and PFSD");
+  if (p->lcloud_inhomog && !pgaw) return fail(CLOUDSC_ENULL, "cloudsc: LCLOUD_INHOMOG needs PGAW");
   return CLOUDSC_OK;
 }
 
@@ -138,7 +141,7 @@ int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngp
     rc = check_fsd(q, bud->pgaw, bud->pfsd);
     if (rc) return rc;
   }
-  const bool diag = budgets || (bud && bud->fsd);
+  const bool diag = budgets || (bud && (bud->fsd || bud->inhomog));
   rc = check_device();
   if (rc) return rc;
   KDims d{ngptot, nproma, klev, ngpblks, 1, nproma};
@@ -204,6 +207,7 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
   const bool budgets = q->lcldbud & (CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T |
                                      CLOUDSC_BUD_VERTINT);
   const bool fsd = q->lrad_cloud_inhomog != 0;
+  const bool inh = q->lcloud_inhomog != 0;
   {
     const void* breq[] = {pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra};
     rc = check_budget(q, klev, kfldx, breq, 10);
@@ -252,7 +256,7 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
                 {kctop, nullptr, budgets ? n1 : 0, sizeof(int32_t)},
                 {pextra, budgets ? pextra : nullptr, budgets ? n2 * (size_t)kfldx : 0, sizeof(R)},
                 // condensate FSD (LRAD_CLOUD_INHOMOG)
-                {pgaw, nullptr, fsd ? n1 : 0, sizeof(R)},       {pfsd, fsd ? pfsd : nullptr, fsd ? n2 : 0, sizeof(R)}};
+                {pgaw, nullptr, (fsd || inh) ? n1 : 0, sizeof(R)},       {pfsd, fsd ? pfsd : nullptr, fsd ? n2 : 0, sizeof(R)}};
   const int narr = sizeof(arrs) / sizeof(arrs[0]);
   size_t off[sizeof(arrs) / sizeof(arrs[0])];
   size_t total = 0;
@@ -288,13 +292,15 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
     bp.kfldx = kfldx; bp.flags = q->lcldbud;
   }
   if (fsd) {
-    bp.pgaw = D(52); bp.pfsd = D(53); bp.fsd = 1;
+    bp.pfsd = D(53); bp.fsd = 1;
   }
+  if (fsd || inh) bp.pgaw = D(52);
+  bp.inhomog = inh ? 1 : 0;
   bp.pvfl = p.pvfl; bp.pvfi = p.pvfi;
   bp.reisthsc = R(q->reisthsc); bp.ralfdcp = R(q->ralfdcp);
   bp.ra = R(q->ra); bp.rpi = R(q->rpi); bp.rcloud_frac_std = R(q->rcloud_frac_std);
   e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, kidia - 1, w->stream, nullptr,
-                                         (budgets || fsd) ? &bp : nullptr);
+                                         (budgets || fsd || inh) ? &bp : nullptr);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: kernel launch", e);
   for (int i = 0; i < narr && e == hipSuccess; ++i)
     if (arrs[i].host_out && arrs[i].n)
@@ -342,7 +348,8 @@ KPtrs<R> blocked_ptrs(const R* pt, const R* pq, const R* tcml_t, const R* tcml_q
   cloudsc_gpu::KBudPtrs<R> bud{pvfa, pdyna, pdynl, pdyni, pdyns, phrsw, phrlw, pvfl, pvfi, kpbltype, kctop,        \
                                pextra, kfldx, qd ? qd->lcldbud : 0, R(qd ? qd->reisthsc : 0.0),                   \
                                R(qd ? qd->ralfdcp : 0.0), pgaw, pfsd, qd ? qd->lrad_cloud_inhomog : 0,            \
-                               R(qd ? qd->ra : 0.0), R(qd ? qd->rpi : 0.0), R(qd ? qd->rcloud_frac_std : 0.0)};   \
+                               R(qd ? qd->ra : 0.0), R(qd ? qd->rpi : 0.0), R(qd ? qd->rcloud_frac_std : 0.0),    \
+                               qd ? qd->lcloud_inhomog : 0};                                                    \
   return blocked<R, RD>(ydparams, ngptot, nproma, klev, ngpblks, ptsphy,                                          \
                         blocked_ptrs<R>(pt, pq, ptendency_cml_t, ptendency_cml_q, ptendency_cml_a,                \
                                         ptendency_cml_cld, ptendency_loc_t, ptendency_loc_q, ptendency_loc_a,     \

@@ -12,12 +12,15 @@ namespace cloudsc_gpu {
 // exp(x) = 2^(x log2 e), pow(x,y) = 2^(y log2 x).  Relative error ~ (1 + |y log2 x|) ulp.
 __device__ __forceinline__ float m_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 __device__ __forceinline__ float m_pow(float x, float y) { return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)); }
+__device__ __forceinline__ float m_log(float x) { return __builtin_amdgcn_logf(x) * 0.6931471805599453f; }
 #else
 __device__ __forceinline__ float m_exp(float x) { return expf(x); }
 __device__ __forceinline__ float m_pow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ float m_log(float x) { return logf(x); }
 #endif
 __device__ __forceinline__ double m_exp(double x) { return exp(x); }
 __device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ double m_log(double x) { return log(x); }
 #ifdef CLOUDSC_FAST_TRANSCENDENTALS
 // FP32 division as reciprocal (v_rcp_f32, 1 ulp) times numerator instead of the ~10-instruction
 // IEEE sequence.  Measured against the FP64 oracle on 8192 L137 columns, the FP32 kernel's

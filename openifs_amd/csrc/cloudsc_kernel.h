@@ -41,7 +41,7 @@ struct KConst {
   T rprecrhmax, rpecons, rvrfactor, rcovpmin, rtaumel, rssicefactor, rdenswat, rdensref;
   T rcl_dr, rcl_x1r, rcl_x2r, rcl_x1s, rcl_lam1r, rcl_lam2r, rcl_lam1s, rcl_lam2s;
   T rcl_cdenom1, rcl_cdenom2, rcl_cdenom3, rcl_const1r, rcl_const2r, rcl_const3r, rcl_const4r;
-  T rcl_const5r, rcl_const6r, rcl_const7r, rcl_kkaac, rcl_kkbac, rcl_kkbauq, rcl_kkbaun;
+  T rcl_const5r, rcl_const6r, rcl_const7r, rcl_kkaac, rcl_kkbac, rcl_kkaau, rcl_kkbauq, rcl_kkbaun;
   T rcl_kk_cloud_num_sea, rcl_kk_cloud_num_land, rcl_fzrab, rcl_apb1, rcl_apb2, rcl_apb3;
   T rcl_const1s, rcl_const2s, rcl_const3s, rcl_const4s, rcl_const5s, rcl_const6s, rcl_const7s, rcl_const8s;
   T rmfadvw, rmfadvwdd;
@@ -72,7 +72,7 @@ struct KConstDefault {
   CLOUDSC_DEF(rcl_lam2r) CLOUDSC_DEF(rcl_lam1s) CLOUDSC_DEF(rcl_lam2s) CLOUDSC_DEF(rcl_cdenom1)
   CLOUDSC_DEF(rcl_cdenom2) CLOUDSC_DEF(rcl_cdenom3) CLOUDSC_DEF(rcl_const1r) CLOUDSC_DEF(rcl_const2r)
   CLOUDSC_DEF(rcl_const3r) CLOUDSC_DEF(rcl_const4r) CLOUDSC_DEF(rcl_const5r) CLOUDSC_DEF(rcl_const6r)
-  CLOUDSC_DEF(rcl_const7r) CLOUDSC_DEF(rcl_kkaac) CLOUDSC_DEF(rcl_kkbac) CLOUDSC_DEF(rcl_kkbauq)
+  CLOUDSC_DEF(rcl_const7r) CLOUDSC_DEF(rcl_kkaac) CLOUDSC_DEF(rcl_kkbac) CLOUDSC_DEF(rcl_kkaau) CLOUDSC_DEF(rcl_kkbauq)
   CLOUDSC_DEF(rcl_kkbaun) CLOUDSC_DEF(rcl_kk_cloud_num_sea) CLOUDSC_DEF(rcl_kk_cloud_num_land)
   CLOUDSC_DEF(rcl_fzrab) CLOUDSC_DEF(rcl_apb1) CLOUDSC_DEF(rcl_apb2) CLOUDSC_DEF(rcl_apb3) CLOUDSC_DEF(rcl_const1s)
   CLOUDSC_DEF(rcl_const2s) CLOUDSC_DEF(rcl_const3s) CLOUDSC_DEF(rcl_const4s) CLOUDSC_DEF(rcl_const5s)

@@ -70,8 +70,8 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
     vacc.s = s_bud + NBUD * BLOCK + threadIdx.x;
     if (bp.flags & BUD_VERTINT) vertint_begin(vacc, bp, ext);
   }
-  // LRAD_CLOUD_INHOMOG: grid length of the column (diagnostic instantiation only)
-  const R zgridlen = (BUD && bp.fsd) ? fsd_gridlen(bp.ra, bp.rpi, LD(bp.pgaw, u1, jo)) : R(0);
+  // LRAD_CLOUD_INHOMOG / LCLOUD_INHOMOG: grid length of the column (diagnostic instantiation only)
+  const R zgridlen = (BUD && (bp.fsd || bp.inhomog)) ? fsd_gridlen(bp.ra, bp.rpi, LD(bp.pgaw, u1, jo)) : R(0);
 
   // Per-column inputs
   const R plsm = LD(p.plsm, u1, jo);
@@ -431,13 +431,19 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
 
       // 4.3 warm rain, implicit KK2000 (:2050-2254)
       if (zliqcld > zepsec) {
+        R c_aut = c.c_aut, c_acc = c.c_acc;
         if constexpr (BUD) {
-          if (bp.fsd) fsd_liq = liquid_fsd(qv, ql, za, zgridlen);  // (:1937-1971)
+          if (bp.fsd || bp.inhomog) fsd_liq = liquid_fsd(qv, ql, za, zgridlen);  // (:1937-1971)
+          if (bp.inhomog) {  // LCLOUD_INHOMOG: enhancement factors from the condensate FSD (:2075-2098)
+            const InhomogFactors<R> e = inhomog_factors(fsd_liq, za, zgridlen, c.rcl_kkbauq, c.rcl_kkbac);
+            c_aut = e.aut * ptsphy * c.rcl_kkaau;
+            c_acc = e.acc * ptsphy * c.rcl_kkaac;
+          }
         }
-        zrainaut = c.c_aut * m_pow(zliqcld, c.rcl_kkbauq - R(1)) * zconst_pow;
+        zrainaut = c_aut * m_pow(zliqcld, c.rcl_kkbauq - R(1)) * zconst_pow;
         qbrl = qbrl + zrainaut;
         if (zraincld > zepsec)
-          zrainacc = c.c_acc * m_pow(zliqcld, c.rcl_kkbac - R(1)) * m_pow(zraincld, c.rcl_kkbac);
+          zrainacc = c_acc * m_pow(zliqcld, c.rcl_kkbac - R(1)) * m_pow(zraincld, c.rcl_kkbac);
         qbrl = qbrl + zrainacc;
       }
 
@@ -959,6 +965,7 @@ KConst<T> make_kconst(const cloudsc_params_t& q, double ptsphy_in) {
   c.rcl_const1r = T(q.rcl_const1r); c.rcl_const2r = T(q.rcl_const2r); c.rcl_const3r = T(q.rcl_const3r);
   c.rcl_const4r = T(q.rcl_const4r); c.rcl_const5r = T(q.rcl_const5r); c.rcl_const6r = T(q.rcl_const6r);
   c.rcl_const7r = T(q.rcl_const7r); c.rcl_kkaac = T(q.rcl_kkaac); c.rcl_kkbac = T(q.rcl_kkbac);
+  c.rcl_kkaau = T(q.rcl_kkaau);
   c.rcl_kkbauq = T(q.rcl_kkbauq); c.rcl_kkbaun = T(q.rcl_kkbaun);
   c.rcl_kk_cloud_num_sea = T(q.rcl_kk_cloud_num_sea); c.rcl_kk_cloud_num_land = T(q.rcl_kk_cloud_num_land);
   c.rcl_fzrab = T(q.rcl_fzrab); c.rcl_apb1 = T(q.rcl_apb1); c.rcl_apb2 = T(q.rcl_apb2); c.rcl_apb3 = T(q.rcl_apb3);

@@ -27,12 +27,12 @@ namespace {
 
 constexpr int32_t kBudOut = CLOUDSC_BUD_C | CLOUDSC_BUD_L | CLOUDSC_BUD_I | CLOUDSC_BUD_T | CLOUDSC_BUD_VERTINT;
 
-// The same rules as the product library: budgets that write PEXTRA and the PFSD of LRAD_CLOUD_INHOMOG only where
-// PEXTRA and PFSD are arguments.
+// The same rules as the product library: budgets that write PEXTRA, the PFSD of LRAD_CLOUD_INHOMOG and the PGAW
+// that LCLOUD_INHOMOG reads only where PEXTRA, PGAW and PFSD are arguments.
 int check_config(const cloudsc_params_t* p, bool diag = false) {
   if (!p) return CLOUDSC_ENULL;
-  if (p->lcloud_inhomog || p->laeraware || p->lspp) return CLOUDSC_EUNSUPPORTED;
-  if (((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog) && !diag) return CLOUDSC_EUNSUPPORTED;
+  if (p->laeraware || p->lspp) return CLOUDSC_EUNSUPPORTED;
+  if (((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog || p->lcloud_inhomog) && !diag) return CLOUDSC_EUNSUPPORTED;
   if (p->lcldbud & ~255) return CLOUDSC_EUNSUPPORTED;
   return CLOUDSC_OK;
 }
@@ -51,6 +51,7 @@ int budget_fields(int32_t f) {
 template <typename B>
 int check_budget(const cloudsc_params_t* p, int klev, const B& b) {
   if (p->lrad_cloud_inhomog && (!b.pgaw || !b.pfsd)) return CLOUDSC_ENULL;
+  if (p->lcloud_inhomog && !b.pgaw) return CLOUDSC_ENULL;
   if (!(p->lcldbud & kBudOut)) return CLOUDSC_OK;
   const void* req[] = {b.pvfa, b.pdyna, b.pdynl, b.pdyni, b.pdyns, b.phrsw, b.phrlw, b.kpbltype, b.kctop, b.pextra};
   for (const void* x : req)
@@ -88,7 +89,7 @@ int run_block(const cloudsc_params_t* p, int kidia, int kfdia, int klon, int kle
                     ldcum,  ktype,  peis,    plu,     plude,   pludeli,  psnde,   pmfu,    pmfd,    pa,
                     pclv,   pcovptot, prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf,
                     pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn,
-                    ((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog) ? &bud : nullptr};
+                    ((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog || p->lcloud_inhomog) ? &bud : nullptr};
   oracle::cloudsc<R, RD>(*p, a);
   return CLOUDSC_OK;
 }
@@ -123,7 +124,7 @@ int run_blocked(const cloudsc_params_t* p, int ngptot, int nproma, int klev, int
     rc = check_budget(p, klev, *bud);
     if (rc) return rc;
   }
-  const bool budgets = bud && ((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog);
+  const bool budgets = bud && ((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog || p->lcloud_inhomog);
   const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1), s1 = nproma;
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 1)

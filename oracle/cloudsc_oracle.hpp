@@ -478,10 +478,11 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
   auto BCC = [&](int jl, int n) -> R& { return zbudcc[(size_t)(n - 1) * KL + (jl - 1)]; };
   auto BL = [&](int jl, int n) -> R& { return zbudl[(size_t)(n - 1) * KL + (jl - 1)]; };
   auto BI = [&](int jl, int n) -> R& { return zbudi[(size_t)(n - 1) * KL + (jl - 1)]; };
-  // LRAD_CLOUD_INHOMOG arrays, initialised for all levels (:944-951)
+  // LRAD_CLOUD_INHOMOG arrays, initialised for all levels (:944-951); LCLOUD_INHOMOG uses ZLFSD too
   const bool lfsd = a.bud && p.lrad_cloud_inhomog;
+  const bool linh = a.bud && p.lcloud_inhomog;
   std::vector<R> zifsd, zlfsd, zliqf, zicef, zratfsd, zanewp;
-  if (lfsd) {
+  if (lfsd || linh) {
     zifsd.assign(n2, RCLOUD_FRAC_STD);
     zlfsd.assign(n2, RCLOUD_FRAC_STD);
     zliqf.assign(n2, R(0));
@@ -725,10 +726,11 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
     for (int jl = kidia; jl <= kfdia; ++jl) {
       const int i = jl - 1;
       if (zliqcld[i] > ZEPSEC) {
-        if (lfsd) {  // inhomogeneity for the radiation (:1937-1971)
+        R zgridlen = R(0);
+        if (lfsd || linh) {  // inhomogeneity (:1937-1971)
           const size_t k = i2(jl, jk);
           const R zqtot = std::min((zqx[i3(jl, jk, QV + 1)] + zqx[i3(jl, jk, QL + 1)]) * R(1000.), R(30.));
-          const R zgridlen = R(2) * RA * std::sqrt(RPI * a.bud->pgaw[i]) * R(0.001);
+          zgridlen = R(2) * RA * std::sqrt(RPI * a.bud->pgaw[i]) * R(0.001);
           const R zphip1 = R(.2) + R(.01) * zqtot + R(.0027) * (zqtot * zqtot) - R(.00008) * (zqtot * zqtot * zqtot);
           const R zphip2 = zphip1 - R(0.2);
           const R zphip3 = R(0.123) * std::exp(-(zphip1 * R(0.55)));
@@ -745,7 +747,22 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         } else {
           zconst = RCL_KK_CLOUD_NUM_SEA;
         }
-        const R zeaut = RCL_INHOMOGAUT, zeacc = RCL_INHOMOGACC;
+        R zeaut = RCL_INHOMOGAUT, zeacc = RCL_INHOMOGACC;
+        if (linh) {  // Boutle et al. (2013) enhancement factors (:2075-2098); ZCLDRAINCORR (:1947)
+          const size_t k = i2(jl, jk);
+          const R zfracsdc = zlfsd[k];
+          const R zcldraincorr = R(1) - R(0.8) * za[k];
+          zeaut = std::pow(R(1) + std::pow(zfracsdc, R(2)), (RCL_KKBAUQ - R(1)) * RCL_KKBAUQ / R(2));
+          const R zphir = std::pow(zgridlen * za[k], R(1) / R(3)) *
+                          std::pow(std::pow(R(0.11) * zgridlen * za[k], R(1.14)) + R(1), R(-0.22));
+          R zfracsdr = (R(1.1) - R(0.8) * za[k]) * zphir;
+          if (za[k] > R(0.95)) zfracsdr = R(0.3) * zphir;
+          zeacc = std::pow(R(1) + std::pow(zfracsdc, R(2)), (RCL_KKBAC - R(1)) * RCL_KKBAC / R(2)) *
+                  std::pow(R(1) + std::pow(zfracsdr, R(2)), (RCL_KKBAC - R(1)) * RCL_KKBAC / R(2)) *
+                  std::exp(zcldraincorr * RCL_KKBAC * RCL_KKBAC *
+                           std::sqrt(std::log(R(1) + std::pow(zfracsdc, R(2))) *
+                                     std::log(R(1) + std::pow(zfracsdr, R(2)))));
+        }
         zrainaut[i] = zeaut * ptsphy * RCL_KKAAU * std::pow(zliqcld[i], RCL_KKBAUQ - R(1)) * std::pow(zconst, RCL_KKBAUN);
         QB(jl, QR, QL) = QB(jl, QR, QL) + zrainaut[i];
         if (zraincld[i] > ZEPSEC) {

@@ -17,11 +17,16 @@ def _blocked_null_call(params, ngptot=64, nproma=32, klev=137, ngpblks=2):
 
 
 def test_unsupported_switches_are_rejected():
-    for sw in ("lcloud_inhomog", "laeraware", "lspp"):
+    for sw in ("laeraware", "lspp"):
         p = C.default_params()
         setattr(p, sw, 1)
         assert _blocked_null_call(p) == -2  # CLOUDSC_EUNSUPPORTED
         assert b"not implemented" in C.lib().cloudsc_last_error()
+    # LCLOUD_INHOMOG reads PGAW, which the plain blocked entry does not take
+    p = C.default_params()
+    p.lcloud_inhomog = 1
+    assert _blocked_null_call(p) == -2
+    assert b"PGAW" in C.lib().cloudsc_last_error()
     # LRAD_CLOUD_INHOMOG writes PFSD, which the plain blocked entry does not take
     p = C.default_params()
     p.lrad_cloud_inhomog = 1
