@@ -27,6 +27,9 @@ Parity status of each oracle function (what pins it; see tests/test_oracle_*.py)
                                  (detrainment ratio, layer-depth/total-water factor, overcast/broken branch),
                                  defaults and bounds; liquid FSD (Boutle et al. 2013, IFS parameters): parity
                                  unpinned
+  LCLOUD_INHOMOG factors         pinned (tests/test_inhomog.py): autoconversion and accretion enhancement equal the
+                                 lognormal and bivariate-lognormal moment ratios by Gauss-Hermite quadrature in an
+                                 overcast warm layer; no change without liquid
   CLOUDSC2 (cloudsc2_oracle.hpp) pinned (tests/test_cloudsc2.py): per-level water closure of the tendencies,
                                  the precipitation fluxes and the detrainment, the enthalpy fluxes, the dry column,
                                  no supersaturation left after CUADJTQ/CUADJTQS, cloud cover 0/1 above the
