@@ -62,6 +62,26 @@ int cloudsc2_sp_c(CLOUDSC2_ARGS(float));
 int cloudsc2_blocked_dp_c(CLOUDSC2_BLOCKED_ARGS(double));
 int cloudsc2_blocked_sp_c(CLOUDSC2_BLOCKED_ARGS(float));
 
+/* CLOUD_S_LAYER with LEPCLD2 (cloud_s_layer.F90:112-176), all blocks of device-resident blocked fields in one
+ * launch: SATUR (KFLAG 2) at PT + TSPHY*PTENDENCY_T gives PQSAT, CLOUDSC2 runs on it, and the wrap-up writes
+ * PTENDENCY_LOC_A = (PCLC - PA)/TSPHY on every level, zero rain and snow tendencies on every column and level,
+ * and zero PPRECTYPE and PFZRA.  Arguments as CLOUD_S_LAYER passes them to CLOUDSC2: PRS1 = PAPHP1, PRSF1 = PAPP1,
+ * PQ = PQM1, PT = PTM1, PTENDENCY_* = PGTEN*, PTENDENCY_LOC_* = PTEN*; PQSAT is PDIAG%PQSAT, PCLC is AUXL%ZPATMP.
+ * PTENDENCY_LOC_A (which needs PA), PTENDENCY_LOC_R/S, PQSAT, PCLC, PPRECTYPE (NPROMA,NGPBLKS) and PFZRA may be
+ * NULL where the caller's key mask has no such field.  The LEPCLD2 = false branch (CLOUDST) is not built. */
+#define CLOUD_S_LAYER_BLOCKED_ARGS(R)                                                                   \
+  const cloudsc_params_t *ydparams, const cloudsc2_params_t *ydparams2, const R *ceta, int ngptot,       \
+      int nproma, int ktdia, int klev, int ngpblks, int ldrain1d, R tsphy, const R *prs1,              \
+      const R *prsf1, const R *pq, const R *pt, const R *pa, const R *pl, const R *pi, const R *plude,  \
+      const R *plu, const R *pmfu, const R *pmfd, const R *ptendency_t, const R *ptendency_q,           \
+      const R *ptendency_l, const R *ptendency_i, R *ptendency_loc_t, R *ptendency_loc_q,              \
+      R *ptendency_loc_a, R *ptendency_loc_l, R *ptendency_loc_i, R *ptendency_loc_r,                  \
+      R *ptendency_loc_s, R *pqsat, R *pclc, R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, R *pcovptot,  \
+      R *pprectype, R *pfzra, void *stream
+
+int cloud_s_layer_blocked_dp_c(CLOUD_S_LAYER_BLOCKED_ARGS(double));
+int cloud_s_layer_blocked_sp_c(CLOUD_S_LAYER_BLOCKED_ARGS(float));
+
 #ifdef __cplusplus
 }
 #endif

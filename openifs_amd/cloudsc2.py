@@ -168,3 +168,70 @@ def synth_device(params, seed, ngptot, nproma, klev, precision="sp", device="cud
         if role == "o":
             out[name] = torch.full(blocked_shape(kind, nproma, klev, ngpblks), fill, dtype=dt, device=device)
     return out
+
+
+# CLOUD_S_LAYER (LEPCLD2) argument arrays after TSPHY: (name, kind, role, optional); kind as CLOUDSC2_ARRAYS plus
+# "1d" = (NPROMA,) per block
+LAYER_ARRAYS = [
+    ("prs1", "h", "i", False), ("prsf1", "2d", "i", False), ("pq", "2d", "i", False), ("pt", "2d", "i", False),
+    ("pa", "2d", "i", True), ("pl", "2d", "i", False), ("pi", "2d", "i", False), ("plude", "2d", "i", False),
+    ("plu", "2d", "i", False), ("pmfu", "2d", "i", False), ("pmfd", "2d", "i", False),
+    ("ptendency_t", "2d", "i", False), ("ptendency_q", "2d", "i", False), ("ptendency_l", "2d", "i", False),
+    ("ptendency_i", "2d", "i", False), ("ptendency_loc_t", "2d", "o", False), ("ptendency_loc_q", "2d", "o", False),
+    ("ptendency_loc_a", "2d", "o", True), ("ptendency_loc_l", "2d", "o", False), ("ptendency_loc_i", "2d", "o", False),
+    ("ptendency_loc_r", "2d", "o", True), ("ptendency_loc_s", "2d", "o", True), ("pqsat", "2d", "o", True),
+    ("pclc", "2d", "o", True), ("pfplsl", "h", "o", False), ("pfplsn", "h", "o", False), ("pfhpsl", "h", "o", False),
+    ("pfhpsn", "h", "o", False), ("pcovptot", "2d", "o", False), ("pprectype", "1d", "o", True),
+    ("pfzra", "1d", "o", True),
+]
+# how CLOUD_S_LAYER passes its fields to CLOUDSC2 (cloud_s_layer.F90:126-139)
+LAYER_FROM_CLOUDSC2 = {"prs1": "paphp1", "prsf1": "papp1", "pq": "pqm1", "pt": "ptm1", "pl": "pl", "pi": "pi",
+                       "plude": "plude", "plu": "plu", "pmfu": "pmfu", "pmfd": "pmfd", "ptendency_t": "pgtent",
+                       "ptendency_q": "pgtenq", "ptendency_l": "pgtenl", "ptendency_i": "pgteni"}
+
+
+def layer_shape(kind, nproma, klev, ngpblks):
+    return (ngpblks, nproma) if kind == "1d" else blocked_shape(kind, nproma, klev, ngpblks)
+
+
+def layer_args(ptr, fields, required_only=False):
+    """Pointers of LAYER_ARRAYS in order; optional arrays missing from ``fields`` are NULL."""
+    out = []
+    for name, kind, role, opt in LAYER_ARRAYS:
+        if name in fields and not (required_only and opt):
+            out.append(ctypes.c_void_p(ptr(fields[name])))
+        elif opt:
+            out.append(ctypes.c_void_p(0))
+        else:
+            raise KeyError(name)
+    return out
+
+
+def cloud_s_layer_blocked(fields, params, params2, ceta, ngptot, nproma, klev, tsphy, precision="sp", ktdia=1,
+                          ldrain1d=False, stream=None):
+    """CLOUD_S_LAYER with LEPCLD2 in one launch of cloud_s_layer_blocked_{dp,sp}_c on device tensors in blocked
+    layout; optional outputs absent from ``fields`` are skipped."""
+    import torch
+
+    ngpblks = (ngptot + nproma - 1) // nproma
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.data_ptr()), ctypes.c_int(ngptot),
+            ctypes.c_int(nproma), ctypes.c_int(ktdia), ctypes.c_int(klev), ctypes.c_int(ngpblks),
+            ctypes.c_int(int(bool(ldrain1d))), _ctype_real(precision)(tsphy)]
+    args += layer_args(lambda t: t.data_ptr(), fields)
+    args.append(ctypes.c_void_p(stream))
+    fn = f"cloud_s_layer_blocked_{precision}_c"
+    C._check(fn, getattr(C.lib(), fn)(*args))
+
+
+def layer_host(c2_inputs, pa, precision="dp"):
+    """CLOUD_S_LAYER input arrays from CLOUDSC2 inputs (synth_host) and a cloud fraction PA; outputs NaN."""
+    dt = _real(precision)
+    res = {lay: np.ascontiguousarray(c2_inputs[c2], dtype=dt) for lay, c2 in LAYER_FROM_CLOUDSC2.items()}
+    res["pa"] = np.ascontiguousarray(pa, dtype=dt)
+    ngpblks, klev, nproma = res["pq"].shape
+    for name, kind, role, opt in LAYER_ARRAYS:
+        if role == "o":
+            res[name] = np.full(layer_shape(kind, nproma, klev, ngpblks), np.nan, dtype=dt)
+    return res

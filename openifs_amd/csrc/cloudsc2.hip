@@ -33,6 +33,7 @@ struct C2Const {
   R ptsphy, zckcodtl, zckcodti, zcons2, zcons3, zmeltp2, zqtmst;
   int32_t lin;      // LPHYLIN or LDRAIN1D: tanh phase function and CUADJTQS
   int32_t evap;     // LEVAPLS2 or LDRAIN1D: precipitation evaporation, the 1.9 RCLCRIT / 1e-4 thresholds
+  int32_t phylin;   // LPHYLIN alone: SATUR's form of the saturation humidity (CLOUD_S_LAYER)
 };
 
 template <typename R>
@@ -47,6 +48,9 @@ struct C2Ptrs {
   R* pteni;
   const R* pgteni;
   R *pclc, *pfplsl, *pfplsn, *pfhpsl, *pfhpsn, *pcovptot;
+  // CLOUD_S_LAYER only (LAYER): the cloud fraction and the wrap-up outputs, NULL where the caller has no such field
+  const R* pa;
+  R *ptena, *ptenr, *ptens, *pqsat, *pprectype, *pfzra;
 };
 
 struct C2Dims {
@@ -107,10 +111,29 @@ __device__ __forceinline__ void adjust(const C2Const<R>& c, R psp, R& pt, R& pq)
 // Work-group size: 64, 128 and 256 run within 1 % of each other at 655,360 x L137 in FP32.
 constexpr int CLOUDSC2_BLOCK = 128;
 
+// SATUR with KFLAG = 2 (satur.F90:110-141): the saturation specific humidity of the mixed phase at (t, p), in the
+// LPHYLIN form (separate water and ice pressures weighted by FOEALFA) or as FOEEWM.
+template <typename R>
+__device__ __forceinline__ R satur(const C2Const<R>& c, R t, R paprsf) {
+  const R a = foealfa(c, t);
+  if (c.phylin) {
+    const R el = c.r2es * m_exp(m_div(c.r3les * (t - c.rtt), t - c.r4les));
+    const R ei = c.r2es * m_exp(m_div(c.r3ies * (t - c.rtt), t - c.r4ies));
+    R qs = m_div(a * el + (R(1.0) - a) * ei, paprsf);
+    if (qs > R(0.5)) qs = R(0.5);
+    return qs * m_div(R(1.0), R(1.0) - c.retv * qs);
+  }
+  const R qs = m_min(R(0.5), m_div(foeewm(c, t, a), paprsf));
+  return qs * m_div(R(1.0), R(1.0) - c.retv * qs);
+}
+
 // UNI: NPROMA is a multiple of 64, so every wave lies in one block and the block's offsets are wave-uniform.
 // Arrays are addressed as a scalar base plus a 32-bit byte offset (LD/ST); the launcher keeps every array of one
 // launch below 4 GiB.
-template <typename R, bool UNI>
+// LAYER: CLOUD_S_LAYER around CLOUDSC2 (cloud_s_layer.F90:112-176): PQS from SATUR at T + TSPHY*TENDENCY%T in the
+// sweep instead of an input, the cloud-fraction tendency (PCLC - PA)/TSPHY, zero rain and snow tendencies and
+// zero PPRECTYPE/PFZRA.
+template <typename R, bool UNI, bool LAYER>
 __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudsc2_kernel(const C2Const<R> c, const C2Ptrs<R> p,
                                                                   const C2Dims d) {
   const uint32_t g = blockIdx.x * CLOUDSC2_BLOCK + threadIdx.x;
@@ -120,8 +143,6 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudsc2_kernel(const C2Const<
   uint32_t b = g / nproma;
   if (UNI) b = __builtin_amdgcn_readfirstlane(b);
   const uint32_t jl = g - b * nproma;
-  const int kfdia = std::min(d.kfdia, d.ngptot - (int)(b * nproma));
-  if ((int)jl < d.kidia - 1 || (int)jl >= kfdia) return;
   constexpr uint32_t E = sizeof(R);
   const uint32_t jo = jl * E;                          // the lane's byte offset within a level
   const uint32_t lstep = nproma * E;                   // one level
@@ -129,7 +150,19 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudsc2_kernel(const C2Const<
   const uint32_t bh = b2 + b * lstep;                  // level 0 of a (NPROMA,KLEV+1) array
   auto at2 = [&](int k) { return b2 + (uint32_t)k * lstep; };
   auto ath = [&](int k) { return bh + (uint32_t)k * lstep; };
+  if constexpr (LAYER) {  // TENDENCY_LOC%CLD(:,:,NCLDQR/NCLDQS) = 0 on every column of the block (:164-165)
+    for (int k = 0; k < klev; ++k) {
+      if (p.ptenr) ST(p.ptenr, at2(k), jo, R(0));
+      if (p.ptens) ST(p.ptens, at2(k), jo, R(0));
+    }
+  }
+  const int kfdia = std::min(d.kfdia, d.ngptot - (int)(b * nproma));
+  if ((int)jl < d.kidia - 1 || (int)jl >= kfdia) return;
   const R ptsphy = c.ptsphy;
+  if constexpr (LAYER) {  // (:168-169)
+    if (p.pprectype) ST(p.pprectype, b * lstep, jo, R(0));
+    if (p.pfzra) ST(p.pfzra, b * lstep, jo, R(0));
+  }
   const R zeps1 = R(1.e-12), zeps2 = R(1.e-10), zqmax = R(0.5);
 
   // eta of the tropopause (:323-334): the last level with 0.1 < eta < 0.4 and T above the level below
@@ -144,8 +177,11 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudsc2_kernel(const C2Const<
   }
   // outputs the level loop does not reach (:297-320): PCLC, PCOVPTOT zero, PFPLSL/PFPLSN zero at the top
   for (int k = 0; k < d.ktdia - 1; ++k) {
-    ST(p.pclc, at2(k), jo, R(0));
+    if (!LAYER || p.pclc) ST(p.pclc, at2(k), jo, R(0));
     ST(p.pcovptot, at2(k), jo, R(0));
+    if constexpr (LAYER) {  // (PCLC - PA)/TSPHY with PCLC = 0 above KTDIA (:155-158)
+      if (p.ptena) ST(p.ptena, at2(k), jo, c.zqtmst * (R(0) - LD(p.pa, at2(k), jo)));
+    }
   }
   ST(p.pfplsl, ath(0), jo, R(0));
   ST(p.pfplsn, ath(0), jo, R(0));
@@ -160,14 +196,16 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudsc2_kernel(const C2Const<
   // The inputs of one level, loaded one level ahead of the sweep so that the loads of level k+1 are in flight
   // while level k computes.
   struct Level {
-    R tm1, gtent, qm1, gtenq, l, gtenl, i, gteni, paph1, qs, app1, lude, lu_next, mfu, mfd;
+    R tm1, gtent, qm1, gtenq, l, gtenl, i, gteni, paph1, qs, app1, lude, lu_next, mfu, mfd, a;
   };
   auto load = [&](int k) {
     const uint32_t j = at2(k);
     Level v;
     v.tm1 = LD(p.ptm1, j, jo); v.gtent = LD(p.pgtent, j, jo); v.qm1 = LD(p.pqm1, j, jo);
     v.gtenq = LD(p.pgtenq, j, jo); v.l = LD(p.pl, j, jo); v.gtenl = LD(p.pgtenl, j, jo); v.i = LD(p.pi, j, jo);
-    v.gteni = LD(p.pgteni, j, jo); v.paph1 = LD(p.paphp1, ath(k + 1), jo); v.qs = LD(p.pqs, j, jo);
+    v.gteni = LD(p.pgteni, j, jo); v.paph1 = LD(p.paphp1, ath(k + 1), jo);
+    v.qs = LAYER ? R(0) : LD(p.pqs, j, jo);
+    v.a = (LAYER && p.ptena) ? LD(p.pa, j, jo) : R(0);
     v.app1 = LD(p.papp1, j, jo); v.lude = LD(p.plude, j, jo);
     v.lu_next = k < klev - 1 ? LD(p.plu, j + lstep, jo) : R(0);
     v.mfu = LD(p.pmfu, j, jo); v.mfd = LD(p.pmfd, j, jo);
@@ -191,7 +229,12 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudsc2_kernel(const C2Const<
     const R zdp = paph1 - paph0;
     const R zzz = m_div(R(1), c.rcpd + c.rcpd * c.rvtmp2 * zqp1);
     const R zlfdcp = c.rlmlt * zzz, zlsdcp = c.rlstt * zzz, zlvdcp = c.rlvtt * zzz;
-    const R pqs = cur.qs, papp1 = cur.app1, plude = cur.lude;
+    const R papp1 = cur.app1, plude = cur.lude;
+    // CLOUD_S_LAYER's SATUR at the first-guess temperature (cloud_s_layer.F90:112-118), else the PQS input
+    const R pqs = LAYER ? satur(c, ztp1, papp1) : cur.qs;
+    if constexpr (LAYER) {
+      if (p.pqsat) ST(p.pqsat, i2, jo, pqs);
+    }
 
     // 3.1 (:352-416)
     R zfwat, zfoeew, zesdp;
@@ -374,7 +417,10 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudsc2_kernel(const C2Const<
     ST(p.ptent, i2, jo, tendency_t(zcondl, zcondi, zrfreeze));
     ST(p.ptenl, i2, jo, (zqlwc - zl) * c.zqtmst);
     ST(p.pteni, i2, jo, (zqiwc - zi) * c.zqtmst);
-    ST(p.pclc, i2, jo, pclc);
+    if (!LAYER || p.pclc) ST(p.pclc, i2, jo, pclc);
+    if constexpr (LAYER) {  // (:155-158)
+      if (p.ptena) ST(p.ptena, i2, jo, c.zqtmst * (pclc - cur.a));
+    }
     ST(p.pcovptot, i2, jo, pcovptot);
     const uint32_t ih = ath(k + 1);
     ST(p.pfplsl, ih, jo, zrfln);
@@ -406,6 +452,7 @@ C2Const<R> make_const(const cloudsc_params_t& q, const cloudsc2_params_t& q2, do
   c.zmeltp2 = c.rtt + R(2.0);
   c.zqtmst = R(1) / ts;
   c.lin = (q2.lphylin || ldrain1d) ? 1 : 0;
+  c.phylin = q2.lphylin ? 1 : 0;
   c.evap = (q2.levapls2 || ldrain1d) ? 1 : 0;
   return c;
 }
@@ -422,16 +469,20 @@ template <typename R>
 C2Ptrs<R> offset_ptrs(const C2Ptrs<R>& p, int64_t b0, int nproma, int klev) {
   const int64_t s2 = b0 * nproma * klev, sh = b0 * nproma * (klev + 1);
   C2Ptrs<R> o = p;
-  o.paphp1 += sh; o.papp1 += s2; o.pqm1 += s2; o.pqs += s2; o.ptm1 += s2; o.pl += s2; o.pi += s2;
+  o.paphp1 += sh; o.papp1 += s2; o.pqm1 += s2; o.pqs = o.pqs ? o.pqs + s2 : o.pqs; o.ptm1 += s2; o.pl += s2; o.pi += s2;
   o.plude += s2; o.plu += s2; o.pmfu += s2; o.pmfd += s2; o.ptent += s2; o.pgtent += s2; o.ptenq += s2;
-  o.pgtenq += s2; o.ptenl += s2; o.pgtenl += s2; o.pteni += s2; o.pgteni += s2; o.pclc += s2; o.pfplsl += sh;
+  o.pgtenq += s2; o.ptenl += s2; o.pgtenl += s2; o.pteni += s2; o.pgteni += s2; o.pfplsl += sh;
   o.pfplsn += sh; o.pfhpsl += sh; o.pfhpsn += sh; o.pcovptot += s2;
+  auto adv = [](auto* x, int64_t n) { return x ? x + n : x; };  // optional arrays stay NULL
+  o.pa = adv(o.pa, s2); o.ptena = adv(o.ptena, s2); o.ptenr = adv(o.ptenr, s2); o.ptens = adv(o.ptens, s2);
+  o.pqsat = adv(o.pqsat, s2); o.pclc = adv(o.pclc, s2);
+  o.pprectype = adv(o.pprectype, b0 * nproma); o.pfzra = adv(o.pfzra, b0 * nproma);
   return o;
 }
 
 template <typename R>
 hipError_t launch(const cloudsc_params_t& q, const cloudsc2_params_t& q2, bool ldrain1d, double ptsphy,
-                  const C2Ptrs<R>& p, const C2Dims& d, hipStream_t stream) {
+                  const C2Ptrs<R>& p, const C2Dims& d, hipStream_t stream, bool layer = false) {
   const C2Const<R> c = make_const<R>(q, q2, ptsphy, ldrain1d);
   const bool uni = d.nproma % 64 == 0;
   // blocks per launch: every (NPROMA,KLEV+1) array of one launch stays below 4 GiB (32-bit byte offsets)
@@ -444,10 +495,14 @@ hipError_t launch(const cloudsc_params_t& q, const cloudsc2_params_t& q2, bool l
     dc.ngptot = (int)(d.ngptot - b0 * d.nproma);
     const C2Ptrs<R> pc = offset_ptrs(p, b0, d.nproma, d.klev);
     const dim3 grid((unsigned)((nb * d.nproma + CLOUDSC2_BLOCK - 1) / CLOUDSC2_BLOCK));
-    if (uni)
-      hipLaunchKernelGGL((cloudsc2_kernel<R, true>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+    if (layer && uni)
+      hipLaunchKernelGGL((cloudsc2_kernel<R, true, true>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+    else if (layer)
+      hipLaunchKernelGGL((cloudsc2_kernel<R, false, true>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+    else if (uni)
+      hipLaunchKernelGGL((cloudsc2_kernel<R, true, false>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
     else
-      hipLaunchKernelGGL((cloudsc2_kernel<R, false>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+      hipLaunchKernelGGL((cloudsc2_kernel<R, false, false>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -556,6 +611,29 @@ int blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* cet
   return CLOUDSC_OK;
 }
 
+// CLOUD_S_LAYER with LEPCLD2 (cloud_s_layer.F90:112-176) over all blocks: SATUR, CLOUDSC2 and the wrap-up in one
+// launch.  PTENDENCY_LOC_A/R/S, PQSAT, PCLC (AUXL%ZPATMP), PPRECTYPE and PFZRA may be NULL.
+template <typename R>
+int layer_blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* ceta, int ngptot, int nproma,
+                  int ktdia, int klev, int ngpblks, int ldrain1d, R tsphy, const C2Ptrs<R>& p, void* stream) {
+  int rc = check(q, q2, ktdia, klev);
+  if (rc) return rc;
+  if (nproma < 1 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma)
+    return fail(CLOUDSC_EDIMS, "cloud_s_layer_blocked: need NGPBLKS = ceil(NGPTOT/NPROMA)");
+  const void* req[] = {p.ceta,  p.paphp1, p.papp1, p.pqm1,   p.ptm1,   p.pl,     p.pi,     p.plude,  p.plu,
+                       p.pmfu,  p.pmfd,   p.ptent, p.pgtent, p.ptenq,  p.pgtenq, p.ptenl,  p.pgtenl, p.pteni,
+                       p.pgteni, p.pfplsl, p.pfplsn, p.pfhpsl, p.pfhpsn, p.pcovptot};
+  for (const void* x : req)
+    if (!x) return fail(CLOUDSC_ENULL, "cloud_s_layer_blocked: NULL array");
+  if (p.ptena && !p.pa) return fail(CLOUDSC_ENULL, "cloud_s_layer_blocked: PTENDENCY_LOC_A needs PA");
+  rc = cloudsc_capi::check_device();
+  if (rc) return rc;
+  C2Dims d{ngptot, nproma, klev, ngpblks, ktdia, 1, nproma};
+  hipError_t e = launch<R>(*q, *q2, ldrain1d != 0, (double)tsphy, p, d, (hipStream_t)stream, true);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloud_s_layer_blocked: kernel launch", e);
+  return CLOUDSC_OK;
+}
+
 }  // namespace cloudsc2_gpu
 
 #define CLOUDSC2_PER_BLOCK(R)                                                                                     \
@@ -570,6 +648,16 @@ int blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* cet
                                   pclc,  pfplsl, pfplsn, pfhpsl, pfhpsn, pcovptot};                              \
   return cloudsc2_gpu::blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d,     \
                                   ptsphy, p, stream)
+
+#define CLOUD_S_LAYER_BLOCKED(R)                                                                                \
+  cloudsc2_gpu::C2Ptrs<R> p{ceta,  prs1,   prsf1,  pq,    nullptr, pt,    pl,    pi,     plude,   plu,         \
+                            pmfu,  pmfd,   ptendency_loc_t, ptendency_t, ptendency_loc_q, ptendency_q,            \
+                            ptendency_loc_l, ptendency_l, ptendency_loc_i, ptendency_i, pclc, pfplsl, pfplsn,    \
+                            pfhpsl, pfhpsn, pcovptot};                                                             \
+  p.pa = pa; p.ptena = ptendency_loc_a; p.ptenr = ptendency_loc_r; p.ptens = ptendency_loc_s; p.pqsat = pqsat;   \
+  p.pprectype = pprectype; p.pfzra = pfzra;                                                                       \
+  return cloudsc2_gpu::layer_blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d, \
+                                        tsphy, p, stream)
 
 extern "C" {
 
@@ -589,5 +677,7 @@ int cloudsc2_dp_c(CLOUDSC2_ARGS(double)) { CLOUDSC2_PER_BLOCK(double); }
 int cloudsc2_sp_c(CLOUDSC2_ARGS(float)) { CLOUDSC2_PER_BLOCK(float); }
 int cloudsc2_blocked_dp_c(CLOUDSC2_BLOCKED_ARGS(double)) { CLOUDSC2_BLOCKED(double); }
 int cloudsc2_blocked_sp_c(CLOUDSC2_BLOCKED_ARGS(float)) { CLOUDSC2_BLOCKED(float); }
+int cloud_s_layer_blocked_dp_c(CLOUD_S_LAYER_BLOCKED_ARGS(double)) { CLOUD_S_LAYER_BLOCKED(double); }
+int cloud_s_layer_blocked_sp_c(CLOUD_S_LAYER_BLOCKED_ARGS(float)) { CLOUD_S_LAYER_BLOCKED(float); }
 
 }  // extern "C"

@@ -277,3 +277,34 @@ def cloudsc2_blocked(precision, params, params2, ceta, ngptot, nproma, ktdia, kl
     if rc != 0:
         raise RuntimeError(f"oracle_cloudsc2_blocked_{precision} returned {rc}")
     return arrays
+
+
+def cloud_s_layer_blocked(precision, params, params2, ceta, ngptot, nproma, ktdia, klev, ldrain1d, tsphy, arrays,
+                          nthreads=0):
+    """CLOUD_S_LAYER (LEPCLD2) over all blocks of blocked host arrays named as openifs_amd.cloudsc2.LAYER_ARRAYS;
+    optional arrays missing from ``arrays`` are passed as NULL."""
+    from openifs_amd.cloudsc2 import LAYER_ARRAYS, layer_shape
+
+    ngpblks = (ngptot + nproma - 1) // nproma
+    real = ctypes.c_double if precision == "dp" else ctypes.c_float
+    rdt = np.float64 if precision == "dp" else np.float32
+    ceta = np.ascontiguousarray(ceta, dtype=rdt)
+    args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.ctypes.data), ctypes.c_int(ngptot),
+            ctypes.c_int(nproma), ctypes.c_int(ktdia), ctypes.c_int(klev), ctypes.c_int(ngpblks),
+            ctypes.c_int(int(bool(ldrain1d))), real(tsphy)]
+    for name, kind, _, opt in LAYER_ARRAYS:
+        a = arrays.get(name)
+        if a is None:
+            if not opt:
+                raise KeyError(name)
+            args.append(ctypes.c_void_p(0))
+            continue
+        want = layer_shape(kind, nproma, klev, ngpblks)
+        if a.dtype != rdt or not a.flags.c_contiguous or a.shape != want:
+            raise TypeError(f"{name}: need a C-contiguous {np.dtype(rdt).name} array of shape {want}")
+        args.append(ctypes.c_void_p(a.ctypes.data))
+    args.append(ctypes.c_int(nthreads))
+    rc = getattr(lib(), f"oracle_cloud_s_layer_blocked_{precision}")(*args)
+    if rc != 0:
+        raise RuntimeError(f"oracle_cloud_s_layer_blocked_{precision} returned {rc}")
+    return arrays

@@ -419,4 +419,77 @@ void cloudsc2(const cloudsc_params_t& p, const cloudsc2_params_t& p2, const Clou
     }
 }
 
+// SATUR, KFLAG = 2 (satur.F90:110-141), levels KTDIA..KLEV.
+template <typename R>
+void satur2(const cloudsc_params_t& p, bool ldphylin, int kidia, int kfdia, int klon, int ktdia, int klev,
+            const R* paprsf, const R* pt, R* pqsat) {
+  const Thermo<R> th(p);
+  const R RETV = R(p.retv), RTT = R(p.rtt), R2ES = R(p.r2es), R3LES = R(p.r3les), R3IES = R(p.r3ies);
+  const R R4LES = R(p.r4les), R4IES = R(p.r4ies);
+  const R ZQMAX = R(0.5);
+  for (int jk = ktdia; jk <= klev; ++jk)
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      const size_t k = (size_t)(jk - 1) * klon + (jl - 1);
+      if (ldphylin) {
+        const R ztarg = pt[k];
+        const R zalfa = th.foealfa(ztarg);
+        const R zfoeewl = R2ES * std::exp(R3LES * (ztarg - RTT) / (ztarg - R4LES));
+        const R zfoeewi = R2ES * std::exp(R3IES * (ztarg - RTT) / (ztarg - R4IES));
+        const R zfoeew = zalfa * zfoeewl + (R(1.0) - zalfa) * zfoeewi;
+        R zqs = zfoeew / paprsf[k];
+        if (zqs > ZQMAX) zqs = ZQMAX;
+        const R zcor = R(1.0) / (R(1.0) - RETV * zqs);
+        pqsat[k] = zqs * zcor;
+      } else {
+        const R zew = th.foeewm(pt[k]);
+        R zqs = zew / paprsf[k];
+        zqs = std::min(ZQMAX, zqs);
+        const R zcor = R(1.0) / (R(1.0) - RETV * zqs);
+        pqsat[k] = zqs * zcor;
+      }
+    }
+}
+
+// The arrays of CLOUD_S_LAYER's LEPCLD2 branch beyond CLOUDSC2's (NULL where the caller has no such field).
+template <typename R>
+struct LayerArgs {
+  const R* pa;
+  R *ptena, *ptenr, *ptens, *pqsat, *pprectype, *pfzra;
+};
+
+// CLOUD_S_LAYER with LEPCLD2 (cloud_s_layer.F90:112-176): SATUR at T + TSPHY*TENDENCY%T into PQSAT, CLOUDSC2 with
+// it, then the cloud-fraction tendency, zero rain and snow tendencies and zero PPRECTYPE/PFZRA.  `a` carries the
+// CLOUDSC2 arguments; its PQS is the PQSAT array filled here and its PCLC is ZPATMP.
+template <typename R>
+void cloud_s_layer(const cloudsc_params_t& p, const cloudsc2_params_t& p2, Cloudsc2Args<R> a, const LayerArgs<R>& l) {
+  const int kidia = a.kidia, kfdia = a.kfdia, klon = a.klon, klev = a.klev;
+  auto i2 = [klon](int jl, int jk) { return (size_t)(jk - 1) * klon + (jl - 1); };
+  std::vector<R> z_t1((size_t)klon * klev, R(0)), zpqsat((size_t)klon * klev, R(0));
+  std::vector<R> zpatmp((size_t)klon * klev, R(0));
+  for (int jk = 1; jk <= klev; ++jk)  // (:113-115)
+    for (int jl = kidia; jl <= kfdia; ++jl) z_t1[i2(jl, jk)] = a.ptm1[i2(jl, jk)] + a.ptsphy * a.pgtent[i2(jl, jk)];
+  satur2<R>(p, p2.lphylin != 0, kidia, kfdia, klon, a.ktdia, klev, a.papp1, z_t1.data(), zpqsat.data());
+  a.pqs = zpqsat.data();
+  R* clc_out = a.pclc;
+  a.pclc = zpatmp.data();
+  cloudsc2<R>(p, p2, a);
+  const R zcons = R(1) / a.ptsphy;  // (:154)
+  for (int jk = 1; jk <= klev; ++jk)
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      const size_t k = i2(jl, jk);
+      if (l.ptena) l.ptena[k] = zcons * (zpatmp[k] - l.pa[k]);
+      if (clc_out) clc_out[k] = zpatmp[k];
+      if (l.pqsat && jk >= a.ktdia) l.pqsat[k] = zpqsat[k];
+    }
+  for (int jk = 1; jk <= klev; ++jk)  // (:164-165), every column
+    for (int jl = 1; jl <= klon; ++jl) {
+      if (l.ptenr) l.ptenr[i2(jl, jk)] = R(0);
+      if (l.ptens) l.ptens[i2(jl, jk)] = R(0);
+    }
+  for (int jl = kidia; jl <= kfdia; ++jl) {  // (:168-169)
+    if (l.pprectype) l.pprectype[jl - 1] = R(0);
+    if (l.pfzra) l.pfzra[jl - 1] = R(0);
+  }
+}
+
 }  // namespace oracle

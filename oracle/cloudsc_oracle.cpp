@@ -13,6 +13,7 @@
 //   oracle_satadj_blocked_{dp,sp,mixed}  all blocks, OpenMP over blocks
 //   oracle_sltend_supersat_blocked_{dp,sp,mixed}  SLTEND's supersaturation adjustment, all blocks
 //   oracle_cloudsc2_{dp,sp}, oracle_cloudsc2_blocked_{dp,sp}  CLOUDSC2 (cloudsc2_oracle.hpp), one block / all blocks
+//   oracle_cloud_s_layer_blocked_{dp,sp}                     CLOUD_S_LAYER with LEPCLD2: SATUR, CLOUDSC2, wrap-up
 //   oracle_supersatcheck_*, oracle_supersatcheck_k_dp (with KFTLIQICE), oracle_lu5_dp,
 //   oracle_truncate_sinks_dp, oracle_thermo_dp, oracle_qsatmixadj_dp  unit-level entry points for the tests
 #include <omp.h>
@@ -390,9 +391,92 @@ int cloudsc2_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2, con
                              ptenq, pgtenq, ptenl, pgtenl, pteni, pgteni, pclc, pfplsl, pfplsn, pfhpsl, pfhpsn, \
                              pcovptot, nthreads)
 
+// CLOUD_S_LAYER (LEPCLD2) over all blocks of blocked host arrays: the CLOUD_S_LAYER_BLOCKED_ARGS order without
+// the stream, then the thread count.
+template <typename R>
+int cloud_s_layer_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2, const R* ceta, int ngptot,
+                          int nproma, int ktdia, int klev, int ngpblks, int ldrain1d, R tsphy, const R* prs1,
+                          const R* prsf1, const R* pq, const R* pt, const R* pa, const R* pl, const R* pi,
+                          const R* plude, const R* plu, const R* pmfu, const R* pmfd, const R* ptendency_t,
+                          const R* ptendency_q, const R* ptendency_l, const R* ptendency_i, R* ptendency_loc_t,
+                          R* ptendency_loc_q, R* ptendency_loc_a, R* ptendency_loc_l, R* ptendency_loc_i,
+                          R* ptendency_loc_r, R* ptendency_loc_s, R* pqsat, R* pclc, R* pfplsl, R* pfplsn,
+                          R* pfhpsl, R* pfhpsn, R* pcovptot, R* pprectype, R* pfzra, int nthreads) {
+  int rc = cloudsc2_check<R>(p, p2, 1, nproma, nproma, ktdia, klev);
+  if (rc) return rc;
+  if (ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma) return CLOUDSC_EDIMS;
+  if (ptendency_loc_a && !pa) return CLOUDSC_ENULL;
+  const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1);
+  auto off = [](auto* x, size_t n) { return x ? x + n : x; };
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int ibl = 0; ibl < ngpblks; ++ibl) {
+    const int kfdia = std::min(nproma, ngptot - ibl * nproma);
+    const size_t o2 = ibl * s2, oh = ibl * sh, o1 = (size_t)ibl * nproma;
+    oracle::Cloudsc2Args<R> a{1,
+                              kfdia,
+                              nproma,
+                              ktdia,
+                              klev,
+                              ldrain1d != 0,
+                              tsphy,
+                              ceta,
+                              prs1 + oh,
+                              prsf1 + o2,
+                              pq + o2,
+                              nullptr,
+                              pt + o2,
+                              pl + o2,
+                              pi + o2,
+                              plude + o2,
+                              plu + o2,
+                              pmfu + o2,
+                              pmfd + o2,
+                              ptendency_loc_t + o2,
+                              ptendency_t + o2,
+                              ptendency_loc_q + o2,
+                              ptendency_q + o2,
+                              ptendency_loc_l + o2,
+                              ptendency_l + o2,
+                              ptendency_loc_i + o2,
+                              ptendency_i + o2,
+                              off(pclc, o2),
+                              pfplsl + oh,
+                              pfplsn + oh,
+                              pfhpsl + oh,
+                              pfhpsn + oh,
+                              pcovptot + o2};
+    const oracle::LayerArgs<R> l{off(pa, o2),    off(ptendency_loc_a, o2), off(ptendency_loc_r, o2),
+                                 off(ptendency_loc_s, o2), off(pqsat, o2), off(pprectype, o1),
+                                 off(pfzra, o1)};
+    oracle::cloud_s_layer<R>(*p, *p2, a, l);
+  }
+  return CLOUDSC_OK;
+}
+
+#define ORACLE_LAYER_ARGS(R)                                                                                   \
+  const cloudsc_params_t *ydparams, const cloudsc2_params_t *ydparams2, const R *ceta, int ngptot, int nproma, \
+      int ktdia, int klev, int ngpblks, int ldrain1d, R tsphy, const R *prs1, const R *prsf1, const R *pq,      \
+      const R *pt, const R *pa, const R *pl, const R *pi, const R *plude, const R *plu, const R *pmfu,          \
+      const R *pmfd, const R *ptendency_t, const R *ptendency_q, const R *ptendency_l, const R *ptendency_i,   \
+      R *ptendency_loc_t, R *ptendency_loc_q, R *ptendency_loc_a, R *ptendency_loc_l, R *ptendency_loc_i,      \
+      R *ptendency_loc_r, R *ptendency_loc_s, R *pqsat, R *pclc, R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn,    \
+      R *pcovptot, R *pprectype, R *pfzra, int nthreads
+
+#define ORACLE_LAYER_CALL(R)                                                                                     \
+  return cloud_s_layer_blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d,      \
+                                  tsphy, prs1, prsf1, pq, pt, pa, pl, pi, plude, plu, pmfu, pmfd, ptendency_t,     \
+                                  ptendency_q, ptendency_l, ptendency_i, ptendency_loc_t, ptendency_loc_q,         \
+                                  ptendency_loc_a, ptendency_loc_l, ptendency_loc_i, ptendency_loc_r,              \
+                                  ptendency_loc_s, pqsat, pclc, pfplsl, pfplsn, pfhpsl, pfhpsn, pcovptot,          \
+                                  pprectype, pfzra, nthreads)
+
 }  // namespace
 
 extern "C" {
+
+int oracle_cloud_s_layer_blocked_dp(ORACLE_LAYER_ARGS(double)) { ORACLE_LAYER_CALL(double); }
+int oracle_cloud_s_layer_blocked_sp(ORACLE_LAYER_ARGS(float)) { ORACLE_LAYER_CALL(float); }
 
 int oracle_cloudsc2_dp(CLOUDSC2_ARGS(double)) { ORACLE_CLOUDSC2_CALL(double) }
 int oracle_cloudsc2_sp(CLOUDSC2_ARGS(float)) { ORACLE_CLOUDSC2_CALL(float) }
