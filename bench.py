@@ -384,6 +384,54 @@ def cloudsc2_bytes_per_column(klev, word=4):
     return word * (20 * klev + 5 * (klev + 1))
 
 
+def layer_bytes_per_column(klev, word=4):
+    """Algorithmic HBM bytes per column of CLOUD_S_LAYER (LEPCLD2) with every optional output: PRS1 and 14 level
+    inputs read, 10 level outputs (7 tendencies, PQSAT, PCLC, PCOVPTOT), 4 half-level fluxes and PPRECTYPE/PFZRA
+    written."""
+    return word * (14 * klev + (klev + 1) + 10 * klev + 4 * (klev + 1) + 2)
+
+
+def layer_bench(params, nproma, steps):
+    """CLOUD_S_LAYER (SATUR + CLOUDSC2 + wrap-up, one launch) in FP32 on 655,360 x L137, every optional output
+    written; kernel time from HIP events on the launch stream."""
+    import numpy as np
+    import torch
+
+    from openifs_amd import cloudsc2 as C2
+
+    q2 = C2.default_params2(params)
+    src = C2.synth_device(params, 20261207, NGPTOT, nproma, KLEV, "sp", "cuda:0")
+    f = {lay: src[c2] for lay, c2 in C2.LAYER_FROM_CLOUDSC2.items()}
+    f["pa"] = torch.rand(src["pqm1"].shape, device="cuda:0")
+    ngpblks = (NGPTOT + nproma - 1) // nproma
+    for name, kind, role, opt in C2.LAYER_ARRAYS:
+        if role == "o":
+            f[name] = torch.empty(C2.layer_shape(kind, nproma, KLEV, ngpblks), device="cuda:0")
+    ceta = torch.from_numpy(C2.reference_eta(KLEV)).to("cuda:0", torch.float32)
+    stream = torch.cuda.current_stream()
+    run = lambda: C2.cloud_s_layer_blocked(f, params, q2, ceta, NGPTOT, nproma, KLEV, 900.0, "sp",
+                                           stream=stream.cuda_stream)
+    for _ in range(2):
+        run()
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for a, b in ev:
+        a.record(stream)
+        run()
+        b.record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    del f, src
+    torch.cuda.empty_cache()
+    bpc = layer_bytes_per_column(KLEV)
+    gbs = bpc * NGPTOT / (ms / 1e3) / 1e9
+    return {"kernel_ms": ms, "columns_per_s": NGPTOT / (ms / 1e3), "algorithmic_bytes_per_column": bpc,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+            "workload": f"CLOUD_S_LAYER FP32 (SATUR, CLOUDSC2, wrap-up), NLEV={KLEV}, {NGPTOT} columns, "
+                        f"NPROMA={nproma}"}
+
+
 def cloudsc2_bench(params, nproma, steps, cpu_threads, cpu_seconds):
     """CLOUDSC2 (the simplified scheme of the linearised physics) in FP32 on 655,360 x L137: kernel time from HIP
     events on the launch stream, algorithmic bytes, accuracy of the first 2048 columns against the FP64 oracle, and
@@ -446,6 +494,7 @@ def cloudsc2_bench(params, nproma, steps, cpu_threads, cpu_seconds):
         res["cpu_baseline"] = {"value": passes * nb * nproma / dt, "unit": "columns/s", "cores": cpu_threads,
                                "kind": "port", "sample": f"{nb * nproma} columns x {passes} passes ({dt:.1f} s), "
                                                          "FP32 oracle, OpenMP over NPROMA blocks"}
+    res["layer"] = layer_bench(params, nproma, steps)
     return res
 
 
