@@ -29,6 +29,8 @@ typedef struct cloudsc2_params_t {
   double rvtmp2;   /* TTHF: RCPV/RCPD-1, set to 0 (suphec.F90:229) */
   int32_t lphylin;  /* TEPHLI: linearised-physics thermodynamics (CUADJTQS, tanh phase function) */
   int32_t levapls2; /* TPHNC: evaporation of large-scale precipitation */
+  int32_t lepcld2;  /* TPHNC: CLOUD_S_LAYER runs CLOUDSC2 (prognostic) if set, CLOUDST (diagnostic) if not */
+  int32_t pad_;
 } cloudsc2_params_t;
 
 /* Defaults of the TECLDP/TEPHLI/TPHNC/TTHF components above for the given cloudsc_params_t (RTICE, RTWAT). */
@@ -62,13 +64,31 @@ int cloudsc2_sp_c(CLOUDSC2_ARGS(float));
 int cloudsc2_blocked_dp_c(CLOUDSC2_BLOCKED_ARGS(double));
 int cloudsc2_blocked_sp_c(CLOUDSC2_BLOCKED_ARGS(float));
 
-/* CLOUD_S_LAYER with LEPCLD2 (cloud_s_layer.F90:112-176), all blocks of device-resident blocked fields in one
- * launch: SATUR (KFLAG 2) at PT + TSPHY*PTENDENCY_T gives PQSAT, CLOUDSC2 runs on it, and the wrap-up writes
- * PTENDENCY_LOC_A = (PCLC - PA)/TSPHY on every level, zero rain and snow tendencies on every column and level,
- * and zero PPRECTYPE and PFZRA.  Arguments as CLOUD_S_LAYER passes them to CLOUDSC2: PRS1 = PAPHP1, PRSF1 = PAPP1,
- * PQ = PQM1, PT = PTM1, PTENDENCY_* = PGTEN*, PTENDENCY_LOC_* = PTEN*; PQSAT is PDIAG%PQSAT, PCLC is AUXL%ZPATMP.
- * PTENDENCY_LOC_A (which needs PA), PTENDENCY_LOC_R/S, PQSAT, PCLC, PPRECTYPE (NPROMA,NGPBLKS) and PFZRA may be
- * NULL where the caller's key mask has no such field.  The LEPCLD2 = false branch (CLOUDST) is not built. */
+/* CLOUDST (cloudst.F90:9-16), the diagnostic scheme, over all blocks of device-resident blocked fields:
+ * CLOUDSC2's arrays without the condensate and mass-flux ones, plus the diagnostic ice and liquid water PQIWP and
+ * PQLWP; every array is required. */
+#define CLOUDST_BLOCKED_ARGS(R)                                                                         \
+  const cloudsc_params_t *ydparams, const cloudsc2_params_t *ydparams2, const R *ceta, int ngptot,       \
+      int nproma, int ktdia, int klev, int ngpblks, int ldrain1d, R ptsphy, const R *paphp1,            \
+      const R *papp1, const R *pqm1, const R *pqs, const R *ptm1, const R *plude, const R *plu,         \
+      R *ptent, const R *pgtent, R *ptenq, const R *pgtenq, R *pclc, R *pqiwp, R *pqlwp, R *pfplsl,     \
+      R *pfplsn, R *pfhpsl, R *pfhpsn, R *pcovptot, void *stream
+
+int cloudst_blocked_dp_c(CLOUDST_BLOCKED_ARGS(double));
+int cloudst_blocked_sp_c(CLOUDST_BLOCKED_ARGS(float));
+
+/* CLOUD_S_LAYER (cloud_s_layer.F90:112-176), all blocks of device-resident blocked fields in one launch.
+ * - SATUR (KFLAG 2) at PT + TSPHY*PTENDENCY_T gives PQSAT.
+ * - The scheme runs on it: CLOUDSC2 if ydparams2->lepcld2, else CLOUDST.
+ * - The wrap-up writes PTENDENCY_LOC_A = (PCLC - PA)/TSPHY on every level, zero rain and snow tendencies on every
+ *   column and level, and zero PPRECTYPE and PFZRA. With CLOUDST it also writes PTENDENCY_LOC_L/I as
+ *   (PQLWP - PL)/TSPHY and (PQIWP - PI)/TSPHY.
+ * Arguments are as CLOUD_S_LAYER passes them to CLOUDSC2: PRS1 = PAPHP1, PRSF1 = PAPP1, PQ = PQM1, PT = PTM1,
+ * PTENDENCY_* = PGTEN*, PTENDENCY_LOC_* = PTEN*. PQSAT is PDIAG%PQSAT; PCLC, PQIWP and PQLWP are AUXL%ZPATMP,
+ * ZPLSMP and ZPLTMP.
+ * These may be NULL where the caller's key mask has no such field: PTENDENCY_LOC_A (which needs PA),
+ * PTENDENCY_LOC_R/S, PQSAT, PCLC, PPRECTYPE and PFZRA (both (NPROMA,NGPBLKS)), and PQIWP and PQLWP.
+ * PMFU, PMFD and PTENDENCY_L/I are read by CLOUDSC2 only. */
 #define CLOUD_S_LAYER_BLOCKED_ARGS(R)                                                                   \
   const cloudsc_params_t *ydparams, const cloudsc2_params_t *ydparams2, const R *ceta, int ngptot,       \
       int nproma, int ktdia, int klev, int ngpblks, int ldrain1d, R tsphy, const R *prs1,              \
@@ -77,7 +97,7 @@ int cloudsc2_blocked_sp_c(CLOUDSC2_BLOCKED_ARGS(float));
       const R *ptendency_l, const R *ptendency_i, R *ptendency_loc_t, R *ptendency_loc_q,              \
       R *ptendency_loc_a, R *ptendency_loc_l, R *ptendency_loc_i, R *ptendency_loc_r,                  \
       R *ptendency_loc_s, R *pqsat, R *pclc, R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, R *pcovptot,  \
-      R *pprectype, R *pfzra, void *stream
+      R *pprectype, R *pfzra, R *pqiwp, R *pqlwp, void *stream
 
 int cloud_s_layer_blocked_dp_c(CLOUD_S_LAYER_BLOCKED_ARGS(double));
 int cloud_s_layer_blocked_sp_c(CLOUD_S_LAYER_BLOCKED_ARGS(float));

@@ -19,7 +19,7 @@ class Cloudsc2Params(ctypes.Structure):
 
     _fields_ = [("rclcrit", ctypes.c_double), ("rlptrc", ctypes.c_double), ("rlpal1", ctypes.c_double),
                 ("rlpal2", ctypes.c_double), ("rvtmp2", ctypes.c_double), ("lphylin", ctypes.c_int32),
-                ("levapls2", ctypes.c_int32)]
+                ("levapls2", ctypes.c_int32), ("lepcld2", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 def default_params2(params):
@@ -175,15 +175,41 @@ def synth_device(params, seed, ngptot, nproma, klev, precision="sp", device="cud
 LAYER_ARRAYS = [
     ("prs1", "h", "i", False), ("prsf1", "2d", "i", False), ("pq", "2d", "i", False), ("pt", "2d", "i", False),
     ("pa", "2d", "i", True), ("pl", "2d", "i", False), ("pi", "2d", "i", False), ("plude", "2d", "i", False),
-    ("plu", "2d", "i", False), ("pmfu", "2d", "i", False), ("pmfd", "2d", "i", False),
-    ("ptendency_t", "2d", "i", False), ("ptendency_q", "2d", "i", False), ("ptendency_l", "2d", "i", False),
-    ("ptendency_i", "2d", "i", False), ("ptendency_loc_t", "2d", "o", False), ("ptendency_loc_q", "2d", "o", False),
+    ("plu", "2d", "i", False), ("pmfu", "2d", "i", True), ("pmfd", "2d", "i", True),
+    ("ptendency_t", "2d", "i", False), ("ptendency_q", "2d", "i", False), ("ptendency_l", "2d", "i", True),
+    ("ptendency_i", "2d", "i", True), ("ptendency_loc_t", "2d", "o", False), ("ptendency_loc_q", "2d", "o", False),
     ("ptendency_loc_a", "2d", "o", True), ("ptendency_loc_l", "2d", "o", False), ("ptendency_loc_i", "2d", "o", False),
     ("ptendency_loc_r", "2d", "o", True), ("ptendency_loc_s", "2d", "o", True), ("pqsat", "2d", "o", True),
     ("pclc", "2d", "o", True), ("pfplsl", "h", "o", False), ("pfplsn", "h", "o", False), ("pfhpsl", "h", "o", False),
     ("pfhpsn", "h", "o", False), ("pcovptot", "2d", "o", False), ("pprectype", "1d", "o", True),
-    ("pfzra", "1d", "o", True),
+    ("pfzra", "1d", "o", True), ("pqiwp", "2d", "o", True), ("pqlwp", "2d", "o", True),
 ]
+# PMFU, PMFD and PTENDENCY_L/I are read only when LEPCLD2 selects CLOUDSC2
+
+# CLOUDST (cloudst.F90:9-16) arrays after PTSPHY, in the order of CLOUDST_BLOCKED_ARGS: (name, kind, role)
+CLOUDST_ARRAYS = [
+    ("paphp1", "h", "i"), ("papp1", "2d", "i"), ("pqm1", "2d", "i"), ("pqs", "2d", "i"), ("ptm1", "2d", "i"),
+    ("plude", "2d", "i"), ("plu", "2d", "i"), ("ptent", "2d", "o"), ("pgtent", "2d", "i"), ("ptenq", "2d", "o"),
+    ("pgtenq", "2d", "i"), ("pclc", "2d", "o"), ("pqiwp", "2d", "o"), ("pqlwp", "2d", "o"), ("pfplsl", "h", "o"),
+    ("pfplsn", "h", "o"), ("pfhpsl", "h", "o"), ("pfhpsn", "h", "o"), ("pcovptot", "2d", "o"),
+]
+
+
+def cloudst_blocked(fields, params, params2, ceta, ngptot, nproma, klev, ptsphy, precision="sp", ktdia=1,
+                    ldrain1d=False, stream=None):
+    """One launch of cloudst_blocked_{dp,sp}_c on device tensors in blocked layout named as CLOUDST_ARRAYS."""
+    import torch
+
+    ngpblks = (ngptot + nproma - 1) // nproma
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.data_ptr()), ctypes.c_int(ngptot),
+            ctypes.c_int(nproma), ctypes.c_int(ktdia), ctypes.c_int(klev), ctypes.c_int(ngpblks),
+            ctypes.c_int(int(bool(ldrain1d))), _ctype_real(precision)(ptsphy)]
+    args += [ctypes.c_void_p(fields[n].data_ptr()) for n, _, _ in CLOUDST_ARRAYS]
+    args.append(ctypes.c_void_p(stream))
+    fn = f"cloudst_blocked_{precision}_c"
+    C._check(fn, getattr(C.lib(), fn)(*args))
 # how CLOUD_S_LAYER passes its fields to CLOUDSC2 (cloud_s_layer.F90:126-139)
 LAYER_FROM_CLOUDSC2 = {"prs1": "paphp1", "prsf1": "papp1", "pq": "pqm1", "pt": "ptm1", "pl": "pl", "pi": "pi",
                        "plude": "plude", "plu": "plu", "pmfu": "pmfu", "pmfd": "pmfd", "ptendency_t": "pgtent",

@@ -51,6 +51,7 @@ struct C2Ptrs {
   // CLOUD_S_LAYER only (LAYER): the cloud fraction and the wrap-up outputs, NULL where the caller has no such field
   const R* pa;
   R *ptena, *ptenr, *ptens, *pqsat, *pprectype, *pfzra;
+  R *pqiwp, *pqlwp;  // CLOUDST: diagnostic ice and liquid water (AUXL%ZPLSMP, ZPLTMP in CLOUD_S_LAYER)
 };
 
 struct C2Dims {
@@ -433,6 +434,257 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudsc2_kernel(const C2Const<
   }
 }
 
+// CLOUDST (cloudst.F90:9-641), the diagnostic scheme: cover from the vapour alone with its own critical humidity
+// profile, no subsidence term, liquid autoconversion only, tendencies from the precipitation flux divergence and the
+// diagnostic water paths PQLWP/PQIWP.  One column per lane, as cloudsc2_kernel.  LAYER: CLOUD_S_LAYER with
+// LEPCLD2 false (cloud_s_layer.F90:112-176): PQS from SATUR, and the A/L/I tendencies (ZPATMP/ZPLTMP/ZPLSMP minus
+// the state)/TSPHY, zero rain and snow tendencies and zero PPRECTYPE/PFZRA.
+template <typename R, bool UNI, bool LAYER>
+__global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudst_kernel(const C2Const<R> c, const C2Ptrs<R> p,
+                                                                 const C2Dims d) {
+  const uint32_t g = blockIdx.x * CLOUDSC2_BLOCK + threadIdx.x;
+  const uint32_t nproma = (uint32_t)d.nproma;
+  const int klev = d.klev;
+  if (g >= (uint32_t)d.ngpblks * nproma) return;
+  uint32_t b = g / nproma;
+  if (UNI) b = __builtin_amdgcn_readfirstlane(b);
+  const uint32_t jl = g - b * nproma;
+  constexpr uint32_t E = sizeof(R);
+  const uint32_t jo = jl * E;
+  const uint32_t lstep = nproma * E;
+  const uint32_t b2 = b * nproma * (uint32_t)klev * E;
+  const uint32_t bh = b2 + b * lstep;
+  auto at2 = [&](int k) { return b2 + (uint32_t)k * lstep; };
+  auto ath = [&](int k) { return bh + (uint32_t)k * lstep; };
+  if constexpr (LAYER) {  // (:164-165), every column of the block
+    for (int k = 0; k < klev; ++k) {
+      if (p.ptenr) ST(p.ptenr, at2(k), jo, R(0));
+      if (p.ptens) ST(p.ptens, at2(k), jo, R(0));
+    }
+  }
+  const int kfdia = std::min(d.kfdia, d.ngptot - (int)(b * nproma));
+  if ((int)jl < d.kidia - 1 || (int)jl >= kfdia) return;
+  const R ptsphy = c.ptsphy;
+  const R zeps1 = R(1.e-12), zeps2 = R(1.e-10), zqmax = R(0.5);
+  const R zcons3 = R(2.5E6) / R(1005.);  // (:215)
+  const R zpecons = R(1.E-5);           // (:217)
+  if constexpr (LAYER) {
+    if (p.pprectype) ST(p.pprectype, b * lstep, jo, R(0));
+    if (p.pfzra) ST(p.pfzra, b * lstep, jo, R(0));
+  }
+  // eta of the tropopause (:234-247)
+  R ztrpaus = R(0.1);
+  for (int k = 0; k < klev - 1; ++k) {
+    const R ceta = p.ceta[k];
+    if (ceta > R(0.1) && ceta < R(0.4)) {
+      const R t0 = LD(p.ptm1, at2(k), jo) + ptsphy * LD(p.pgtent, at2(k), jo);
+      const R t1 = LD(p.ptm1, at2(k + 1), jo) + ptsphy * LD(p.pgtent, at2(k + 1), jo);
+      if (t0 > t1) ztrpaus = ceta;
+    }
+  }
+  // outputs above KTDIA (:311-330): cover, water paths and PCOVPTOT zero; fluxes zero at the top
+  auto wrap_state = [&](int k, R clc, R lwp, R iwp) {  // CLOUD_S_LAYER's tendencies from ZPATMP/ZPLTMP/ZPLSMP
+    if constexpr (LAYER) {
+      const uint32_t j = at2(k);
+      if (p.ptena) ST(p.ptena, j, jo, c.zqtmst * (clc - LD(p.pa, j, jo)));
+      ST(p.ptenl, j, jo, c.zqtmst * (lwp - LD(p.pl, j, jo)));
+      ST(p.pteni, j, jo, c.zqtmst * (iwp - LD(p.pi, j, jo)));
+    }
+  };
+  for (int k = 0; k < d.ktdia - 1; ++k) {
+    if (!LAYER || p.pclc) ST(p.pclc, at2(k), jo, R(0));
+    if (!LAYER || p.pqiwp) ST(p.pqiwp, at2(k), jo, R(0));
+    if (!LAYER || p.pqlwp) ST(p.pqlwp, at2(k), jo, R(0));
+    ST(p.pcovptot, at2(k), jo, R(0));
+    wrap_state(k, R(0), R(0), R(0));
+  }
+  ST(p.pfplsl, ath(0), jo, R(0));
+  ST(p.pfplsn, ath(0), jo, R(0));
+  ST(p.pfhpsl, ath(0), jo, -R(0) * c.rlvtt);  // (:629-634)
+  ST(p.pfhpsn, ath(0), jo, -R(0) * c.rlstt);
+  for (int k = 1; k < d.ktdia; ++k) {
+    ST(p.pfhpsl, ath(k), jo, -LD(p.pfplsl, ath(k), jo) * c.rlvtt);
+    ST(p.pfhpsn, ath(k), jo, -LD(p.pfplsn, ath(k), jo) * c.rlstt);
+  }
+  const R paph_surf = LD(p.paphp1, ath(klev), jo);
+
+  R zrfl = R(0), zsfl = R(0), zcovptot = R(0);
+  R paph0 = LD(p.paphp1, ath(d.ktdia - 1), jo);
+  for (int k = d.ktdia - 1; k < klev; ++k) {
+    const uint32_t i2 = at2(k);
+    const R ceta = p.ceta[k];
+    R ztp1 = LD(p.ptm1, i2, jo) + ptsphy * LD(p.pgtent, i2, jo);
+    R zqp1 = LD(p.pqm1, i2, jo) + ptsphy * LD(p.pgtenq, i2, jo);
+    const R paph1 = LD(p.paphp1, ath(k + 1), jo);
+    const R zdp = paph1 - paph0;
+    const R papp1 = LD(p.papp1, i2, jo), plude = LD(p.plude, i2, jo);
+    const R pqs = LAYER ? satur(c, ztp1, papp1) : LD(p.pqs, i2, jo);
+    if constexpr (LAYER) {
+      if (p.pqsat) ST(p.pqsat, i2, jo, pqs);
+    }
+    // critical relative humidity (:263-290)
+    R zcrh = R(0);
+    {
+      const R zeta3 = ztrpaus, zrh1 = R(0.9), zrh2 = R(0.5), zrh3 = R(1.0), zeta1 = R(0.75);
+      const R zdeta1 = R(1.0) - zeta1;
+      if (ceta < zeta3) {
+        zcrh = zrh3 + m_div((zrh2 - zrh3) * ceta, zeta3);
+      } else if (ceta >= zeta3 && ceta < zeta1) {
+        zcrh = zrh2;
+      } else if (ceta >= zeta1) {
+        zcrh = zrh1 + (zrh2 - zrh1) * m_sqrt(m_div(R(1.0) - ceta, zdeta1));
+      }
+    }
+    const R zscalm = R(0.9) * m_pow(m_max(ceta - R(0.2), zeps1), R(0.2));
+    const R zzz = m_div(R(1), c.rcpd + c.rcpd * c.rvtmp2 * zqp1);
+    const R zlfdcp = c.rlmlt * zzz, zlsdcp = c.rlstt * zzz, zlvdcp = c.rlvtt * zzz;
+    // (:342-385)
+    R zalfaw, zesdp;
+    if (c.lin) {
+      const R zoealfaw = R(0.545) * (tanh(R(0.17) * (ztp1 - c.rlptrc)) + R(1.0));
+      const bool cold = ztp1 < c.rtt;
+      zalfaw = cold ? zoealfaw : R(1.0);
+      const R z3es = cold ? c.r3ies : c.r3les, z4es = cold ? c.r4ies : c.r4les;
+      zesdp = m_div(c.r2es * m_exp(m_div(z3es * (ztp1 - c.rtt), ztp1 - z4es)), papp1);
+      if (zesdp > zqmax) zesdp = zqmax;
+    } else {
+      zalfaw = foealfa(c, ztp1);
+      zesdp = m_div(foeewm(c, ztp1, zalfaw), papp1);
+    }
+    const R dl = ztp1 - c.r4les, di = ztp1 - c.r4ies;
+    const R zfac = zalfaw * m_div(c.r5les, dl * dl) + (R(1.0) - zalfaw) * m_div(c.r5ies, di * di);
+    const R zcor = m_div(R(1.0), R(1.0) - c.retv * zesdp);
+    const R zdqsdtemp = zfac * zcor * pqs;
+    const R zcorqs = R(1.0) + zcons3 * zdqsdtemp;
+    const R zqlim = zqp1 > pqs ? pqs : zqp1;
+    const R zqcrit = zcrh * pqs;
+    // layer cloud amounts (:387-403)
+    R pclc, zqlwc;
+    if (zqp1 <= zqcrit) {
+      pclc = R(0.0);
+      zqlwc = R(0.0);
+    } else if (zqp1 >= pqs) {
+      pclc = R(1.0);
+      zqlwc = (R(1.0) - zscalm) * (pqs - zqcrit);
+    } else {
+      const R zqpd = pqs - zqp1, zqcd = pqs - zqcrit;
+      pclc = R(1.0) - m_sqrt(m_div(zqpd, zqcd - zscalm * (zqp1 - zqcrit)));
+      zqlwc = (zscalm * zqpd + (R(1.0) - zscalm) * zqcd) * (pclc * pclc);
+    }
+    // convective cloud (:407-424)
+    const R zgdp = m_div(c.rg, paph1 - paph0);
+    const R zlude = plude * ptsphy * zgdp;
+    if (k < klev - 1) {
+      const R plu_n = LD(p.plu, i2 + lstep, jo);
+      if (zlude >= c.rlmin && plu_n >= zeps2) {
+        pclc = pclc + (R(1.0) - pclc) * (R(1.0) - m_exp(-m_div(zlude, plu_n)));
+        zqlwc = zqlwc + zlude;
+      }
+    }
+    zcovptot = m_max(zcovptot, pclc);  // (:428-434)
+    const R zcovpclr = m_max(zcovptot - pclc, R(0.0));
+    // melting of incoming snow (:448-460)
+    R zrfln, zsfln;
+    if (zsfl != R(0.0)) {
+      const R zcons = m_div(c.zcons2 * zdp, zlfdcp);
+      const R zsnmlt = m_min(zsfl, zcons * m_max(R(0.0), ztp1 - c.zmeltp2));
+      zrfln = zrfl + zsnmlt;
+      zsfln = zsfl - zsnmlt;
+      ztp1 = ztp1 - m_div(zsnmlt, zcons);
+    } else {
+      zrfln = zrfl;
+      zsfln = zsfl;
+    }
+    // precipitation (:462-542)
+    R zpr = R(0.0);
+    if (pclc > zeps2) {
+      const R zlcrit = c.evap ? c.rclcrit : c.rclcrit * R(2.);
+      const R zcld = m_div(zqlwc, pclc);
+      const R r = m_div(zcld, zlcrit);
+      const R zd = c.zckcodtl * (R(1.0) - m_exp(-(r * r)));
+      const R zlnew = pclc * zcld * m_exp(-zd);
+      zpr = zqlwc - zlnew;
+      zqlwc = zqlwc - zpr;
+    }
+    R zdr = c.zcons2 * zdp * zpr;
+    R zfwat;
+    if (c.lin) {
+      const R zoealfa = R(0.545) * (tanh(R(0.17) * (ztp1 - c.rlptrc)) + R(1.0));
+      zfwat = ztp1 < c.rtt ? zoealfa : R(1.0);
+    } else {
+      zfwat = foealfa(c, ztp1);
+    }
+    R zrfreeze = R(0), zfwatr;
+    if (ztp1 < c.rtt) {
+      zrfreeze = zfwat * zdr;
+      zfwatr = R(0.0);
+    } else {
+      zfwatr = R(1.0);
+    }
+    zrfln = zrfln + zfwatr * zdr;
+    zsfln = zsfln + (R(1.0) - zfwatr) * zdr;
+    R pcovptot = R(0);
+    const R zprtot = zrfln + zsfln;
+    if (zprtot > zeps2 && zcovpclr > zeps2 && c.evap) {
+      R zpreclr = m_div(zprtot * zcovpclr, zcovptot);
+      const R om = R(1.0) - pclc;
+      const R zqe = pqs - m_div((pqs - zqlim) * zcovpclr, om * om);
+      const R zbeta = c.rg * zpecons *
+                      m_pow(m_div(m_div(m_sqrt(m_div(papp1, paph_surf)), R(5.09E-3)) * zpreclr, zcovpclr), R(0.5777));
+      const R zb = m_div(ptsphy * zbeta * (pqs - zqe), R(1.0) + zbeta * ptsphy * zcorqs);
+      const R zdtgdp = m_div(ptsphy * c.rg, paph1 - paph0);
+      R zdpr = m_div(zcovpclr * zb, zdtgdp);
+      zdpr = m_min(zdpr, zpreclr);
+      zpreclr = zpreclr - zdpr;
+      if (zpreclr <= R(0.0)) zcovptot = pclc;
+      pcovptot = zcovptot;
+      const R zdprsfl = m_div(zdpr * zrfln, zprtot);
+      zrfln = zrfln - zdprsfl;
+      const R zdpssfl = m_div(zdpr * zsfln, zprtot);
+      zsfln = zsfln - zdpssfl;
+    }
+    const R pqlwp = zfwat * zqlwc, pqiwp = (R(1.0) - zfwat) * zqlwc;
+    // tendencies from the flux divergence, first guess and the final adjustment (:546-616)
+    auto tendency_q = [&](R rfln, R sfln) {
+      return -((rfln - zrfl) + (sfln - zsfl)) * m_div(c.rg, zdp) + plude * zgdp;
+    };
+    auto tendency_t = [&](R rfln, R sfln, R rfreeze) {
+      return (zlvdcp * (rfln - zrfl) + zlsdcp * (sfln - zsfl)) * m_div(c.rg, zdp) -
+             plude * zgdp * (zfwat * zlvdcp + (R(1.0) - zfwat) * zlsdcp) + (zlsdcp - zlvdcp) * rfreeze * zgdp;
+    };
+    ztp1 = ztp1 + ptsphy * tendency_t(zrfln, zsfln, zrfreeze);
+    zqp1 = zqp1 + ptsphy * tendency_q(zrfln, zsfln);
+    const R zqold = zqp1;
+    adjust(c, papp1, ztp1, zqp1);
+    const R zdq = m_max(R(0.0), zqold - zqp1);
+    zdr = c.zcons2 * zdp * zdq;
+    if (ztp1 < c.rtt) {
+      zrfreeze = zrfreeze + zfwat * zdr;
+      zfwatr = R(0.0);
+    } else {
+      zfwatr = R(1.0);
+    }
+    zrfln = zrfln + zfwatr * zdr;
+    zsfln = zsfln + (R(1.0) - zfwatr) * zdr;
+
+    ST(p.ptenq, i2, jo, tendency_q(zrfln, zsfln));
+    ST(p.ptent, i2, jo, tendency_t(zrfln, zsfln, zrfreeze));
+    if (!LAYER || p.pclc) ST(p.pclc, i2, jo, pclc);
+    if (!LAYER || p.pqlwp) ST(p.pqlwp, i2, jo, pqlwp);
+    if (!LAYER || p.pqiwp) ST(p.pqiwp, i2, jo, pqiwp);
+    ST(p.pcovptot, i2, jo, pcovptot);
+    wrap_state(k, pclc, pqlwp, pqiwp);
+    const uint32_t ih = ath(k + 1);
+    ST(p.pfplsl, ih, jo, zrfln);
+    ST(p.pfplsn, ih, jo, zsfln);
+    ST(p.pfhpsl, ih, jo, -zrfln * c.rlvtt);
+    ST(p.pfhpsn, ih, jo, -zsfln * c.rlstt);
+    zrfl = zrfln;
+    zsfl = zsfln;
+    paph0 = paph1;
+  }
+}
+
 template <typename R>
 C2Const<R> make_const(const cloudsc_params_t& q, const cloudsc2_params_t& q2, double ptsphy, bool ldrain1d) {
   C2Const<R> c{};
@@ -469,20 +721,23 @@ template <typename R>
 C2Ptrs<R> offset_ptrs(const C2Ptrs<R>& p, int64_t b0, int nproma, int klev) {
   const int64_t s2 = b0 * nproma * klev, sh = b0 * nproma * (klev + 1);
   C2Ptrs<R> o = p;
-  o.paphp1 += sh; o.papp1 += s2; o.pqm1 += s2; o.pqs = o.pqs ? o.pqs + s2 : o.pqs; o.ptm1 += s2; o.pl += s2; o.pi += s2;
-  o.plude += s2; o.plu += s2; o.pmfu += s2; o.pmfd += s2; o.ptent += s2; o.pgtent += s2; o.ptenq += s2;
-  o.pgtenq += s2; o.ptenl += s2; o.pgtenl += s2; o.pteni += s2; o.pgteni += s2; o.pfplsl += sh;
+  auto adv0 = [](auto* x, int64_t n) { return x ? x + n : x; };
+  o.paphp1 += sh; o.papp1 += s2; o.pqm1 += s2; o.pqs = adv0(o.pqs, s2); o.ptm1 += s2; o.pl = adv0(o.pl, s2);
+  o.pi = adv0(o.pi, s2); o.plude += s2; o.plu += s2; o.pmfu = adv0(o.pmfu, s2); o.pmfd = adv0(o.pmfd, s2);
+  o.ptent += s2; o.pgtent += s2; o.ptenq += s2; o.pgtenq += s2; o.ptenl = adv0(o.ptenl, s2);
+  o.pgtenl = adv0(o.pgtenl, s2); o.pteni = adv0(o.pteni, s2); o.pgteni = adv0(o.pgteni, s2); o.pfplsl += sh;
   o.pfplsn += sh; o.pfhpsl += sh; o.pfhpsn += sh; o.pcovptot += s2;
   auto adv = [](auto* x, int64_t n) { return x ? x + n : x; };  // optional arrays stay NULL
   o.pa = adv(o.pa, s2); o.ptena = adv(o.ptena, s2); o.ptenr = adv(o.ptenr, s2); o.ptens = adv(o.ptens, s2);
   o.pqsat = adv(o.pqsat, s2); o.pclc = adv(o.pclc, s2);
   o.pprectype = adv(o.pprectype, b0 * nproma); o.pfzra = adv(o.pfzra, b0 * nproma);
+  o.pqiwp = adv(o.pqiwp, s2); o.pqlwp = adv(o.pqlwp, s2);
   return o;
 }
 
 template <typename R>
 hipError_t launch(const cloudsc_params_t& q, const cloudsc2_params_t& q2, bool ldrain1d, double ptsphy,
-                  const C2Ptrs<R>& p, const C2Dims& d, hipStream_t stream, bool layer = false) {
+                  const C2Ptrs<R>& p, const C2Dims& d, hipStream_t stream, bool layer = false, bool st = false) {
   const C2Const<R> c = make_const<R>(q, q2, ptsphy, ldrain1d);
   const bool uni = d.nproma % 64 == 0;
   // blocks per launch: every (NPROMA,KLEV+1) array of one launch stays below 4 GiB (32-bit byte offsets)
@@ -495,7 +750,15 @@ hipError_t launch(const cloudsc_params_t& q, const cloudsc2_params_t& q2, bool l
     dc.ngptot = (int)(d.ngptot - b0 * d.nproma);
     const C2Ptrs<R> pc = offset_ptrs(p, b0, d.nproma, d.klev);
     const dim3 grid((unsigned)((nb * d.nproma + CLOUDSC2_BLOCK - 1) / CLOUDSC2_BLOCK));
-    if (layer && uni)
+    if (st && layer && uni)
+      hipLaunchKernelGGL((cloudst_kernel<R, true, true>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+    else if (st && layer)
+      hipLaunchKernelGGL((cloudst_kernel<R, false, true>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+    else if (st && uni)
+      hipLaunchKernelGGL((cloudst_kernel<R, true, false>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+    else if (st)
+      hipLaunchKernelGGL((cloudst_kernel<R, false, false>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
+    else if (layer && uni)
       hipLaunchKernelGGL((cloudsc2_kernel<R, true, true>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
     else if (layer)
       hipLaunchKernelGGL((cloudsc2_kernel<R, false, true>), grid, dim3(CLOUDSC2_BLOCK), 0, stream, c, pc, dc);
@@ -620,17 +883,41 @@ int layer_blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const 
   if (rc) return rc;
   if (nproma < 1 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma)
     return fail(CLOUDSC_EDIMS, "cloud_s_layer_blocked: need NGPBLKS = ceil(NGPTOT/NPROMA)");
-  const void* req[] = {p.ceta,  p.paphp1, p.papp1, p.pqm1,   p.ptm1,   p.pl,     p.pi,     p.plude,  p.plu,
-                       p.pmfu,  p.pmfd,   p.ptent, p.pgtent, p.ptenq,  p.pgtenq, p.ptenl,  p.pgtenl, p.pteni,
-                       p.pgteni, p.pfplsl, p.pfplsn, p.pfhpsl, p.pfhpsn, p.pcovptot};
+  const bool c2 = q2->lepcld2 != 0;  // CLOUDSC2 (prognostic) or CLOUDST (diagnostic) (:121-150)
+  const void* req[] = {p.ceta,  p.paphp1, p.papp1,  p.pqm1,   p.ptm1,   p.pl,     p.pi,     p.plude,
+                       p.plu,   p.ptent,  p.pgtent, p.ptenq,  p.pgtenq, p.ptenl,  p.pteni,  p.pfplsl,
+                       p.pfplsn, p.pfhpsl, p.pfhpsn, p.pcovptot};
   for (const void* x : req)
     if (!x) return fail(CLOUDSC_ENULL, "cloud_s_layer_blocked: NULL array");
+  if (c2 && (!p.pmfu || !p.pmfd || !p.pgtenl || !p.pgteni))
+    return fail(CLOUDSC_ENULL, "cloud_s_layer_blocked: LEPCLD2 needs PMFU, PMFD and the condensate tendencies");
   if (p.ptena && !p.pa) return fail(CLOUDSC_ENULL, "cloud_s_layer_blocked: PTENDENCY_LOC_A needs PA");
   rc = cloudsc_capi::check_device();
   if (rc) return rc;
   C2Dims d{ngptot, nproma, klev, ngpblks, ktdia, 1, nproma};
-  hipError_t e = launch<R>(*q, *q2, ldrain1d != 0, (double)tsphy, p, d, (hipStream_t)stream, true);
+  hipError_t e = launch<R>(*q, *q2, ldrain1d != 0, (double)tsphy, p, d, (hipStream_t)stream, true, !c2);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloud_s_layer_blocked: kernel launch", e);
+  return CLOUDSC_OK;
+}
+
+// CLOUDST over all blocks of device-resident blocked fields.
+template <typename R>
+int cloudst_blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, int ngptot, int nproma, int ktdia,
+                    int klev, int ngpblks, int ldrain1d, R ptsphy, const C2Ptrs<R>& p, void* stream) {
+  int rc = check(q, q2, ktdia, klev);
+  if (rc) return rc;
+  if (nproma < 1 || ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma)
+    return fail(CLOUDSC_EDIMS, "cloudst_blocked: need NGPBLKS = ceil(NGPTOT/NPROMA)");
+  const void* req[] = {p.ceta,  p.paphp1, p.papp1, p.pqm1,   p.pqs,   p.ptm1,   p.plude,  p.plu,  p.ptent,
+                       p.pgtent, p.ptenq, p.pgtenq, p.pclc,   p.pqiwp, p.pqlwp,  p.pfplsl, p.pfplsn,
+                       p.pfhpsl, p.pfhpsn, p.pcovptot};
+  for (const void* x : req)
+    if (!x) return fail(CLOUDSC_ENULL, "cloudst_blocked: NULL array");
+  rc = cloudsc_capi::check_device();
+  if (rc) return rc;
+  C2Dims d{ngptot, nproma, klev, ngpblks, ktdia, 1, nproma};
+  hipError_t e = launch<R>(*q, *q2, ldrain1d != 0, (double)ptsphy, p, d, (hipStream_t)stream, false, true);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudst_blocked: kernel launch", e);
   return CLOUDSC_OK;
 }
 
@@ -655,9 +942,18 @@ int layer_blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const 
                             ptendency_loc_l, ptendency_l, ptendency_loc_i, ptendency_i, pclc, pfplsl, pfplsn,    \
                             pfhpsl, pfhpsn, pcovptot};                                                             \
   p.pa = pa; p.ptena = ptendency_loc_a; p.ptenr = ptendency_loc_r; p.ptens = ptendency_loc_s; p.pqsat = pqsat;   \
-  p.pprectype = pprectype; p.pfzra = pfzra;                                                                       \
+  p.pprectype = pprectype; p.pfzra = pfzra; p.pqiwp = pqiwp; p.pqlwp = pqlwp;                                    \
   return cloudsc2_gpu::layer_blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d, \
                                         tsphy, p, stream)
+
+#define CLOUDST_BLOCKED(R)                                                                                      \
+  cloudsc2_gpu::C2Ptrs<R> p{};                                                                                     \
+  p.ceta = ceta; p.paphp1 = paphp1; p.papp1 = papp1; p.pqm1 = pqm1; p.pqs = pqs; p.ptm1 = ptm1; p.plude = plude;   \
+  p.plu = plu; p.ptent = ptent; p.pgtent = pgtent; p.ptenq = ptenq; p.pgtenq = pgtenq; p.pclc = pclc;               \
+  p.pqiwp = pqiwp; p.pqlwp = pqlwp; p.pfplsl = pfplsl; p.pfplsn = pfplsn; p.pfhpsl = pfhpsl; p.pfhpsn = pfhpsn;     \
+  p.pcovptot = pcovptot;                                                                                          \
+  return cloudsc2_gpu::cloudst_blocked<R>(ydparams, ydparams2, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d,     \
+                                          ptsphy, p, stream)
 
 extern "C" {
 
@@ -670,6 +966,8 @@ int cloudsc2_params_default(cloudsc2_params_t* p2, const cloudsc_params_t* p) {
   p2->rvtmp2 = 0.0;                                         // suphec.F90:229
   p2->lphylin = 0;                                          // suphli.F90:107
   p2->levapls2 = 0;                                         // su0phy.F90:1361
+  p2->lepcld2 = 0;                                          // su0phy.F90:1360
+  p2->pad_ = 0;
   return CLOUDSC_OK;
 }
 
@@ -677,6 +975,8 @@ int cloudsc2_dp_c(CLOUDSC2_ARGS(double)) { CLOUDSC2_PER_BLOCK(double); }
 int cloudsc2_sp_c(CLOUDSC2_ARGS(float)) { CLOUDSC2_PER_BLOCK(float); }
 int cloudsc2_blocked_dp_c(CLOUDSC2_BLOCKED_ARGS(double)) { CLOUDSC2_BLOCKED(double); }
 int cloudsc2_blocked_sp_c(CLOUDSC2_BLOCKED_ARGS(float)) { CLOUDSC2_BLOCKED(float); }
+int cloudst_blocked_dp_c(CLOUDST_BLOCKED_ARGS(double)) { CLOUDST_BLOCKED(double); }
+int cloudst_blocked_sp_c(CLOUDST_BLOCKED_ARGS(float)) { CLOUDST_BLOCKED(float); }
 int cloud_s_layer_blocked_dp_c(CLOUD_S_LAYER_BLOCKED_ARGS(double)) { CLOUD_S_LAYER_BLOCKED(double); }
 int cloud_s_layer_blocked_sp_c(CLOUD_S_LAYER_BLOCKED_ARGS(float)) { CLOUD_S_LAYER_BLOCKED(float); }
 

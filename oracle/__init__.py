@@ -308,3 +308,28 @@ def cloud_s_layer_blocked(precision, params, params2, ceta, ngptot, nproma, ktdi
     if rc != 0:
         raise RuntimeError(f"oracle_cloud_s_layer_blocked_{precision} returned {rc}")
     return arrays
+
+
+def cloudst_blocked(precision, params, params2, ceta, ngptot, nproma, ktdia, klev, ldrain1d, ptsphy, arrays,
+                    nthreads=0):
+    """CLOUDST over all blocks of blocked host arrays named as openifs_amd.cloudsc2.CLOUDST_ARRAYS."""
+    from openifs_amd.cloudsc2 import CLOUDST_ARRAYS
+
+    ngpblks = (ngptot + nproma - 1) // nproma
+    real = ctypes.c_double if precision == "dp" else ctypes.c_float
+    rdt = np.float64 if precision == "dp" else np.float32
+    ceta = np.ascontiguousarray(ceta, dtype=rdt)
+    args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.ctypes.data), ctypes.c_int(ngptot),
+            ctypes.c_int(nproma), ctypes.c_int(ktdia), ctypes.c_int(klev), ctypes.c_int(ngpblks),
+            ctypes.c_int(int(bool(ldrain1d))), real(ptsphy)]
+    for name, kind, _ in CLOUDST_ARRAYS:
+        a = arrays[name]
+        want = (ngpblks, klev + 1 if kind == "h" else klev, nproma)
+        if a.dtype != rdt or not a.flags.c_contiguous or a.shape != want:
+            raise TypeError(f"{name}: need a C-contiguous {np.dtype(rdt).name} array of shape {want}")
+        args.append(ctypes.c_void_p(a.ctypes.data))
+    args.append(ctypes.c_int(nthreads))
+    rc = getattr(lib(), f"oracle_cloudst_blocked_{precision}")(*args)
+    if rc != 0:
+        raise RuntimeError(f"oracle_cloudst_blocked_{precision} returned {rc}")
+    return arrays

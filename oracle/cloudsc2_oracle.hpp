@@ -419,6 +419,306 @@ void cloudsc2(const cloudsc_params_t& p, const cloudsc2_params_t& p2, const Clou
     }
 }
 
+// CLOUDST (cloudst.F90:9-641), the diagnostic scheme of CLOUD_S_LAYER when LEPCLD2 is false.  The arguments are
+// CLOUDSC2's without PL/PI/PMFU/PMFD/PTENL/PGTENL/PTENI/PGTENI, plus the diagnostic water paths.
+template <typename R>
+struct CloudstArgs {
+  int kidia, kfdia, klon, ktdia, klev;
+  bool ldrain1d;
+  R ptsphy;
+  const R* ceta;
+  const R *paphp1, *papp1, *pqm1, *pqs, *ptm1, *plude, *plu;
+  R* ptent;
+  const R* pgtent;
+  R* ptenq;
+  const R* pgtenq;
+  R *pclc, *pqiwp, *pqlwp, *pfplsl, *pfplsn, *pfhpsl, *pfhpsn, *pcovptot;
+};
+
+template <typename R>
+void cloudst(const cloudsc_params_t& p, const cloudsc2_params_t& p2, const CloudstArgs<R>& a) {
+  const int kidia = a.kidia, kfdia = a.kfdia, klon = a.klon, klev = a.klev, ktdia = a.ktdia;
+  const R ptsphy = a.ptsphy;
+  const Thermo<R> th(p);
+  auto i2 = [klon](int jl, int jk) { return (size_t)(jk - 1) * klon + (jl - 1); };
+  const R RG = R(p.rg), RCPD = R(p.rcpd), RETV = R(p.retv), RTT = R(p.rtt), RLVTT = R(p.rlvtt);
+  const R RLSTT = R(p.rlstt), RLMLT = R(p.rlmlt), R2ES = R(p.r2es), R3LES = R(p.r3les), R3IES = R(p.r3ies);
+  const R R4LES = R(p.r4les), R4IES = R(p.r4ies), R5LES = R(p.r5les), R5IES = R(p.r5ies);
+  const R RKCONV = R(p.rkconv), RLMIN = R(p.rlmin);
+  const R RCLCRIT = R(p2.rclcrit), RLPTRC = R(p2.rlptrc), RVTMP2 = R(p2.rvtmp2);
+  const bool LPHYLIN = p2.lphylin != 0, LEVAPLS2 = p2.levapls2 != 0, LDRAIN1D = a.ldrain1d;
+  const R ZSCAL = R(0.9);  // :165
+
+  // 1.1 (:213-221)
+  const R ZCKCODT = R(2.0) * RKCONV * ptsphy;
+  const R ZCONS2 = R(1.0) / (ptsphy * RG);
+  const R ZCONS3 = R(2.5E6) / R(1005.);
+  const R ZMELTP2 = RTT + R(2.0);
+  const R ZPECONS = R(1.E-5);
+  const R ZQMAX = R(0.5), ZEPS1 = R(1.E-12), ZEPS2 = R(1.E-10);
+
+  const size_t n2 = (size_t)klon * klev;
+  std::vector<R> ztp1(n2), zqp1(n2), zcrh(n2), zdp(n2), zlfdcp(n2), zlsdcp(n2), zlvdcp(n2), zqlwc(n2);
+  std::vector<R> zrfreeze(n2), zfwat(n2), zlude(n2), zscalm(klev);
+  const int KL = klon;
+  std::vector<R> ztrpaus(KL), zrfl(KL), zsfl(KL), zrfln(KL), zsfln(KL), zcovptot(KL), zcovpclr(KL), zdr(KL);
+  std::vector<R> zdqsdtemp(KL), zcorqs(KL), zqlim(KL), zqcrit(KL), zgdp(KL), zdtgdp(KL), zdqdt(KL), zdtdt(KL);
+  std::vector<R> zpp(KL), zqold(KL), zdq(KL);
+
+  for (int jk = 1; jk <= klev; ++jk)  // (:225-230)
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      ztp1[i2(jl, jk)] = a.ptm1[i2(jl, jk)] + ptsphy * a.pgtent[i2(jl, jk)];
+      zqp1[i2(jl, jk)] = a.pqm1[i2(jl, jk)] + ptsphy * a.pgtenq[i2(jl, jk)];
+    }
+  for (int jl = kidia; jl <= kfdia; ++jl) ztrpaus[jl - 1] = R(0.1);  // (:234-247)
+  for (int jk = 1; jk <= klev - 1; ++jk)
+    for (int jl = kidia; jl <= kfdia; ++jl)
+      if (a.ceta[jk - 1] > R(0.1) && a.ceta[jk - 1] < R(0.4) && ztp1[i2(jl, jk)] > ztp1[i2(jl, jk + 1)])
+        ztrpaus[jl - 1] = a.ceta[jk - 1];
+  // 2.1 critical relative humidity (:263-306)
+  for (int jk = 1; jk <= klev; ++jk) {
+    const R ceta = a.ceta[jk - 1];
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      const R zeta3 = ztrpaus[jl - 1];
+      const R zrh1 = R(0.9), zrh2 = R(0.5), zrh3 = R(1.0), zeta1 = R(0.75);
+      const R zdeta1 = R(1.0) - zeta1;
+      R& c = zcrh[i2(jl, jk)];
+      if (ceta < zeta3) {
+        c = zrh3 + (zrh2 - zrh3) * ceta / zeta3;
+      } else if (ceta >= zeta3 && ceta < zeta1) {
+        c = zrh2;
+      } else if (ceta >= zeta1) {
+        c = zrh1 + (zrh2 - zrh1) * std::sqrt((R(1.0) - ceta) / zdeta1);
+      }
+    }
+    zscalm[jk - 1] = ZSCAL * std::pow(std::max(ceta - R(0.2), ZEPS1), R(0.2));
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      const size_t k = i2(jl, jk);
+      zdp[k] = a.paphp1[i2(jl, jk + 1)] - a.paphp1[k];
+      const R zzz = R(1.0) / (RCPD + RCPD * RVTMP2 * zqp1[k]);
+      zlfdcp[k] = RLMLT * zzz;
+      zlsdcp[k] = RLSTT * zzz;
+      zlvdcp[k] = RLVTT * zzz;
+    }
+  }
+  // 2.2 (:311-330)
+  for (int jk = 1; jk <= klev; ++jk)
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      const size_t k = i2(jl, jk);
+      a.pclc[k] = a.pqiwp[k] = a.pqlwp[k] = R(0.0);
+      zqlwc[k] = zrfreeze[k] = R(0.0);
+      a.pcovptot[k] = R(0.0);
+    }
+  for (int jl = kidia; jl <= kfdia; ++jl) {
+    const int i = jl - 1;
+    zrfl[i] = zsfl[i] = R(0.0);
+    a.pfplsl[i2(jl, 1)] = a.pfplsn[i2(jl, 1)] = R(0.0);
+    zcovptot[i] = zcovpclr[i] = R(0.0);
+  }
+  // 3. level loop (:340-623)
+  for (int jk = ktdia; jk <= klev; ++jk) {
+    for (int jl = kidia; jl <= kfdia; ++jl) {  // (:342-385)
+      const int i = jl - 1;
+      const size_t k = i2(jl, jk);
+      zdr[i] = R(0.0);
+      const R t = ztp1[k];
+      R zalfaw, zesdp;
+      if (LPHYLIN || LDRAIN1D) {
+        const R zoealfaw = R(0.545) * (std::tanh(R(0.17) * (t - RLPTRC)) + R(1.0));
+        R z3es, z4es;
+        if (t < RTT) {
+          zalfaw = zoealfaw;
+          z3es = R3IES;
+          z4es = R4IES;
+        } else {
+          zalfaw = R(1.0);
+          z3es = R3LES;
+          z4es = R4LES;
+        }
+        const R zfoeew = R2ES * std::exp(z3es * (t - RTT) / (t - z4es));
+        zesdp = zfoeew / a.papp1[k];
+        if (zesdp > ZQMAX) zesdp = ZQMAX;
+      } else {
+        zalfaw = th.foealfa(t);
+        zesdp = th.foeewm(t) / a.papp1[k];
+      }
+      const R dl = t - R4LES, di = t - R4IES;
+      const R zfacw = R5LES / (dl * dl);
+      const R zfaci = R5IES / (di * di);
+      const R zfac = zalfaw * zfacw + (R(1.0) - zalfaw) * zfaci;
+      const R zcor = R(1.0) / (R(1.0) - RETV * zesdp);
+      zdqsdtemp[i] = zfac * zcor * a.pqs[k];
+      zcorqs[i] = R(1.0) + ZCONS3 * zdqsdtemp[i];
+      zqlim[i] = zqp1[k];
+      if (zqp1[k] > a.pqs[k]) zqlim[i] = a.pqs[k];
+      zqcrit[i] = zcrh[k] * a.pqs[k];
+    }
+    for (int jl = kidia; jl <= kfdia; ++jl) {  // layer cloud amounts (:387-403)
+      const int i = jl - 1;
+      const size_t k = i2(jl, jk);
+      if (zqp1[k] <= zqcrit[i]) {
+        a.pclc[k] = R(0.0);
+        zqlwc[k] = R(0.0);
+      } else if (zqp1[k] >= a.pqs[k]) {
+        a.pclc[k] = R(1.0);
+        zqlwc[k] = (R(1.0) - zscalm[jk - 1]) * (a.pqs[k] - zqcrit[i]);
+      } else {
+        const R zqpd = a.pqs[k] - zqp1[k];
+        const R zqcd = a.pqs[k] - zqcrit[i];
+        a.pclc[k] = R(1.0) - std::sqrt(zqpd / (zqcd - zscalm[jk - 1] * (zqp1[k] - zqcrit[i])));
+        zqlwc[k] = (zscalm[jk - 1] * zqpd + (R(1.0) - zscalm[jk - 1]) * zqcd) * (a.pclc[k] * a.pclc[k]);
+      }
+    }
+    for (int jl = kidia; jl <= kfdia; ++jl) {  // convective cloud (:407-424)
+      const int i = jl - 1;
+      const size_t k = i2(jl, jk);
+      zgdp[i] = RG / (a.paphp1[i2(jl, jk + 1)] - a.paphp1[k]);
+      zlude[k] = a.plude[k] * ptsphy * zgdp[i];
+      bool llo1 = false;
+      if (jk < klev) llo1 = zlude[k] >= RLMIN && a.plu[i2(jl, jk + 1)] >= ZEPS2;
+      if (llo1) {
+        a.pclc[k] = a.pclc[k] + (R(1.0) - a.pclc[k]) * (R(1.0) - std::exp(-zlude[k] / a.plu[i2(jl, jk + 1)]));
+        zqlwc[k] = zqlwc[k] + zlude[k];
+      }
+    }
+    for (int jl = kidia; jl <= kfdia; ++jl) {  // (:428-434)
+      const int i = jl - 1;
+      zcovptot[i] = std::max(zcovptot[i], a.pclc[i2(jl, jk)]);
+      zcovpclr[i] = zcovptot[i] - a.pclc[i2(jl, jk)];
+      zcovpclr[i] = std::max(zcovpclr[i], R(0.0));
+    }
+    for (int jl = kidia; jl <= kfdia; ++jl) {  // melting of incoming snow (:448-460)
+      const int i = jl - 1;
+      const size_t k = i2(jl, jk);
+      if (zsfl[i] != R(0.0)) {
+        const R zcons = ZCONS2 * zdp[k] / zlfdcp[k];
+        const R zsnmlt = std::min(zsfl[i], zcons * std::max(R(0.0), (ztp1[k] - ZMELTP2)));
+        zrfln[i] = zrfl[i] + zsnmlt;
+        zsfln[i] = zsfl[i] - zsnmlt;
+        ztp1[k] = ztp1[k] - zsnmlt / zcons;
+      } else {
+        zrfln[i] = zrfl[i];
+        zsfln[i] = zsfl[i];
+      }
+    }
+    for (int jl = kidia; jl <= kfdia; ++jl) {  // precipitation (:462-542)
+      const int i = jl - 1;
+      const size_t k = i2(jl, jk);
+      R zpr;
+      if (a.pclc[k] > ZEPS2) {
+        const R zlcrit = (LEVAPLS2 || LDRAIN1D) ? RCLCRIT : RCLCRIT * R(2.);
+        const R zcld = zqlwc[k] / a.pclc[k];
+        const R r = zcld / zlcrit;
+        const R zd = ZCKCODT * (R(1.0) - std::exp(-(r * r)));
+        const R zlnew = a.pclc[k] * zcld * std::exp(-zd);
+        zpr = zqlwc[k] - zlnew;
+        zqlwc[k] = zqlwc[k] - zpr;
+      } else {
+        zpr = R(0.0);
+      }
+      zdr[i] = zdr[i] + ZCONS2 * zdp[k] * zpr;
+      if (LPHYLIN || LDRAIN1D) {
+        const R zoealfa = R(0.545) * (std::tanh(R(0.17) * (ztp1[k] - RLPTRC)) + R(1.0));
+        zfwat[k] = ztp1[k] < RTT ? zoealfa : R(1.0);
+      } else {
+        zfwat[k] = th.foealfa(ztp1[k]);
+      }
+      R zfwatr;
+      if (ztp1[k] < RTT) {
+        zrfreeze[k] = zfwat[k] * zdr[i];
+        zfwatr = R(0.0);
+      } else {
+        zfwatr = R(1.0);
+      }
+      zrfln[i] = zrfln[i] + zfwatr * zdr[i];
+      zsfln[i] = zsfln[i] + (R(1.0) - zfwatr) * zdr[i];
+      const R zprtot = zrfln[i] + zsfln[i];
+      const bool llo2 = zprtot > ZEPS2 && zcovpclr[i] > ZEPS2 && (LEVAPLS2 || LDRAIN1D);
+      if (llo2) {
+        R zpreclr = zprtot * zcovpclr[i] / zcovptot[i];
+        const R om = R(1.0) - a.pclc[k];
+        const R zqe = a.pqs[k] - (a.pqs[k] - zqlim[i]) * zcovpclr[i] / (om * om);
+        const R zbeta = RG * ZPECONS *
+                        std::pow(std::sqrt(a.papp1[k] / a.paphp1[i2(jl, klev + 1)]) / R(5.09E-3) * zpreclr / zcovpclr[i],
+                                 R(0.5777));
+        const R zb = ptsphy * zbeta * (a.pqs[k] - zqe) / (R(1.0) + zbeta * ptsphy * zcorqs[i]);
+        zdtgdp[i] = ptsphy * RG / (a.paphp1[i2(jl, jk + 1)] - a.paphp1[k]);
+        R zdpr = zcovpclr[i] * zb / zdtgdp[i];
+        zdpr = std::min(zdpr, zpreclr);
+        zpreclr = zpreclr - zdpr;
+        if (zpreclr <= R(0.0)) zcovptot[i] = a.pclc[k];
+        a.pcovptot[k] = zcovptot[i];
+        const R zdprsfl = zdpr * zrfln[i] / zprtot;
+        zrfln[i] = zrfln[i] - zdprsfl;
+        const R zdpssfl = zdpr * zsfln[i] / zprtot;
+        zsfln[i] = zsfln[i] - zdpssfl;
+      }
+      a.pqlwp[k] = zfwat[k] * zqlwc[k];
+      a.pqiwp[k] = (R(1.0) - zfwat[k]) * zqlwc[k];
+    }
+    // tendencies from the flux divergence, first guess, CUADJTQ (:546-583)
+    auto tendencies = [&](int jl) {
+      const int i = jl - 1;
+      const size_t k = i2(jl, jk);
+      zdqdt[i] = -((zrfln[i] - zrfl[i]) + (zsfln[i] - zsfl[i])) * (RG / zdp[k]) + a.plude[k] * zgdp[i];
+      zdtdt[i] = (zlvdcp[k] * (zrfln[i] - zrfl[i]) + zlsdcp[k] * (zsfln[i] - zsfl[i])) * (RG / zdp[k]) -
+                 a.plude[k] * zgdp[i] * (zfwat[k] * zlvdcp[k] + (R(1.0) - zfwat[k]) * zlsdcp[k]) +
+                 (zlsdcp[k] - zlvdcp[k]) * zrfreeze[k] * zgdp[i];
+    };
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      const int i = jl - 1;
+      const size_t k = i2(jl, jk);
+      tendencies(jl);
+      ztp1[k] = ztp1[k] + ptsphy * zdtdt[i];
+      zqp1[k] = zqp1[k] + ptsphy * zdqdt[i];
+      zpp[i] = a.papp1[k];
+      zqold[i] = zqp1[k];
+    }
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      const size_t k = i2(jl, jk);
+      if (LPHYLIN || LDRAIN1D)
+        cuadjtqs0(th, RETV, zpp[jl - 1], ztp1[k], zqp1[k]);
+      else
+        cuadjtq0(th, RETV, zpp[jl - 1], ztp1[k], zqp1[k]);
+    }
+    for (int jl = kidia; jl <= kfdia; ++jl) {  // (:585-600)
+      const int i = jl - 1;
+      const size_t k = i2(jl, jk);
+      zdq[i] = std::max(R(0.0), zqold[i] - zqp1[k]);
+      zdr[i] = ZCONS2 * zdp[k] * zdq[i];
+      R zrfreeze2, zfwatr;
+      if (ztp1[k] < RTT) {
+        zrfreeze2 = zfwat[k] * zdr[i];
+        zfwatr = R(0.0);
+      } else {
+        zrfreeze2 = R(0.0);
+        zfwatr = R(1.0);
+      }
+      zrfln[i] = zrfln[i] + zfwatr * zdr[i];
+      zsfln[i] = zsfln[i] + (R(1.0) - zfwatr) * zdr[i];
+      zrfreeze[k] = zrfreeze[k] + zrfreeze2;
+    }
+    for (int jl = kidia; jl <= kfdia; ++jl) {  // (:602-616)
+      const int i = jl - 1;
+      const size_t k = i2(jl, jk);
+      tendencies(jl);
+      a.ptenq[k] = zdqdt[i];
+      a.ptent[k] = zdtdt[i];
+      a.pfplsl[i2(jl, jk + 1)] = zrfln[i];
+      a.pfplsn[i2(jl, jk + 1)] = zsfln[i];
+    }
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      zrfl[jl - 1] = zrfln[jl - 1];
+      zsfl[jl - 1] = zsfln[jl - 1];
+    }
+  }
+  for (int jk = 1; jk <= klev + 1; ++jk)  // enthalpy fluxes (:629-634)
+    for (int jl = kidia; jl <= kfdia; ++jl) {
+      a.pfhpsl[i2(jl, jk)] = -a.pfplsl[i2(jl, jk)] * RLVTT;
+      a.pfhpsn[i2(jl, jk)] = -a.pfplsn[i2(jl, jk)] * RLSTT;
+    }
+}
+
 // SATUR, KFLAG = 2 (satur.F90:110-141), levels KTDIA..KLEV.
 template <typename R>
 void satur2(const cloudsc_params_t& p, bool ldphylin, int kidia, int kfdia, int klon, int ktdia, int klev,
@@ -454,12 +754,13 @@ void satur2(const cloudsc_params_t& p, bool ldphylin, int kidia, int kfdia, int 
 template <typename R>
 struct LayerArgs {
   const R* pa;
-  R *ptena, *ptenr, *ptens, *pqsat, *pprectype, *pfzra;
+  R *ptena, *ptenr, *ptens, *pqsat, *pprectype, *pfzra, *pqiwp, *pqlwp;
 };
 
-// CLOUD_S_LAYER with LEPCLD2 (cloud_s_layer.F90:112-176): SATUR at T + TSPHY*TENDENCY%T into PQSAT, CLOUDSC2 with
-// it, then the cloud-fraction tendency, zero rain and snow tendencies and zero PPRECTYPE/PFZRA.  `a` carries the
-// CLOUDSC2 arguments; its PQS is the PQSAT array filled here and its PCLC is ZPATMP.
+// CLOUD_S_LAYER (cloud_s_layer.F90:112-176): SATUR at T + TSPHY*TENDENCY%T into PQSAT, CLOUDSC2 (LEPCLD2) or
+// CLOUDST with it, then the cloud-fraction tendency (and with CLOUDST the liquid and ice tendencies from the
+// diagnostic water), zero rain and snow tendencies and zero PPRECTYPE/PFZRA.  `a` carries the CLOUDSC2 arguments
+// (CLOUDST reads its subset); its PQS is the PQSAT array filled here and its PCLC is ZPATMP.
 template <typename R>
 void cloud_s_layer(const cloudsc_params_t& p, const cloudsc2_params_t& p2, Cloudsc2Args<R> a, const LayerArgs<R>& l) {
   const int kidia = a.kidia, kfdia = a.kfdia, klon = a.klon, klev = a.klev;
@@ -472,7 +773,16 @@ void cloud_s_layer(const cloudsc_params_t& p, const cloudsc2_params_t& p2, Cloud
   a.pqs = zpqsat.data();
   R* clc_out = a.pclc;
   a.pclc = zpatmp.data();
-  cloudsc2<R>(p, p2, a);
+  std::vector<R> zplsmp((size_t)klon * klev, R(0)), zpltmp((size_t)klon * klev, R(0));
+  if (p2.lepcld2) {
+    cloudsc2<R>(p, p2, a);
+  } else {
+    const CloudstArgs<R> s{a.kidia,  a.kfdia, a.klon,  a.ktdia, a.klev,  a.ldrain1d, a.ptsphy, a.ceta,
+                           a.paphp1, a.papp1, a.pqm1,  a.pqs,   a.ptm1,  a.plude,    a.plu,    a.ptent,
+                           a.pgtent, a.ptenq, a.pgtenq, a.pclc, zplsmp.data(), zpltmp.data(), a.pfplsl, a.pfplsn,
+                           a.pfhpsl, a.pfhpsn, a.pcovptot};
+    cloudst<R>(p, p2, s);
+  }
   const R zcons = R(1) / a.ptsphy;  // (:154)
   for (int jk = 1; jk <= klev; ++jk)
     for (int jl = kidia; jl <= kfdia; ++jl) {
@@ -480,6 +790,12 @@ void cloud_s_layer(const cloudsc_params_t& p, const cloudsc2_params_t& p2, Cloud
       if (l.ptena) l.ptena[k] = zcons * (zpatmp[k] - l.pa[k]);
       if (clc_out) clc_out[k] = zpatmp[k];
       if (l.pqsat && jk >= a.ktdia) l.pqsat[k] = zpqsat[k];
+      if (!p2.lepcld2) {  // (:159-163)
+        a.ptenl[k] = zcons * (zpltmp[k] - a.pl[k]);
+        a.pteni[k] = zcons * (zplsmp[k] - a.pi[k]);
+        if (l.pqlwp) l.pqlwp[k] = zpltmp[k];
+        if (l.pqiwp) l.pqiwp[k] = zplsmp[k];
+      }
     }
   for (int jk = 1; jk <= klev; ++jk)  // (:164-165), every column
     for (int jl = 1; jl <= klon; ++jl) {

@@ -401,11 +401,13 @@ int cloud_s_layer_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2
                           const R* ptendency_q, const R* ptendency_l, const R* ptendency_i, R* ptendency_loc_t,
                           R* ptendency_loc_q, R* ptendency_loc_a, R* ptendency_loc_l, R* ptendency_loc_i,
                           R* ptendency_loc_r, R* ptendency_loc_s, R* pqsat, R* pclc, R* pfplsl, R* pfplsn,
-                          R* pfhpsl, R* pfhpsn, R* pcovptot, R* pprectype, R* pfzra, int nthreads) {
+                          R* pfhpsl, R* pfhpsn, R* pcovptot, R* pprectype, R* pfzra, R* pqiwp, R* pqlwp,
+                          int nthreads) {
   int rc = cloudsc2_check<R>(p, p2, 1, nproma, nproma, ktdia, klev);
   if (rc) return rc;
   if (ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma) return CLOUDSC_EDIMS;
   if (ptendency_loc_a && !pa) return CLOUDSC_ENULL;
+  if (p2->lepcld2 && (!pmfu || !pmfd || !ptendency_l || !ptendency_i)) return CLOUDSC_ENULL;
   const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1);
   auto off = [](auto* x, size_t n) { return x ? x + n : x; };
   if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -430,16 +432,16 @@ int cloud_s_layer_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2
                               pi + o2,
                               plude + o2,
                               plu + o2,
-                              pmfu + o2,
-                              pmfd + o2,
+                              off(pmfu, o2),
+                              off(pmfd, o2),
                               ptendency_loc_t + o2,
                               ptendency_t + o2,
                               ptendency_loc_q + o2,
                               ptendency_q + o2,
                               ptendency_loc_l + o2,
-                              ptendency_l + o2,
+                              off(ptendency_l, o2),
                               ptendency_loc_i + o2,
-                              ptendency_i + o2,
+                              off(ptendency_i, o2),
                               off(pclc, o2),
                               pfplsl + oh,
                               pfplsn + oh,
@@ -448,7 +450,7 @@ int cloud_s_layer_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2
                               pcovptot + o2};
     const oracle::LayerArgs<R> l{off(pa, o2),    off(ptendency_loc_a, o2), off(ptendency_loc_r, o2),
                                  off(ptendency_loc_s, o2), off(pqsat, o2), off(pprectype, o1),
-                                 off(pfzra, o1)};
+                                 off(pfzra, o1),      off(pqiwp, o2),           off(pqlwp, o2)};
     oracle::cloud_s_layer<R>(*p, *p2, a, l);
   }
   return CLOUDSC_OK;
@@ -461,7 +463,7 @@ int cloud_s_layer_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2
       const R *pmfd, const R *ptendency_t, const R *ptendency_q, const R *ptendency_l, const R *ptendency_i,   \
       R *ptendency_loc_t, R *ptendency_loc_q, R *ptendency_loc_a, R *ptendency_loc_l, R *ptendency_loc_i,      \
       R *ptendency_loc_r, R *ptendency_loc_s, R *pqsat, R *pclc, R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn,    \
-      R *pcovptot, R *pprectype, R *pfzra, int nthreads
+      R *pcovptot, R *pprectype, R *pfzra, R *pqiwp, R *pqlwp, int nthreads
 
 #define ORACLE_LAYER_CALL(R)                                                                                     \
   return cloud_s_layer_blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d,      \
@@ -469,12 +471,52 @@ int cloud_s_layer_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2
                                   ptendency_q, ptendency_l, ptendency_i, ptendency_loc_t, ptendency_loc_q,         \
                                   ptendency_loc_a, ptendency_loc_l, ptendency_loc_i, ptendency_loc_r,              \
                                   ptendency_loc_s, pqsat, pclc, pfplsl, pfplsn, pfhpsl, pfhpsn, pcovptot,          \
-                                  pprectype, pfzra, nthreads)
+                                  pprectype, pfzra, pqiwp, pqlwp, nthreads)
+
+// CLOUDST over all blocks: the CLOUDST_BLOCKED_ARGS order without the stream, then the thread count.
+template <typename R>
+int cloudst_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2, const R* ceta, int ngptot, int nproma,
+                    int ktdia, int klev, int ngpblks, int ldrain1d, R ptsphy, const R* paphp1, const R* papp1,
+                    const R* pqm1, const R* pqs, const R* ptm1, const R* plude, const R* plu, R* ptent,
+                    const R* pgtent, R* ptenq, const R* pgtenq, R* pclc, R* pqiwp, R* pqlwp, R* pfplsl, R* pfplsn,
+                    R* pfhpsl, R* pfhpsn, R* pcovptot, int nthreads) {
+  int rc = cloudsc2_check<R>(p, p2, 1, nproma, nproma, ktdia, klev);
+  if (rc) return rc;
+  if (ngptot < 1 || ngpblks != (ngptot + nproma - 1) / nproma) return CLOUDSC_EDIMS;
+  const size_t s2 = (size_t)nproma * klev, sh = (size_t)nproma * (klev + 1);
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int ibl = 0; ibl < ngpblks; ++ibl) {
+    const int kfdia = std::min(nproma, ngptot - ibl * nproma);
+    const size_t o2 = ibl * s2, oh = ibl * sh;
+    const oracle::CloudstArgs<R> a{1,           kfdia,       nproma,      ktdia,       klev,        ldrain1d != 0,
+                                   ptsphy,      ceta,        paphp1 + oh, papp1 + o2,  pqm1 + o2,   pqs + o2,
+                                   ptm1 + o2,   plude + o2,  plu + o2,    ptent + o2,  pgtent + o2, ptenq + o2,
+                                   pgtenq + o2, pclc + o2,   pqiwp + o2,  pqlwp + o2,  pfplsl + oh, pfplsn + oh,
+                                   pfhpsl + oh, pfhpsn + oh, pcovptot + o2};
+    oracle::cloudst<R>(*p, *p2, a);
+  }
+  return CLOUDSC_OK;
+}
+
+#define ORACLE_CLOUDST_ARGS(R)                                                                                 \
+  const cloudsc_params_t *ydparams, const cloudsc2_params_t *ydparams2, const R *ceta, int ngptot, int nproma, \
+      int ktdia, int klev, int ngpblks, int ldrain1d, R ptsphy, const R *paphp1, const R *papp1, const R *pqm1, \
+      const R *pqs, const R *ptm1, const R *plude, const R *plu, R *ptent, const R *pgtent, R *ptenq,           \
+      const R *pgtenq, R *pclc, R *pqiwp, R *pqlwp, R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, R *pcovptot,      \
+      int nthreads
+
+#define ORACLE_CLOUDST_CALL(R)                                                                                \
+  return cloudst_blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d, ptsphy, \
+                            paphp1, papp1, pqm1, pqs, ptm1, plude, plu, ptent, pgtent, ptenq, pgtenq, pclc, pqiwp,  \
+                            pqlwp, pfplsl, pfplsn, pfhpsl, pfhpsn, pcovptot, nthreads)
 
 }  // namespace
 
 extern "C" {
 
+int oracle_cloudst_blocked_dp(ORACLE_CLOUDST_ARGS(double)) { ORACLE_CLOUDST_CALL(double); }
+int oracle_cloudst_blocked_sp(ORACLE_CLOUDST_ARGS(float)) { ORACLE_CLOUDST_CALL(float); }
 int oracle_cloud_s_layer_blocked_dp(ORACLE_LAYER_ARGS(double)) { ORACLE_LAYER_CALL(double); }
 int oracle_cloud_s_layer_blocked_sp(ORACLE_LAYER_ARGS(float)) { ORACLE_LAYER_CALL(float); }
 

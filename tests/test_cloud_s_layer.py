@@ -1,5 +1,5 @@
-"""CLOUD_S_LAYER with LEPCLD2 (cloud_s_layer.F90:112-176): SATUR at T + TSPHY*TENDENCY%T, CLOUDSC2 on that
-saturation humidity, and the wrap-up, in one launch.
+"""CLOUD_S_LAYER (cloud_s_layer.F90:112-176): SATUR at T + TSPHY*TENDENCY%T, CLOUDSC2 (LEPCLD2) or CLOUDST
+(cloudst.F90, the default) on that saturation humidity, and the wrap-up, in one launch.
 
 CPU pins of the oracle:
   - SATUR (satur.F90:110-141):
@@ -13,6 +13,14 @@ CPU pins of the oracle:
     - PPRECTYPE and PFZRA are zero;
     - the per-level water closure of CLOUDSC2 holds.
   - optional outputs may be NULL without changing the others.
+  - CLOUDST:
+    - no supersaturation is left after its CUADJTQ at the final state;
+    - a dry column gets exactly zero outputs;
+    - the diagnostic water splits by phase: PQIWP = 0 where the final layer is warm (T >= RTT), PQLWP = 0 where
+      it is colder than RTICE;
+    - the cover is 0 where q <= 0.5 q_s, 1 at saturation, and nondecreasing in q;
+    - the layer with LEPCLD2 false equals CLOUDST on its PQSAT, with PA + dt PTENDENCY_LOC_A = PCLC,
+      PL + dt PTENDENCY_LOC_L = PQLWP and PI + dt PTENDENCY_LOC_I = PQIWP.
 GPU: FP64 equals the oracle to 1e-12 (default and LPHYLIN, KTDIA 1 and 5, ragged last block); FP32 is as accurate
 as the FP32 oracle.
 """
@@ -22,7 +30,15 @@ import pytest
 import oracle
 from openifs_amd import cloudsc2 as C2
 from tests.helpers import relrms
-from tests.test_cloudsc2 import PTSPHY, params2, water_residual
+from tests.test_cloudsc2 import PTSPHY, water_residual
+from tests.test_cloudsc2 import final_rh
+from tests.test_cloudsc2 import params2 as _params2
+
+
+def params2(p, lphylin=0, levapls2=0, lepcld2=1):
+    q = _params2(p, lphylin, levapls2)
+    q.lepcld2 = lepcld2
+    return q
 
 
 def layer_case(p, seed, ngptot, nproma, klev, prec="dp"):
@@ -63,8 +79,7 @@ def test_satur_at_triple_point_and_clip(params137, case):
 
 def test_satur_lphylin_form_equals_foeewm(params137, case):
     a0 = run_layer("dp", params137, params2(params137, 0), case, 500, 32, 137)
-    q2 = params2(params137, 0)
-    q2.lphylin = 1
+    q2 = params2(params137, 1)
     # the phase function and thresholds of CLOUDSC2 change with LPHYLIN, but SATUR's value does not
     a1 = run_layer("dp", params137, q2, case, 500, 32, 137)
     real = ~np.isnan(a0["pqsat"])
@@ -75,7 +90,7 @@ def test_satur_lphylin_form_equals_foeewm(params137, case):
 def test_layer_is_cloudsc2_on_its_pqsat(params137, case, lphylin, ktdia):
     p, q2 = params137, params2(params137, lphylin)
     lay = run_layer("dp", p, q2, case, 500, 32, 137, ktdia)
-    c2 = {c2n: case[ln].copy() for ln, c2n in C2.LAYER_FROM_CLOUDSC2.items()}
+    c2 = {c2n: case[ln].copy() for ln, c2n in C2.LAYER_FROM_CLOUDSC2.items()}  # noqa: F841
     c2["pqs"] = np.where(np.isnan(lay["pqsat"]), 0.3, lay["pqsat"])  # levels above KTDIA are not read
     for name, kind, role in C2.CLOUDSC2_ARRAYS:
         if role == "o":
@@ -171,3 +186,152 @@ def test_gpu_layer_optional_outputs_null(params137):
     part = gpu_layer(p, q2, host, ngptot, nproma, klev, "sp", drop=opt)
     for n in part:
         np.testing.assert_array_equal(part[n], full[n], err_msg=n)
+
+
+# ----------------------------------------------------------------------------------------------------------------
+# CLOUDST
+# ----------------------------------------------------------------------------------------------------------------
+def st_inputs(c2):
+    return {n: c2[n].copy() for n, k, r in C2.CLOUDST_ARRAYS if r == "i"}
+
+
+def run_cloudst(prec, p, q2, inp, ngptot, nproma, klev, ktdia=1, ldrain1d=False):
+    dt = np.float64 if prec == "dp" else np.float32
+    ngpblks = (ngptot + nproma - 1) // nproma
+    a = {k: v.astype(dt) for k, v in inp.items()}
+    for n, kind, role in C2.CLOUDST_ARRAYS:
+        if role == "o":
+            a[n] = np.full(C2.blocked_shape(kind, nproma, klev, ngpblks), np.nan, dtype=dt)
+    oracle.cloudst_blocked(prec, p, q2, C2.reference_eta(klev), ngptot, nproma, ktdia, klev, ldrain1d, PTSPHY, a,
+                           nthreads=8)
+    return a
+
+
+@pytest.fixture(scope="module")
+def st_case(params137):
+    return st_inputs(C2.synth_host(params137, 23, 512, 32, 137))
+
+
+@pytest.mark.parametrize("lphylin,levap", [(0, 0), (0, 1), (1, 0)])
+def test_cloudst_no_supersaturation_left(params137, st_case, lphylin, levap):
+    p = params137
+    a = run_cloudst("dp", p, params2(p, lphylin, levap, 0), st_case, 512, 32, 137)
+    if lphylin:
+        # CUADJTQS takes its phase from the temperature it is given: exclude warm points that the evaporating
+        # detrainment cools across RTT before the adjustment (the bookkeeping of final_rh covers the rest)
+        t1 = a["ptm1"] + PTSPHY * a["pgtent"]
+        cool = p.rlvtt / p.rcpd * a["plude"] * p.rg / np.diff(a["paphp1"], axis=1) * PTSPHY
+        keep = ~((t1 > p.rtt) & (t1 - cool < p.rtt + 1.0))
+        a = {k: (np.where(keep, v, np.nan) if v.shape == keep.shape else v) for k, v in a.items()}
+    rh = final_rh(p, a, bool(lphylin))
+    rh = rh[~np.isnan(rh)]
+    assert rh.max() < 1.005
+    assert (rh > 0.999).sum() > 1000
+
+
+def test_cloudst_dry_column(params137, st_case):
+    a = {k: v.copy() for k, v in st_case.items()}
+    a["pgtenq"][...] = 0.0
+    a["plude"][...] = 0.0
+    a["pqm1"] = 0.05 * a["pqs"]
+    for q2 in (params2(params137, 0, 0, 0), params2(params137, 1, 1, 0)):
+        out = run_cloudst("dp", params137, q2, a, 512, 32, 137)
+        for n, k, r in C2.CLOUDST_ARRAYS:
+            if r == "o":
+                assert np.all(out[n] == 0.0), n
+
+
+def test_cloudst_water_phase_split(params137, st_case):
+    p = params137
+    a = run_cloudst("dp", p, params2(p, 0, 0, 0), st_case, 512, 32, 137)
+    t1 = a["ptm1"] + PTSPHY * a["pgtent"]
+    assert np.all(a["pqlwp"] >= 0.0) and np.all(a["pqiwp"] >= 0.0)
+    # the phase follows the temperature after melting, at most a few K below the first guess
+    warm, cold = t1 > p.rtt + 5.0, t1 < p.rtice
+    assert np.all(a["pqiwp"][warm] == 0.0) and np.all(a["pqlwp"][cold] == 0.0)
+    assert (a["pqlwp"][warm] > 0).any() and (a["pqiwp"][cold] > 0).any()
+
+
+def test_cloudst_cover_sweep(params137, st_case):
+    p, k, n = params137, 120, 64
+    a = {key: np.ascontiguousarray(np.broadcast_to(v[:1, :, :1], (1, v.shape[1], n))) for key, v in st_case.items()}
+    a["pgtenq"][0, k, :] = 0.0
+    a["plude"][0, k, :] = 0.0
+    qs = a["pqs"][0, k, 0]
+    qt = np.sort(np.r_[np.linspace(0.0, 1.3 * qs, n - 1), qs * (1.0 - 1e-9)])
+    a["pqm1"][0, k, :] = qt
+    out = run_cloudst("dp", p, params2(p, 0, 0, 0), a, n, n, 137)
+    clc = out["pclc"][0, k]
+    assert np.all(np.diff(clc) >= 0.0)
+    assert np.all(clc[qt <= 0.5 * qs] == 0.0)  # the critical humidity is at least 0.5
+    assert np.all(clc[qt >= qs] == 1.0)
+    assert ((clc > 0.0) & (clc < 1.0)).sum() >= 5
+    # below eta 0.75 the critical humidity falls from 0.9 at the surface as 0.9 - 0.4 sqrt((1 - eta)/0.25)
+    # (cloudst.F90:283-286): cover starts exactly there
+    eta = C2.reference_eta(137)[k]
+    assert eta >= 0.75
+    crh = 0.9 + (0.5 - 0.9) * np.sqrt((1.0 - eta) / 0.25)
+    assert np.all(clc[qt <= crh * qs * (1 - 1e-9)] == 0.0) and np.all(clc[qt > crh * qs * (1 + 1e-9)] > 0.0)
+
+
+@pytest.mark.parametrize("ktdia", [1, 5])
+def test_layer_cloudst_branch(params137, case, ktdia):
+    p, q2 = params137, params2(params137, 0, 1, 0)
+    lay = run_layer("dp", p, q2, case, 500, 32, 137, ktdia)
+    st = {c2n: case[ln].copy() for ln, c2n in C2.LAYER_FROM_CLOUDSC2.items()}
+    st["pqs"] = np.where(np.isnan(lay["pqsat"]), 0.3, lay["pqsat"])
+    st = run_cloudst("dp", p, q2, {n: st[n] for n, k, r in C2.CLOUDST_ARRAYS if r == "i"}, 500, 32, 137, ktdia)
+    for ln, sn in [("ptendency_loc_t", "ptent"), ("ptendency_loc_q", "ptenq"), ("pclc", "pclc"),
+                   ("pqlwp", "pqlwp"), ("pqiwp", "pqiwp"), ("pfplsl", "pfplsl"), ("pfplsn", "pfplsn"),
+                   ("pcovptot", "pcovptot")]:
+        np.testing.assert_array_equal(lay[ln], st[sn], err_msg=ln)
+    real = np.zeros((16, 32), bool)
+    real.reshape(-1)[:500] = True
+    for tend, state, diag in [("ptendency_loc_a", "pa", "pclc"), ("ptendency_loc_l", "pl", "pqlwp"),
+                              ("ptendency_loc_i", "pi", "pqiwp")]:
+        lhs = (lay[state] + PTSPHY * lay[tend]).transpose(0, 2, 1)[real]
+        np.testing.assert_allclose(lhs, lay[diag].transpose(0, 2, 1)[real], rtol=0, atol=1e-14, err_msg=tend)
+    assert np.all(lay["ptendency_loc_r"] == 0.0) and np.all(lay["ptendency_loc_s"] == 0.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lphylin,levap,ktdia,ngptot,nproma,klev", [(0, 0, 1, 2048, 64, 137), (0, 1, 1, 2048, 64, 137),
+                                                                    (1, 0, 5, 777, 32, 91)])
+def test_gpu_cloudst_fp64_matches_oracle(params137, params91, lphylin, levap, ktdia, ngptot, nproma, klev):
+    import torch
+
+    p = params137 if klev == 137 else params91
+    q2 = params2(p, lphylin, levap, 0)
+    inp = st_inputs(C2.synth_host(p, 29, ngptot, nproma, klev, "dp"))
+    print(f"cloudst dp lphylin={lphylin} levap={levap} ktdia={ktdia} {ngptot}x{klev}", flush=True)
+    ngpblks = (ngptot + nproma - 1) // nproma
+    f = {k: torch.from_numpy(v).to("cuda") for k, v in inp.items()}
+    for n, kind, role in C2.CLOUDST_ARRAYS:
+        if role == "o":
+            f[n] = torch.full(C2.blocked_shape(kind, nproma, klev, ngpblks), float("nan"), dtype=torch.float64,
+                              device="cuda")
+    ceta = torch.from_numpy(C2.reference_eta(klev)).to("cuda")
+    C2.cloudst_blocked(f, p, q2, ceta, ngptot, nproma, klev, PTSPHY, "dp", ktdia)
+    torch.cuda.synchronize()
+    ref = run_cloudst("dp", p, q2, inp, ngptot, nproma, klev, ktdia)
+    for n, kind, role in C2.CLOUDST_ARRAYS:
+        if role == "o":
+            g = f[n].cpu().numpy()
+            np.testing.assert_array_equal(np.isnan(g), np.isnan(ref[n]), err_msg=n)
+            assert relrms(g, ref[n]) <= 1e-12, n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["dp", "sp"])
+def test_gpu_layer_cloudst_matches_oracle(params137, prec):
+    p, ngptot, nproma, klev = params137, 4096, 128, 137
+    q2 = params2(p, 0, 1, 0)
+    host = layer_case(p, 31, ngptot, nproma, klev, prec)
+    print(f"cloud_s_layer cloudst {prec}", flush=True)
+    out = gpu_layer(p, q2, host, ngptot, nproma, klev, prec)
+    r64 = run_layer("dp", p, q2, host, ngptot, nproma, klev)
+    r32 = run_layer(prec, p, q2, host, ngptot, nproma, klev)
+    for n in OUTS:
+        np.testing.assert_array_equal(np.isnan(out[n]), np.isnan(r64[n]), err_msg=n)
+        tol = 1e-12 if prec == "dp" else 2.0 * relrms(r32[n], r64[n]) + 1e-6
+        assert relrms(out[n], r64[n]) <= tol, n
