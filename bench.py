@@ -391,15 +391,16 @@ def layer_bytes_per_column(klev, word=4):
     return word * (14 * klev + (klev + 1) + 10 * klev + 4 * (klev + 1) + 2)
 
 
-def layer_bench(params, nproma, steps):
-    """CLOUD_S_LAYER (SATUR + CLOUDSC2 + wrap-up, one launch) in FP32 on 655,360 x L137, every optional output
-    written; kernel time from HIP events on the launch stream."""
+def layer_bench(params, nproma, steps, lepcld2=1):
+    """CLOUD_S_LAYER (SATUR + CLOUDSC2 or CLOUDST + wrap-up, one launch) in FP32 on 655,360 x L137, every optional
+    output written; kernel time from HIP events on the launch stream."""
     import numpy as np
     import torch
 
     from openifs_amd import cloudsc2 as C2
 
     q2 = C2.default_params2(params)
+    q2.lepcld2 = lepcld2
     src = C2.synth_device(params, 20261207, NGPTOT, nproma, KLEV, "sp", "cuda:0")
     f = {lay: src[c2] for lay, c2 in C2.LAYER_FROM_CLOUDSC2.items()}
     f["pa"] = torch.rand(src["pqm1"].shape, device="cuda:0")
@@ -423,13 +424,14 @@ def layer_bench(params, nproma, steps):
     ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
     del f, src
     torch.cuda.empty_cache()
-    bpc = layer_bytes_per_column(KLEV)
+    # CLOUDST reads neither PMFU/PMFD nor the condensate tendencies, and also writes PQLWP/PQIWP
+    bpc = layer_bytes_per_column(KLEV) + (0 if lepcld2 else 4 * (2 * KLEV - 4 * KLEV))
     gbs = bpc * NGPTOT / (ms / 1e3) / 1e9
     return {"kernel_ms": ms, "columns_per_s": NGPTOT / (ms / 1e3), "algorithmic_bytes_per_column": bpc,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS, "traffic": None},
-            "workload": f"CLOUD_S_LAYER FP32 (SATUR, CLOUDSC2, wrap-up), NLEV={KLEV}, {NGPTOT} columns, "
-                        f"NPROMA={nproma}"}
+            "workload": f"CLOUD_S_LAYER FP32 (SATUR, {'CLOUDSC2' if lepcld2 else 'CLOUDST'}, wrap-up), NLEV={KLEV}, "
+                        f"{NGPTOT} columns, NPROMA={nproma}"}
 
 
 def cloudsc2_bench(params, nproma, steps, cpu_threads, cpu_seconds):
@@ -494,7 +496,8 @@ def cloudsc2_bench(params, nproma, steps, cpu_threads, cpu_seconds):
         res["cpu_baseline"] = {"value": passes * nb * nproma / dt, "unit": "columns/s", "cores": cpu_threads,
                                "kind": "port", "sample": f"{nb * nproma} columns x {passes} passes ({dt:.1f} s), "
                                                          "FP32 oracle, OpenMP over NPROMA blocks"}
-    res["layer"] = layer_bench(params, nproma, steps)
+    res["layer"] = layer_bench(params, nproma, steps, lepcld2=1)
+    res["layer_cloudst"] = layer_bench(params, nproma, steps, lepcld2=0)
     return res
 
 
