@@ -483,12 +483,13 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudst_kernel(const C2Const<R
     }
   }
   // outputs above KTDIA (:311-330): cover, water paths and PCOVPTOT zero; fluxes zero at the top
-  auto wrap_state = [&](int k, R clc, R lwp, R iwp) {  // CLOUD_S_LAYER's tendencies from ZPATMP/ZPLTMP/ZPLSMP
+  // CLOUD_S_LAYER's tendencies from ZPATMP/ZPLTMP/ZPLSMP and the state PA/PL/PI of level k
+  auto wrap_state = [&](int k, R clc, R lwp, R iwp, R a, R l, R i) {
     if constexpr (LAYER) {
       const uint32_t j = at2(k);
-      if (p.ptena) ST(p.ptena, j, jo, c.zqtmst * (clc - LD(p.pa, j, jo)));
-      ST(p.ptenl, j, jo, c.zqtmst * (lwp - LD(p.pl, j, jo)));
-      ST(p.pteni, j, jo, c.zqtmst * (iwp - LD(p.pi, j, jo)));
+      if (p.ptena) ST(p.ptena, j, jo, c.zqtmst * (clc - a));
+      ST(p.ptenl, j, jo, c.zqtmst * (lwp - l));
+      ST(p.pteni, j, jo, c.zqtmst * (iwp - i));
     }
   };
   for (int k = 0; k < d.ktdia - 1; ++k) {
@@ -496,7 +497,9 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudst_kernel(const C2Const<R
     if (!LAYER || p.pqiwp) ST(p.pqiwp, at2(k), jo, R(0));
     if (!LAYER || p.pqlwp) ST(p.pqlwp, at2(k), jo, R(0));
     ST(p.pcovptot, at2(k), jo, R(0));
-    wrap_state(k, R(0), R(0), R(0));
+    if constexpr (LAYER)
+      wrap_state(k, R(0), R(0), R(0), p.ptena ? LD(p.pa, at2(k), jo) : R(0), LD(p.pl, at2(k), jo),
+                 LD(p.pi, at2(k), jo));
   }
   ST(p.pfplsl, ath(0), jo, R(0));
   ST(p.pfplsn, ath(0), jo, R(0));
@@ -508,17 +511,37 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudst_kernel(const C2Const<R
   }
   const R paph_surf = LD(p.paphp1, ath(klev), jo);
 
+  // the inputs of level k+1 are loaded while level k computes
+  struct Level {
+    R tm1, gtent, qm1, gtenq, paph1, app1, lude, qs, lu_next, a, l, i;
+  };
+  auto load = [&](int k) {
+    const uint32_t j = at2(k);
+    Level v;
+    v.tm1 = LD(p.ptm1, j, jo); v.gtent = LD(p.pgtent, j, jo); v.qm1 = LD(p.pqm1, j, jo);
+    v.gtenq = LD(p.pgtenq, j, jo); v.paph1 = LD(p.paphp1, ath(k + 1), jo); v.app1 = LD(p.papp1, j, jo);
+    v.lude = LD(p.plude, j, jo);
+    v.qs = LAYER ? R(0) : LD(p.pqs, j, jo);
+    v.lu_next = k < klev - 1 ? LD(p.plu, j + lstep, jo) : R(0);
+    v.a = (LAYER && p.ptena) ? LD(p.pa, j, jo) : R(0);
+    v.l = LAYER ? LD(p.pl, j, jo) : R(0);
+    v.i = LAYER ? LD(p.pi, j, jo) : R(0);
+    return v;
+  };
   R zrfl = R(0), zsfl = R(0), zcovptot = R(0);
   R paph0 = LD(p.paphp1, ath(d.ktdia - 1), jo);
+  Level nxt = load(d.ktdia - 1);
   for (int k = d.ktdia - 1; k < klev; ++k) {
+    const Level cur = nxt;
+    if (k + 1 < klev) nxt = load(k + 1);
     const uint32_t i2 = at2(k);
     const R ceta = p.ceta[k];
-    R ztp1 = LD(p.ptm1, i2, jo) + ptsphy * LD(p.pgtent, i2, jo);
-    R zqp1 = LD(p.pqm1, i2, jo) + ptsphy * LD(p.pgtenq, i2, jo);
-    const R paph1 = LD(p.paphp1, ath(k + 1), jo);
+    R ztp1 = cur.tm1 + ptsphy * cur.gtent;
+    R zqp1 = cur.qm1 + ptsphy * cur.gtenq;
+    const R paph1 = cur.paph1;
     const R zdp = paph1 - paph0;
-    const R papp1 = LD(p.papp1, i2, jo), plude = LD(p.plude, i2, jo);
-    const R pqs = LAYER ? satur(c, ztp1, papp1) : LD(p.pqs, i2, jo);
+    const R papp1 = cur.app1, plude = cur.lude;
+    const R pqs = LAYER ? satur(c, ztp1, papp1) : cur.qs;
     if constexpr (LAYER) {
       if (p.pqsat) ST(p.pqsat, i2, jo, pqs);
     }
@@ -575,7 +598,7 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudst_kernel(const C2Const<R
     const R zgdp = m_div(c.rg, paph1 - paph0);
     const R zlude = plude * ptsphy * zgdp;
     if (k < klev - 1) {
-      const R plu_n = LD(p.plu, i2 + lstep, jo);
+      const R plu_n = cur.lu_next;
       if (zlude >= c.rlmin && plu_n >= zeps2) {
         pclc = pclc + (R(1.0) - pclc) * (R(1.0) - m_exp(-m_div(zlude, plu_n)));
         zqlwc = zqlwc + zlude;
@@ -673,7 +696,7 @@ __global__ __launch_bounds__(CLOUDSC2_BLOCK) void cloudst_kernel(const C2Const<R
     if (!LAYER || p.pqlwp) ST(p.pqlwp, i2, jo, pqlwp);
     if (!LAYER || p.pqiwp) ST(p.pqiwp, i2, jo, pqiwp);
     ST(p.pcovptot, i2, jo, pcovptot);
-    wrap_state(k, pclc, pqlwp, pqiwp);
+    wrap_state(k, pclc, pqlwp, pqiwp, cur.a, cur.l, cur.i);
     const uint32_t ih = ath(k + 1);
     ST(p.pfplsl, ih, jo, zrfln);
     ST(p.pfplsn, ih, jo, zsfln);
