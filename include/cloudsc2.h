@@ -64,7 +64,19 @@ int cloudsc2_sp_c(CLOUDSC2_ARGS(float));
 int cloudsc2_blocked_dp_c(CLOUDSC2_BLOCKED_ARGS(double));
 int cloudsc2_blocked_sp_c(CLOUDSC2_BLOCKED_ARGS(float));
 
-/* CLOUDST (cloudst.F90:9-16), the diagnostic scheme, over all blocks of device-resident blocked fields:
+/* CLOUDST (cloudst.F90:9-16), the diagnostic scheme, one NPROMA block of host arrays (Fortran order): the
+ * reference's argument list after the derived types, as cloudsc2_*_c. */
+#define CLOUDST_ARGS(R)                                                                                 \
+  const cloudsc_params_t *ydparams, const cloudsc2_params_t *ydparams2, const R *ceta, int kidia,        \
+      int kfdia, int klon, int ktdia, int klev, int ldrain1d, R ptsphy, const R *paphp1, const R *papp1, \
+      const R *pqm1, const R *pqs, const R *ptm1, const R *plude, const R *plu, R *ptent,               \
+      const R *pgtent, R *ptenq, const R *pgtenq, R *pclc, R *pqiwp, R *pqlwp, R *pfplsl, R *pfplsn,   \
+      R *pfhpsl, R *pfhpsn, R *pcovptot
+
+int cloudst_dp_c(CLOUDST_ARGS(double));
+int cloudst_sp_c(CLOUDST_ARGS(float));
+
+/* CLOUDST over all blocks of device-resident blocked fields:
  * CLOUDSC2's arrays without the condensate and mass-flux ones, plus the diagnostic ice and liquid water PQIWP and
  * PQLWP; every array is required. */
 #define CLOUDST_BLOCKED_ARGS(R)                                                                         \

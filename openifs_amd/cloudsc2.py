@@ -261,3 +261,28 @@ def layer_host(c2_inputs, pa, precision="dp"):
         if role == "o":
             res[name] = np.full(layer_shape(kind, nproma, klev, ngpblks), np.nan, dtype=dt)
     return res
+
+
+def CLOUDST(params, params2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d, ptsphy, arrays, precision="sp",
+            func=None):
+    """One NPROMA block of CLOUDST through cloudst_{dp,sp}_c (or ``func`` with the same signature); ``arrays`` maps
+    the CLOUDST_ARRAYS names to Fortran-ordered numpy arrays, outputs updated in place."""
+    dt = _real(precision)
+    ceta = np.ascontiguousarray(ceta, dtype=dt)
+    args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.ctypes.data), ctypes.c_int(kidia),
+            ctypes.c_int(kfdia), ctypes.c_int(klon), ctypes.c_int(ktdia), ctypes.c_int(klev),
+            ctypes.c_int(int(bool(ldrain1d))), _ctype_real(precision)(ptsphy)]
+    for name, kind, role in CLOUDST_ARRAYS:
+        a = arrays[name]
+        want = (klon, klev + 1) if kind == "h" else (klon, klev)
+        if a.dtype != dt or not a.flags.f_contiguous or a.shape != want:
+            raise TypeError(f"{name}: need a Fortran-contiguous {np.dtype(dt).name} array of shape {want}")
+        args.append(ctypes.c_void_p(a.ctypes.data))
+    if func is None:
+        fn = f"cloudst_{precision}_c"
+        C._check(fn, getattr(C.lib(), fn)(*args))
+    else:
+        rc = func(*args)
+        if rc != 0:
+            raise RuntimeError(f"CLOUDST returned {rc}")
+    return arrays

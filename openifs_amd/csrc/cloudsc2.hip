@@ -804,39 +804,19 @@ struct Workspace {
 };
 thread_local Workspace g_ws;
 
-template <typename R>
-int per_block(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* ceta, int kidia, int kfdia, int klon,
-              int ktdia, int klev, int ldrain1d, R ptsphy, const R* paphp1, const R* papp1, const R* pqm1,
-              const R* pqs, const R* ptm1, const R* pl, const R* pi, const R* plude, const R* plu, const R* pmfu,
-              const R* pmfd, R* ptent, const R* pgtent, R* ptenq, const R* pgtenq, R* ptenl, const R* pgtenl,
-              R* pteni, const R* pgteni, R* pclc, R* pfplsl, R* pfplsn, R* pfhpsl, R* pfhpsn, R* pcovptot) {
-  int rc = check(q, q2, ktdia, klev);
-  if (rc) return rc;
-  if (klon < 1 || kidia < 1 || kfdia < kidia || kfdia > klon)
-    return fail(CLOUDSC_EDIMS, "cloudsc2: need 1 <= KIDIA <= KFDIA <= KLON");
-  const void* req[] = {ceta,  paphp1, papp1, pqm1,  pqs,    ptm1,   pl,    pi,     plude, plu,   pmfu,  pmfd,
-                       ptent, pgtent, ptenq, pgtenq, ptenl, pgtenl, pteni, pgteni, pclc,  pfplsl, pfplsn, pfhpsl,
-                       pfhpsn, pcovptot};
-  for (const void* x : req)
-    if (!x) return fail(CLOUDSC_ENULL, "cloudsc2: NULL array");
-  rc = cloudsc_capi::check_device();
-  if (rc) return rc;
-  const size_t n2 = (size_t)klon * klev, nh = (size_t)klon * (klev + 1);
-  struct Arr {
-    const void* host;
-    void* out;
-    size_t n;
-  };
-  // outputs are copied in too, so that what CLOUDSC2 leaves untouched comes back unchanged
-  Arr arrs[] = {{ceta, nullptr, (size_t)klev}, {paphp1, nullptr, nh}, {papp1, nullptr, n2}, {pqm1, nullptr, n2},
-                {pqs, nullptr, n2},   {ptm1, nullptr, n2},  {pl, nullptr, n2},     {pi, nullptr, n2},
-                {plude, nullptr, n2}, {plu, nullptr, n2},   {pmfu, nullptr, n2},   {pmfd, nullptr, n2},
-                {ptent, ptent, n2},   {pgtent, nullptr, n2}, {ptenq, ptenq, n2},   {pgtenq, nullptr, n2},
-                {ptenl, ptenl, n2},   {pgtenl, nullptr, n2}, {pteni, pteni, n2},   {pgteni, nullptr, n2},
-                {pclc, pclc, n2},     {pfplsl, pfplsl, nh},  {pfplsn, pfplsn, nh}, {pfhpsl, pfhpsl, nh},
-                {pfhpsn, pfhpsn, nh}, {pcovptot, pcovptot, n2}};
-  const int narr = sizeof(arrs) / sizeof(arrs[0]);
-  size_t off[sizeof(arrs) / sizeof(arrs[0])], total = 0;
+// One host array of a per-block call: copied to the workspace, and back to `out` when that is set.
+struct HostArr {
+  const void* host;
+  void* out;
+  size_t n;
+};
+
+// Copies the arrays into this thread's device workspace, runs `launch_fn(D, stream)` with D(i) the device copy of
+// array i, copies the outputs back and waits.
+template <typename R, typename F>
+int run_on_device(const char* what, const HostArr* arrs, int narr, F launch_fn) {
+  size_t off[64], total = 0;
+  if (narr > 64) return fail(CLOUDSC_EDIMS, what);
   for (int i = 0; i < narr; ++i) {
     off[i] = total;
     total += (arrs[i].n * sizeof(R) + 255) & ~(size_t)255;
@@ -864,17 +844,82 @@ int per_block(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* c
     e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * sizeof(R), hipMemcpyHostToDevice, w.stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: copy to device", e);
   auto D = [&](int i) { return (R*)(base + off[i]); };
-  C2Ptrs<R> p{D(0),  D(1),  D(2),  D(3),  D(4),  D(5),  D(6),  D(7),  D(8),  D(9),  D(10), D(11), D(12),
-              D(13), D(14), D(15), D(16), D(17), D(18), D(19), D(20), D(21), D(22), D(23), D(24), D(25)};
-  C2Dims d{kfdia, klon, klev, 1, ktdia, kidia, kfdia};
-  e = launch<R>(*q, *q2, ldrain1d != 0, (double)ptsphy, p, d, w.stream);
-  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: kernel launch", e);
+  e = launch_fn(D, w.stream);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, what, e);
   for (int i = 0; i < narr && e == hipSuccess; ++i)
     if (arrs[i].out)
       e = hipMemcpyAsync(arrs[i].out, base + off[i], arrs[i].n * sizeof(R), hipMemcpyDeviceToHost, w.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: copy to host", e);
   return CLOUDSC_OK;
+}
+
+template <typename R>
+int per_block(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* ceta, int kidia, int kfdia, int klon,
+              int ktdia, int klev, int ldrain1d, R ptsphy, const R* paphp1, const R* papp1, const R* pqm1,
+              const R* pqs, const R* ptm1, const R* pl, const R* pi, const R* plude, const R* plu, const R* pmfu,
+              const R* pmfd, R* ptent, const R* pgtent, R* ptenq, const R* pgtenq, R* ptenl, const R* pgtenl,
+              R* pteni, const R* pgteni, R* pclc, R* pfplsl, R* pfplsn, R* pfhpsl, R* pfhpsn, R* pcovptot) {
+  int rc = check(q, q2, ktdia, klev);
+  if (rc) return rc;
+  if (klon < 1 || kidia < 1 || kfdia < kidia || kfdia > klon)
+    return fail(CLOUDSC_EDIMS, "cloudsc2: need 1 <= KIDIA <= KFDIA <= KLON");
+  const void* req[] = {ceta,  paphp1, papp1, pqm1,  pqs,    ptm1,   pl,    pi,     plude, plu,   pmfu,  pmfd,
+                       ptent, pgtent, ptenq, pgtenq, ptenl, pgtenl, pteni, pgteni, pclc,  pfplsl, pfplsn, pfhpsl,
+                       pfhpsn, pcovptot};
+  for (const void* x : req)
+    if (!x) return fail(CLOUDSC_ENULL, "cloudsc2: NULL array");
+  rc = cloudsc_capi::check_device();
+  if (rc) return rc;
+  const size_t n2 = (size_t)klon * klev, nh = (size_t)klon * (klev + 1);
+  // outputs are copied in too, so that what CLOUDSC2 leaves untouched comes back unchanged
+  HostArr arrs[] = {{ceta, nullptr, (size_t)klev}, {paphp1, nullptr, nh}, {papp1, nullptr, n2}, {pqm1, nullptr, n2},
+                    {pqs, nullptr, n2},   {ptm1, nullptr, n2},  {pl, nullptr, n2},     {pi, nullptr, n2},
+                    {plude, nullptr, n2}, {plu, nullptr, n2},   {pmfu, nullptr, n2},   {pmfd, nullptr, n2},
+                    {ptent, ptent, n2},   {pgtent, nullptr, n2}, {ptenq, ptenq, n2},   {pgtenq, nullptr, n2},
+                    {ptenl, ptenl, n2},   {pgtenl, nullptr, n2}, {pteni, pteni, n2},   {pgteni, nullptr, n2},
+                    {pclc, pclc, n2},     {pfplsl, pfplsl, nh},  {pfplsn, pfplsn, nh}, {pfhpsl, pfhpsl, nh},
+                    {pfhpsn, pfhpsn, nh}, {pcovptot, pcovptot, n2}};
+  return run_on_device<R>("cloudsc2", arrs, sizeof(arrs) / sizeof(arrs[0]), [&](auto D, hipStream_t st) {
+    C2Ptrs<R> p{D(0),  D(1),  D(2),  D(3),  D(4),  D(5),  D(6),  D(7),  D(8),  D(9),  D(10), D(11), D(12),
+                D(13), D(14), D(15), D(16), D(17), D(18), D(19), D(20), D(21), D(22), D(23), D(24), D(25)};
+    C2Dims d{kfdia, klon, klev, 1, ktdia, kidia, kfdia};
+    return launch<R>(*q, *q2, ldrain1d != 0, (double)ptsphy, p, d, st);
+  });
+}
+
+// CLOUDST on one block of host arrays (the per-block drop-in, as cloudsc2_*_c).
+template <typename R>
+int per_block_st(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* ceta, int kidia, int kfdia,
+                 int klon, int ktdia, int klev, int ldrain1d, R ptsphy, const R* paphp1, const R* papp1,
+                 const R* pqm1, const R* pqs, const R* ptm1, const R* plude, const R* plu, R* ptent,
+                 const R* pgtent, R* ptenq, const R* pgtenq, R* pclc, R* pqiwp, R* pqlwp, R* pfplsl, R* pfplsn,
+                 R* pfhpsl, R* pfhpsn, R* pcovptot) {
+  int rc = check(q, q2, ktdia, klev);
+  if (rc) return rc;
+  if (klon < 1 || kidia < 1 || kfdia < kidia || kfdia > klon)
+    return fail(CLOUDSC_EDIMS, "cloudst: need 1 <= KIDIA <= KFDIA <= KLON");
+  const void* req[] = {ceta,  paphp1, papp1,  pqm1,  pqs,   ptm1,   plude,  plu,    ptent,  pgtent,
+                       ptenq, pgtenq, pclc,   pqiwp, pqlwp, pfplsl, pfplsn, pfhpsl, pfhpsn, pcovptot};
+  for (const void* x : req)
+    if (!x) return fail(CLOUDSC_ENULL, "cloudst: NULL array");
+  rc = cloudsc_capi::check_device();
+  if (rc) return rc;
+  const size_t n2 = (size_t)klon * klev, nh = (size_t)klon * (klev + 1);
+  HostArr arrs[] = {{ceta, nullptr, (size_t)klev}, {paphp1, nullptr, nh}, {papp1, nullptr, n2}, {pqm1, nullptr, n2},
+                    {pqs, nullptr, n2},     {ptm1, nullptr, n2},   {plude, nullptr, n2},  {plu, nullptr, n2},
+                    {ptent, ptent, n2},     {pgtent, nullptr, n2}, {ptenq, ptenq, n2},    {pgtenq, nullptr, n2},
+                    {pclc, pclc, n2},       {pqiwp, pqiwp, n2},    {pqlwp, pqlwp, n2},    {pfplsl, pfplsl, nh},
+                    {pfplsn, pfplsn, nh},   {pfhpsl, pfhpsl, nh},  {pfhpsn, pfhpsn, nh},  {pcovptot, pcovptot, n2}};
+  return run_on_device<R>("cloudst", arrs, sizeof(arrs) / sizeof(arrs[0]), [&](auto D, hipStream_t st) {
+    C2Ptrs<R> p{};
+    p.ceta = D(0); p.paphp1 = D(1); p.papp1 = D(2); p.pqm1 = D(3); p.pqs = D(4); p.ptm1 = D(5); p.plude = D(6);
+    p.plu = D(7); p.ptent = D(8); p.pgtent = D(9); p.ptenq = D(10); p.pgtenq = D(11); p.pclc = D(12);
+    p.pqiwp = D(13); p.pqlwp = D(14); p.pfplsl = D(15); p.pfplsn = D(16); p.pfhpsl = D(17); p.pfhpsn = D(18);
+    p.pcovptot = D(19);
+    C2Dims d{kfdia, klon, klev, 1, ktdia, kidia, kfdia};
+    return launch<R>(*q, *q2, ldrain1d != 0, (double)ptsphy, p, d, st, false, true);
+  });
 }
 
 template <typename R>
@@ -969,6 +1014,11 @@ int cloudst_blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, int 
   return cloudsc2_gpu::layer_blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d, \
                                         tsphy, p, stream)
 
+#define CLOUDST_PER_BLOCK(R)                                                                                   \
+  return cloudsc2_gpu::per_block_st<R>(ydparams, ydparams2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d, ptsphy, \
+                                       paphp1, papp1, pqm1, pqs, ptm1, plude, plu, ptent, pgtent, ptenq, pgtenq, pclc,  \
+                                       pqiwp, pqlwp, pfplsl, pfplsn, pfhpsl, pfhpsn, pcovptot)
+
 #define CLOUDST_BLOCKED(R)                                                                                      \
   cloudsc2_gpu::C2Ptrs<R> p{};                                                                                     \
   p.ceta = ceta; p.paphp1 = paphp1; p.papp1 = papp1; p.pqm1 = pqm1; p.pqs = pqs; p.ptm1 = ptm1; p.plude = plude;   \
@@ -998,6 +1048,8 @@ int cloudsc2_dp_c(CLOUDSC2_ARGS(double)) { CLOUDSC2_PER_BLOCK(double); }
 int cloudsc2_sp_c(CLOUDSC2_ARGS(float)) { CLOUDSC2_PER_BLOCK(float); }
 int cloudsc2_blocked_dp_c(CLOUDSC2_BLOCKED_ARGS(double)) { CLOUDSC2_BLOCKED(double); }
 int cloudsc2_blocked_sp_c(CLOUDSC2_BLOCKED_ARGS(float)) { CLOUDSC2_BLOCKED(float); }
+int cloudst_dp_c(CLOUDST_ARGS(double)) { CLOUDST_PER_BLOCK(double); }
+int cloudst_sp_c(CLOUDST_ARGS(float)) { CLOUDST_PER_BLOCK(float); }
 int cloudst_blocked_dp_c(CLOUDST_BLOCKED_ARGS(double)) { CLOUDST_BLOCKED(double); }
 int cloudst_blocked_sp_c(CLOUDST_BLOCKED_ARGS(float)) { CLOUDST_BLOCKED(float); }
 int cloud_s_layer_blocked_dp_c(CLOUD_S_LAYER_BLOCKED_ARGS(double)) { CLOUD_S_LAYER_BLOCKED(double); }

@@ -515,6 +515,17 @@ int cloudst_blocked(const cloudsc_params_t* p, const cloudsc2_params_t* p2, cons
 
 extern "C" {
 
+#define ORACLE_CLOUDST_BLOCK_CALL(R)                                                                            \
+  int rc = cloudsc2_check<R>(ydparams, ydparams2, kidia, kfdia, klon, ktdia, klev);                               \
+  if (rc) return rc;                                                                                              \
+  const oracle::CloudstArgs<R> a{kidia, kfdia, klon,  ktdia,  klev,   ldrain1d != 0, ptsphy, ceta,   paphp1, papp1, \
+                                 pqm1,  pqs,   ptm1,  plude,  plu,    ptent,         pgtent, ptenq,  pgtenq, pclc,  \
+                                 pqiwp, pqlwp, pfplsl, pfplsn, pfhpsl, pfhpsn,       pcovptot};                    \
+  oracle::cloudst<R>(*ydparams, *ydparams2, a);                                                                   \
+  return CLOUDSC_OK;
+
+int oracle_cloudst_dp(CLOUDST_ARGS(double)) { ORACLE_CLOUDST_BLOCK_CALL(double) }
+int oracle_cloudst_sp(CLOUDST_ARGS(float)) { ORACLE_CLOUDST_BLOCK_CALL(float) }
 int oracle_cloudst_blocked_dp(ORACLE_CLOUDST_ARGS(double)) { ORACLE_CLOUDST_CALL(double); }
 int oracle_cloudst_blocked_sp(ORACLE_CLOUDST_ARGS(float)) { ORACLE_CLOUDST_CALL(float); }
 int oracle_cloud_s_layer_blocked_dp(ORACLE_LAYER_ARGS(double)) { ORACLE_LAYER_CALL(double); }

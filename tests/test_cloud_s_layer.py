@@ -335,3 +335,35 @@ def test_gpu_layer_cloudst_matches_oracle(params137, prec):
         np.testing.assert_array_equal(np.isnan(out[n]), np.isnan(r64[n]), err_msg=n)
         tol = 1e-12 if prec == "dp" else 2.0 * relrms(r32[n], r64[n]) + 1e-6
         assert relrms(out[n], r64[n]) <= tol, n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["dp", "sp"])
+def test_gpu_cloudst_per_block_drop_in(params137, prec):
+    """cloudst_*_c (host arrays, one block, KIDIA:KFDIA window) equals the oracle of the same precision and leaves the
+    columns outside the window untouched."""
+    p, klon, klev = params137, 48, 137
+    q2 = params2(p, 0, 1, 0)
+    inp = st_inputs(C2.synth_host(p, 37, klon, klon, klev, "dp"))
+    dt = np.float64 if prec == "dp" else np.float32
+
+    def fortran():
+        a = {k: np.asfortranarray(v[0].T.astype(dt)) for k, v in inp.items()}
+        for n, kind, role in C2.CLOUDST_ARRAYS:
+            if role == "o":
+                a[n] = np.full((klon, klev + 1) if kind == "h" else (klon, klev), np.nan, dtype=dt, order="F")
+        return a
+
+    a_gpu, a_ref = fortran(), fortran()
+    ceta = C2.reference_eta(klev)
+    print(f"cloudst per-block {prec}", flush=True)
+    C2.CLOUDST(p, q2, ceta, 3, 45, klon, 2, klev, False, PTSPHY, a_gpu, prec)
+    C2.CLOUDST(p, q2, ceta, 3, 45, klon, 2, klev, False, PTSPHY, a_ref, prec,
+               func=getattr(oracle.lib(), f"oracle_cloudst_{prec}"))
+    for n, kind, role in C2.CLOUDST_ARRAYS:
+        if role != "o":
+            continue
+        g, r = a_gpu[n], a_ref[n]
+        np.testing.assert_array_equal(np.isnan(g), np.isnan(r), err_msg=n)
+        assert np.all(np.isnan(g[:2])) and np.all(np.isnan(g[45:])), n
+        assert relrms(g, r) <= (1e-12 if prec == "dp" else 1e-4), n
