@@ -367,3 +367,45 @@ def test_gpu_cloudst_per_block_drop_in(params137, prec):
         np.testing.assert_array_equal(np.isnan(g), np.isnan(r), err_msg=n)
         assert np.all(np.isnan(g[:2])) and np.all(np.isnan(g[45:])), n
         assert relrms(g, r) <= (1e-12 if prec == "dp" else 1e-4), n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lepcld2", [1, 0])
+def test_gpu_layer_full_size_properties(params137, lepcld2):
+    """655,360 columns at L137 in FP32, both branches: finite outputs, the wrap-up identity
+    PA + dt PTENDENCY_LOC_A = PCLC to FP32 rounding, zero rain and snow tendencies, and a sample of blocks equal
+    to the oracle by the FP32 criterion."""
+    import torch
+
+    p, ngptot, nproma, klev = params137, 655360, 128, 137
+    q2 = params2(p, 0, 1, lepcld2)
+    print(f"cloud_s_layer sp 655360x137 lepcld2={lepcld2}", flush=True)
+    src = C2.synth_device(p, 41, ngptot, nproma, klev, "sp")
+    f = {lay: src[c2] for lay, c2 in C2.LAYER_FROM_CLOUDSC2.items()}
+    f["pa"] = torch.rand(src["pqm1"].shape, device="cuda", generator=torch.Generator(device="cuda").manual_seed(3))
+    ngpblks = ngptot // nproma
+    for name, kind, role, opt in C2.LAYER_ARRAYS:
+        if role == "o":
+            f[name] = torch.full(C2.layer_shape(kind, nproma, klev, ngpblks), float("nan"), device="cuda")
+    ceta = torch.from_numpy(C2.reference_eta(klev)).to("cuda", torch.float32)
+    C2.cloud_s_layer_blocked(f, p, q2, ceta, ngptot, nproma, klev, PTSPHY, "sp")
+    torch.cuda.synchronize()
+    written = [n for n, k, r, o in C2.LAYER_ARRAYS if r == "o" and (lepcld2 == 0 or n not in ("pqiwp", "pqlwp"))]
+    for n in written:
+        assert torch.isfinite(f[n]).all(), n
+    err = (f["pa"] + PTSPHY * f["ptendency_loc_a"] - f["pclc"]).abs().max()
+    assert float(err) < 1e-5
+    assert (f["ptendency_loc_r"] == 0).all() and (f["ptendency_loc_s"] == 0).all()
+    blocks = [0, 2049, 5119]
+    host = {n: f[n][blocks].cpu().numpy().copy() for n, k, r, o in C2.LAYER_ARRAYS if r == "i"}
+    out = {n: f[n][blocks].double().cpu().numpy() for n in written}
+    del f, src
+    torch.cuda.empty_cache()
+    for n, k, r, o in C2.LAYER_ARRAYS:
+        if r == "o":
+            host[n] = np.full(C2.layer_shape(k, nproma, klev, len(blocks)), np.nan, dtype=np.float32)
+    n_cols = len(blocks) * nproma
+    r64 = run_layer("dp", p, q2, host, n_cols, nproma, klev)
+    r32 = run_layer("sp", p, q2, host, n_cols, nproma, klev)
+    for n in written:
+        assert relrms(out[n], r64[n]) <= 2.0 * relrms(r32[n], r64[n]) + 1e-6, n
