@@ -187,3 +187,43 @@ def test_fortran_host_cloudsc2(tmp_path, params137, prec):
         for n in out:
             np.testing.assert_array_equal(np.isnan(out[n]), np.isnan(orc[n]), err_msg=n)
             assert relrms(out[n], orc[n]) <= 1e-12, n
+
+
+@needs_driver
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["dp", "sp"])
+def test_fortran_host_cloudst(tmp_path, params137, prec):
+    """CLOUDST through CLOUDST_*_C from Fortran equals the Python host bit for bit; FP64 equals the oracle."""
+    import oracle
+    from openifs_amd import cloudsc2 as C2
+
+    klon, klev, ktdia = 40, 137, 3
+    c2 = C2.synth_host(params137, 43, klon, klon, klev, prec)
+    dt = np.float64 if prec == "dp" else np.float32
+    arrs = {}
+    for n, kind, role in C2.CLOUDST_ARRAYS:
+        shape = (klon, klev + 1) if kind == "h" else (klon, klev)
+        arrs[n] = np.asfortranarray(c2[n][0].T.astype(dt)) if role == "i" else np.full(shape, np.nan, dt, order="F")
+    d = str(tmp_path)
+    _write_common(d, klon, klev, 2, klon - 1)
+    np.array([ktdia, 0, 0, 1], dtype=np.int32).tofile(os.path.join(d, "c2.bin"))
+    C2.reference_eta(klev).astype(np.float64).tofile(os.path.join(d, "ceta.bin"))
+    for n in arrs:
+        arrs[n].ravel(order="F").tofile(os.path.join(d, f"{n}.bin"))
+    subprocess.run([DRIVER, "cloudst", d, prec], check=True, timeout=120)
+    outs = [n for n, k, r in C2.CLOUDST_ARRAYS if r == "o"]
+    out = {n: np.fromfile(os.path.join(d, f"{n}.out"), dtype=dt).reshape(arrs[n].shape, order="F") for n in outs}
+    q2 = C2.default_params2(params137)
+    q2.levapls2 = 1
+    ceta = C2.reference_eta(klev)
+    ref = {k: v.copy(order="F") for k, v in arrs.items()}
+    C2.CLOUDST(params137, q2, ceta, 2, klon - 1, klon, ktdia, klev, False, 900.0, ref, prec)
+    for n in outs:
+        np.testing.assert_array_equal(out[n], ref[n], err_msg=n)
+    if prec == "dp":
+        orc = {k: v.copy(order="F") for k, v in arrs.items()}
+        C2.CLOUDST(params137, q2, ceta, 2, klon - 1, klon, ktdia, klev, False, 900.0, orc, "dp",
+                   func=oracle.lib().oracle_cloudst_dp)
+        for n in outs:
+            np.testing.assert_array_equal(np.isnan(out[n]), np.isnan(orc[n]), err_msg=n)
+            assert relrms(out[n], orc[n]) <= 1e-12, n
