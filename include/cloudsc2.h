@@ -114,6 +114,22 @@ int cloudst_blocked_sp_c(CLOUDST_BLOCKED_ARGS(float));
 int cloud_s_layer_blocked_dp_c(CLOUD_S_LAYER_BLOCKED_ARGS(double));
 int cloud_s_layer_blocked_sp_c(CLOUD_S_LAYER_BLOCKED_ARGS(float));
 
+/* CLOUD_S_LAYER on one NPROMA block of host arrays (Fortran order, (KLON,KLEV[+1]) and (KLON) for PPRECTYPE/PFZRA):
+ * the arrays of CLOUD_S_LAYER_BLOCKED_ARGS with the same optional ones (a NULL required array: CLOUDSC_ENULL).
+ * Outputs the layer does not write come back unchanged. */
+#define CLOUD_S_LAYER_ARGS(R)                                                                           \
+  const cloudsc_params_t *ydparams, const cloudsc2_params_t *ydparams2, const R *ceta, int kidia,        \
+      int kfdia, int klon, int ktdia, int klev, int ldrain1d, R tsphy, const R *prs1, const R *prsf1,   \
+      const R *pq, const R *pt, const R *pa, const R *pl, const R *pi, const R *plude, const R *plu,     \
+      const R *pmfu, const R *pmfd, const R *ptendency_t, const R *ptendency_q, const R *ptendency_l,   \
+      const R *ptendency_i, R *ptendency_loc_t, R *ptendency_loc_q, R *ptendency_loc_a,                 \
+      R *ptendency_loc_l, R *ptendency_loc_i, R *ptendency_loc_r, R *ptendency_loc_s, R *pqsat, R *pclc, \
+      R *pfplsl, R *pfplsn, R *pfhpsl, R *pfhpsn, R *pcovptot, R *pprectype, R *pfzra, R *pqiwp,       \
+      R *pqlwp
+
+int cloud_s_layer_dp_c(CLOUD_S_LAYER_ARGS(double));
+int cloud_s_layer_sp_c(CLOUD_S_LAYER_ARGS(float));
+
 #ifdef __cplusplus
 }
 #endif

@@ -286,3 +286,27 @@ def CLOUDST(params, params2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d, pt
         if rc != 0:
             raise RuntimeError(f"CLOUDST returned {rc}")
     return arrays
+
+
+def CLOUD_S_LAYER(params, params2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d, tsphy, arrays, precision="sp"):
+    """One NPROMA block of CLOUD_S_LAYER through cloud_s_layer_{dp,sp}_c: ``arrays`` maps LAYER_ARRAYS names to
+    Fortran-ordered numpy arrays ((KLON,KLEV[+1]), (KLON) for PPRECTYPE/PFZRA); optional ones may be absent."""
+    dt = _real(precision)
+    ceta = np.ascontiguousarray(ceta, dtype=dt)
+    args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.ctypes.data), ctypes.c_int(kidia),
+            ctypes.c_int(kfdia), ctypes.c_int(klon), ctypes.c_int(ktdia), ctypes.c_int(klev),
+            ctypes.c_int(int(bool(ldrain1d))), _ctype_real(precision)(tsphy)]
+    for name, kind, role, opt in LAYER_ARRAYS:
+        a = arrays.get(name)
+        if a is None:
+            if not opt:
+                raise KeyError(name)
+            args.append(ctypes.c_void_p(0))
+            continue
+        want = (klon,) if kind == "1d" else ((klon, klev + 1) if kind == "h" else (klon, klev))
+        if a.dtype != dt or not a.flags.f_contiguous or a.shape != want:
+            raise TypeError(f"{name}: need a Fortran-contiguous {np.dtype(dt).name} array of shape {want}")
+        args.append(ctypes.c_void_p(a.ctypes.data))
+    fn = f"cloud_s_layer_{precision}_c"
+    C._check(fn, getattr(C.lib(), fn)(*args))
+    return arrays

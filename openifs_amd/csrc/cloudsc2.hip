@@ -841,13 +841,14 @@ int run_on_device(const char* what, const HostArr* arrs, int narr, F launch_fn) 
   }
   char* base = (char*)w.dev;
   for (int i = 0; i < narr && e == hipSuccess; ++i)
-    e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * sizeof(R), hipMemcpyHostToDevice, w.stream);
+    if (arrs[i].host)
+      e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * sizeof(R), hipMemcpyHostToDevice, w.stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: copy to device", e);
-  auto D = [&](int i) { return (R*)(base + off[i]); };
+  auto D = [&](int i) { return arrs[i].host ? (R*)(base + off[i]) : (R*)nullptr; };  // NULL stays NULL
   e = launch_fn(D, w.stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, what, e);
   for (int i = 0; i < narr && e == hipSuccess; ++i)
-    if (arrs[i].out)
+    if (arrs[i].out && arrs[i].host)
       e = hipMemcpyAsync(arrs[i].out, base + off[i], arrs[i].n * sizeof(R), hipMemcpyDeviceToHost, w.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w.stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: copy to host", e);
@@ -942,6 +943,50 @@ int blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* cet
   return CLOUDSC_OK;
 }
 
+// CLOUD_S_LAYER on one block of host arrays (Fortran order), the arguments of CLOUD_S_LAYER_ARGS; NULL optional
+// arrays stay NULL on the device.
+template <typename R>
+int layer_per_block(const cloudsc_params_t* q, const cloudsc2_params_t* q2, const R* ceta, int kidia, int kfdia,
+                    int klon, int ktdia, int klev, int ldrain1d, R tsphy, const R* const* in, R* const* out) {
+  int rc = check(q, q2, ktdia, klev);
+  if (rc) return rc;
+  if (klon < 1 || kidia < 1 || kfdia < kidia || kfdia > klon)
+    return fail(CLOUDSC_EDIMS, "cloud_s_layer: need 1 <= KIDIA <= KFDIA <= KLON");
+  const bool c2 = q2->lepcld2 != 0;
+  for (int i : {0, 1, 2, 3, 5, 6, 7, 8, 11, 12})
+    if (!in[i]) return fail(CLOUDSC_ENULL, "cloud_s_layer: NULL array");
+  for (int i : {0, 1, 3, 4, 9, 10, 11, 12, 13})
+    if (!out[i]) return fail(CLOUDSC_ENULL, "cloud_s_layer: NULL array");
+  if (c2 && (!in[9] || !in[10] || !in[13] || !in[14]))
+    return fail(CLOUDSC_ENULL, "cloud_s_layer: LEPCLD2 needs PMFU, PMFD and the condensate tendencies");
+  if (out[2] && !in[4]) return fail(CLOUDSC_ENULL, "cloud_s_layer: PTENDENCY_LOC_A needs PA");
+  if (!ceta) return fail(CLOUDSC_ENULL, "cloud_s_layer: NULL CETA");
+  rc = cloudsc_capi::check_device();
+  if (rc) return rc;
+  const size_t n2 = (size_t)klon * klev, nh = (size_t)klon * (klev + 1), n1 = (size_t)klon;
+  // in: prs1 prsf1 pq pt pa pl pi plude plu pmfu pmfd ptendency_t/q/l/i (15);
+  // out: loc_t loc_q loc_a loc_l loc_i loc_r loc_s pqsat pclc pfplsl pfplsn pfhpsl pfhpsn pcovptot pprectype pfzra
+  //      pqiwp pqlwp (18)
+  HostArr arrs[1 + 15 + 18];
+  arrs[0] = {ceta, nullptr, (size_t)klev};
+  for (int i = 0; i < 15; ++i) arrs[1 + i] = {in[i], nullptr, i == 0 ? nh : n2};
+  for (int i = 0; i < 18; ++i) {
+    const size_t n = (i >= 9 && i <= 12) ? nh : (i == 14 || i == 15) ? n1 : n2;
+    arrs[16 + i] = {out[i], out[i], n};
+  }
+  return run_on_device<R>("cloud_s_layer", arrs, 34, [&](auto D, hipStream_t st) {
+    C2Ptrs<R> p{};
+    p.ceta = D(0); p.paphp1 = D(1); p.papp1 = D(2); p.pqm1 = D(3); p.ptm1 = D(4); p.pa = D(5); p.pl = D(6);
+    p.pi = D(7); p.plude = D(8); p.plu = D(9); p.pmfu = D(10); p.pmfd = D(11); p.pgtent = D(12);
+    p.pgtenq = D(13); p.pgtenl = D(14); p.pgteni = D(15);
+    p.ptent = D(16); p.ptenq = D(17); p.ptena = D(18); p.ptenl = D(19); p.pteni = D(20); p.ptenr = D(21);
+    p.ptens = D(22); p.pqsat = D(23); p.pclc = D(24); p.pfplsl = D(25); p.pfplsn = D(26); p.pfhpsl = D(27);
+    p.pfhpsn = D(28); p.pcovptot = D(29); p.pprectype = D(30); p.pfzra = D(31); p.pqiwp = D(32); p.pqlwp = D(33);
+    C2Dims d{kfdia, klon, klev, 1, ktdia, kidia, kfdia};
+    return launch<R>(*q, *q2, ldrain1d != 0, (double)tsphy, p, d, st, true, !c2);
+  });
+}
+
 // CLOUD_S_LAYER with LEPCLD2 (cloud_s_layer.F90:112-176) over all blocks: SATUR, CLOUDSC2 and the wrap-up in one
 // launch.  PTENDENCY_LOC_A/R/S, PQSAT, PCLC (AUXL%ZPATMP), PPRECTYPE and PFZRA may be NULL.
 template <typename R>
@@ -1014,6 +1059,15 @@ int cloudst_blocked(const cloudsc_params_t* q, const cloudsc2_params_t* q2, int 
   return cloudsc2_gpu::layer_blocked<R>(ydparams, ydparams2, ceta, ngptot, nproma, ktdia, klev, ngpblks, ldrain1d, \
                                         tsphy, p, stream)
 
+#define CLOUD_S_LAYER_PER_BLOCK(R)                                                                             \
+  const R* in[15] = {prs1, prsf1, pq, pt, pa, pl, pi, plude, plu, pmfu, pmfd, ptendency_t, ptendency_q,          \
+                     ptendency_l, ptendency_i};                                                                   \
+  R* out[18] = {ptendency_loc_t, ptendency_loc_q, ptendency_loc_a, ptendency_loc_l, ptendency_loc_i,             \
+                ptendency_loc_r, ptendency_loc_s, pqsat, pclc, pfplsl, pfplsn, pfhpsl, pfhpsn, pcovptot,         \
+                pprectype, pfzra, pqiwp, pqlwp};                                                                  \
+  return cloudsc2_gpu::layer_per_block<R>(ydparams, ydparams2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d,   \
+                                          tsphy, in, out)
+
 #define CLOUDST_PER_BLOCK(R)                                                                                   \
   return cloudsc2_gpu::per_block_st<R>(ydparams, ydparams2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d, ptsphy, \
                                        paphp1, papp1, pqm1, pqs, ptm1, plude, plu, ptent, pgtent, ptenq, pgtenq, pclc,  \
@@ -1048,6 +1102,8 @@ int cloudsc2_dp_c(CLOUDSC2_ARGS(double)) { CLOUDSC2_PER_BLOCK(double); }
 int cloudsc2_sp_c(CLOUDSC2_ARGS(float)) { CLOUDSC2_PER_BLOCK(float); }
 int cloudsc2_blocked_dp_c(CLOUDSC2_BLOCKED_ARGS(double)) { CLOUDSC2_BLOCKED(double); }
 int cloudsc2_blocked_sp_c(CLOUDSC2_BLOCKED_ARGS(float)) { CLOUDSC2_BLOCKED(float); }
+int cloud_s_layer_dp_c(CLOUD_S_LAYER_ARGS(double)) { CLOUD_S_LAYER_PER_BLOCK(double); }
+int cloud_s_layer_sp_c(CLOUD_S_LAYER_ARGS(float)) { CLOUD_S_LAYER_PER_BLOCK(float); }
 int cloudst_dp_c(CLOUDST_ARGS(double)) { CLOUDST_PER_BLOCK(double); }
 int cloudst_sp_c(CLOUDST_ARGS(float)) { CLOUDST_PER_BLOCK(float); }
 int cloudst_blocked_dp_c(CLOUDST_BLOCKED_ARGS(double)) { CLOUDST_BLOCKED(double); }

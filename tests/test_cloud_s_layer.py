@@ -409,3 +409,46 @@ def test_gpu_layer_full_size_properties(params137, lepcld2):
     r32 = run_layer("sp", p, q2, host, n_cols, nproma, klev)
     for n in written:
         assert relrms(out[n], r64[n]) <= 2.0 * relrms(r32[n], r64[n]) + 1e-6, n
+
+
+def test_layer_per_block_rejects_null_before_device_work(params137):
+    klon, klev = 8, 137
+    arrs = {}
+    for n, kind, role, opt in C2.LAYER_ARRAYS:
+        shape = (klon,) if kind == "1d" else ((klon, klev + 1) if kind == "h" else (klon, klev))
+        arrs[n] = np.zeros(shape, order="F")
+    q2 = params2(params137, 0, 0, 1)
+    del arrs["pmfu"]  # optional for CLOUDST, required by CLOUDSC2
+    from openifs_amd import cloudsc as C
+
+    with pytest.raises(C.CloudscError) as e:
+        C2.CLOUD_S_LAYER(params137, q2, C2.reference_eta(klev), 1, klon, klon, 1, klev, False, PTSPHY, arrs, "dp")
+    assert e.value.code == -3 and "LEPCLD2" in str(e.value)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,lepcld2", [("dp", 1), ("dp", 0), ("sp", 0)])
+def test_gpu_layer_per_block_equals_blocked(params137, prec, lepcld2):
+    """cloud_s_layer_*_c on one block (KIDIA:KFDIA window 3..45 of 48) equals the blocked entry's block bit for bit
+    inside the window and leaves the columns outside it untouched."""
+    p, klon, klev = params137, 48, 137
+    q2 = params2(p, 0, 1, lepcld2)
+    host = layer_case(p, 51, klon, klon, klev, prec)
+    print(f"cloud_s_layer per-block {prec} lepcld2={lepcld2}", flush=True)
+    ref = gpu_layer(p, q2, host, klon, klon, klev, prec)  # one full block through the blocked entry
+    dt = np.float64 if prec == "dp" else np.float32
+    arrs = {}
+    for n, kind, role, opt in C2.LAYER_ARRAYS:
+        v = host[n][0]
+        arrs[n] = np.asfortranarray((v if kind == "1d" else v.T).astype(dt))
+    C2.CLOUD_S_LAYER(p, q2, C2.reference_eta(klev), 3, 45, klon, 1, klev, False, PTSPHY, arrs, prec)
+    for n, kind, role, opt in C2.LAYER_ARRAYS:
+        if role != "o" or (lepcld2 and n in ("pqiwp", "pqlwp")):
+            continue
+        g = arrs[n].astype(np.float64)
+        r = ref[n][0] if kind == "1d" else ref[n][0].T
+        if n in ("ptendency_loc_r", "ptendency_loc_s"):
+            assert np.all(g == 0.0), n  # every column
+            continue
+        np.testing.assert_array_equal(g[2:45], r[2:45], err_msg=n)
+        assert np.all(np.isnan(g[:2])) and np.all(np.isnan(g[45:])), n
