@@ -452,3 +452,41 @@ def test_gpu_layer_per_block_equals_blocked(params137, prec, lepcld2):
             continue
         np.testing.assert_array_equal(g[2:45], r[2:45], err_msg=n)
         assert np.all(np.isnan(g[:2])) and np.all(np.isnan(g[45:])), n
+
+
+def _null_blocked_call(fn, q, q2, nargs, ngptot=64, nproma=32, klev=137, ngpblks=2, ptrs=None, ceta=None):
+    import ctypes
+    from openifs_amd import cloudsc as C
+
+    args = [ctypes.byref(q), ctypes.byref(q2), ceta or ctypes.c_void_p(0), ctypes.c_int(ngptot),
+            ctypes.c_int(nproma), ctypes.c_int(1), ctypes.c_int(klev), ctypes.c_int(ngpblks), ctypes.c_int(0),
+            ctypes.c_float(900.0)]
+    args += ptrs if ptrs is not None else [ctypes.c_void_p(0)] * nargs
+    args.append(ctypes.c_void_p(0))
+    return getattr(C.lib(), fn)(*args), C.lib().cloudsc_last_error()
+
+
+def test_blocked_entries_reject_on_the_host(params137):
+    """cloudst_blocked and cloud_s_layer_blocked check dimensions and required arrays before any device call; the
+    dummy host addresses below are never dereferenced because a check fails first."""
+    import ctypes
+
+    q2 = params2(params137, 0, 0, 0)
+    rc, msg = _null_blocked_call("cloudst_blocked_sp_c", params137, q2, len(C2.CLOUDST_ARRAYS), ngpblks=3)
+    assert rc == -1 and b"NGPBLKS" in msg  # CLOUDSC_EDIMS
+    rc, msg = _null_blocked_call("cloudst_blocked_sp_c", params137, q2, len(C2.CLOUDST_ARRAYS), klev=1)
+    assert rc == -1 and b"KLEV" in msg
+    rc, msg = _null_blocked_call("cloudst_blocked_sp_c", params137, q2, len(C2.CLOUDST_ARRAYS))
+    assert rc == -3 and b"NULL" in msg  # CLOUDSC_ENULL
+    buf = np.zeros(1, np.float32)
+    dummy = ctypes.c_void_p(buf.ctypes.data)
+    names = [n for n, *_ in C2.LAYER_ARRAYS]
+    ptrs = [ctypes.c_void_p(0) if opt else dummy for n, k, r, opt in C2.LAYER_ARRAYS]
+    # LEPCLD2 without PMFU/PMFD and the condensate tendencies
+    rc, msg = _null_blocked_call("cloud_s_layer_blocked_sp_c", params137, params2(params137, 0, 0, 1), 0,
+                                 ptrs=ptrs, ceta=dummy)
+    assert rc == -3 and b"LEPCLD2" in msg
+    # PTENDENCY_LOC_A without PA
+    ptrs[names.index("ptendency_loc_a")] = dummy
+    rc, msg = _null_blocked_call("cloud_s_layer_blocked_sp_c", params137, q2, 0, ptrs=ptrs, ceta=dummy)
+    assert rc == -3 and b"needs PA" in msg
