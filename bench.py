@@ -429,7 +429,9 @@ def layer_bench(params, nproma, steps, lepcld2=1):
     gbs = bpc * NGPTOT / (ms / 1e3) / 1e9
     return {"kernel_ms": ms, "columns_per_s": NGPTOT / (ms / 1e3), "algorithmic_bytes_per_column": bpc,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS, "traffic": None},
+                         "frac": gbs / HBM_PEAK_GBS,
+                         "traffic": read_traffic(nproma, f"cloud_s_layer_{'cloudsc2' if lepcld2 else 'cloudst'}"
+                                                         "_sp_l137_655360_traffic.md")},
             "workload": f"CLOUD_S_LAYER FP32 (SATUR, {'CLOUDSC2' if lepcld2 else 'CLOUDST'}, wrap-up), NLEV={KLEV}, "
                         f"{NGPTOT} columns, NPROMA={nproma}"}
 
