@@ -63,6 +63,7 @@ extern "C" {
 #define CLOUDSC_ENODEVICE (-5)    /* no gfx950 device visible */
 #define CLOUDSC_ENOMEM (-6)       /* device allocation failed */
 #define CLOUDSC_ESTATE (-7)       /* a field is not present where it is asked for (include/cloudsc_field.h) */
+#define CLOUDSC_ENONFINITE (-8)   /* the per-block call wrote non-finite outputs (count in cloudsc_last_error()) */
 
 /*
  * Flattened configuration: every TECLDP / TTHF / TCST / TECUMF / TERAD /
@@ -213,6 +214,12 @@ int cloudsc_blocked_diag_mixed_c(CLOUDSC_BLOCKED_DIAG_ARGS(float));
 
 /* Last HIP error text of the calling thread (empty string if none). */
 const char *cloudsc_last_error(void);
+
+/* Replaces the reference's negative-condensate warnings WRITE(NULOUT,*) 'CLOUDSC: WARNING Negative cloud
+ * liquid!' / 'ice!' (cloudsc.F90:3794, :3802): the number of points of the calling thread's last per-block
+ * CLOUDSC call (cloudsc_{dp,sp,mixed}_c) where the condensate fell below -RLMIN before the clean-up.  The
+ * reference prints one line per point; a host that wants the lines prints them from these counts. */
+int cloudsc_last_warnings(int64_t *n_negative_liquid, int64_t *n_negative_ice);
 
 #ifdef __cplusplus
 }

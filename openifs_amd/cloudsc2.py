@@ -85,6 +85,33 @@ def CLOUDSC2(params, params2, ceta, kidia, kfdia, klon, ktdia, klev, ldrain1d, p
     return arrays
 
 
+def check_device_fields(fields, spec, nproma, klev, ngpblks, precision, ceta=None, shape=None):
+    """Validates device tensors before a blocked launch: every array of ``spec`` ((name, kind, ...) tuples) that is
+    present must be a contiguous CUDA tensor of the precision's dtype and of its blocked shape, and ``ceta`` a
+    contiguous CUDA tensor of KLEV values of that dtype.  A wrong tensor would make the kernel index past its
+    allocation, so this raises TypeError before any launch."""
+    import torch
+
+    if precision not in PRECISIONS:
+        raise ValueError(precision)
+    dt = torch.float64 if precision == "dp" else torch.float32
+    shape = shape or blocked_shape
+
+    def need(name, t, want):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != dt or not t.is_contiguous() \
+                or tuple(t.shape) != tuple(want):
+            got = (f"{tuple(t.shape)} {t.dtype} {'cuda' if t.is_cuda else 'cpu'} "
+                   f"{'contiguous' if t.is_contiguous() else 'strided'}") if isinstance(t, torch.Tensor) else type(t)
+            raise TypeError(f"{name}: need a contiguous CUDA {dt} tensor of shape {tuple(want)}, got {got}")
+
+    if ceta is not None:
+        need("ceta", ceta, (klev,))
+    for entry in spec:
+        name, kind = entry[0], entry[1]
+        if name in fields:
+            need(name, fields[name], shape(kind, nproma, klev, ngpblks))
+
+
 def cloudsc2_blocked(fields, params, params2, ceta, ngptot, nproma, klev, ptsphy, precision="sp", ktdia=1,
                      ldrain1d=False, stream=None):
     """One launch of cloudsc2_blocked_{dp,sp}_c: ``fields`` maps the argument names to device tensors in blocked
@@ -92,6 +119,10 @@ def cloudsc2_blocked(fields, params, params2, ceta, ngptot, nproma, klev, ptsphy
     import torch
 
     ngpblks = (ngptot + nproma - 1) // nproma
+    missing = [n for n, _, _ in CLOUDSC2_ARRAYS if n not in fields]
+    if missing:
+        raise KeyError(f"cloudsc2_blocked: missing {missing}")
+    check_device_fields(fields, CLOUDSC2_ARRAYS, nproma, klev, ngpblks, precision, ceta)
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.data_ptr()), ctypes.c_int(ngptot),
@@ -201,6 +232,10 @@ def cloudst_blocked(fields, params, params2, ceta, ngptot, nproma, klev, ptsphy,
     import torch
 
     ngpblks = (ngptot + nproma - 1) // nproma
+    missing = [n for n, _, _ in CLOUDST_ARRAYS if n not in fields]
+    if missing:
+        raise KeyError(f"cloudst_blocked: missing {missing}")
+    check_device_fields(fields, CLOUDST_ARRAYS, nproma, klev, ngpblks, precision, ceta)
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.data_ptr()), ctypes.c_int(ngptot),
@@ -240,6 +275,7 @@ def cloud_s_layer_blocked(fields, params, params2, ceta, ngptot, nproma, klev, t
     import torch
 
     ngpblks = (ngptot + nproma - 1) // nproma
+    check_device_fields(fields, LAYER_ARRAYS, nproma, klev, ngpblks, precision, ceta, shape=layer_shape)
     if stream is None:
         stream = torch.cuda.current_stream().cuda_stream
     args = [ctypes.byref(params), ctypes.byref(params2), ctypes.c_void_p(ceta.data_ptr()), ctypes.c_int(ngptot),

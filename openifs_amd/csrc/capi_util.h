@@ -16,4 +16,18 @@ int check_device();
 int check_config(const cloudsc_params_t* p, bool diag = false);
 // CLOUDSC_ENULL if LRAD_CLOUD_INHOMOG is on and PGAW or PFSD is NULL, or LCLOUD_INHOMOG is on and PGAW is NULL.
 int check_fsd(const cloudsc_params_t* p, const void* pgaw, const void* pfsd);
+
+// Device buffer and stream of the per-block host entries.  Every such entry copies its host arrays in, launches,
+// copies back and waits on this stream before it returns, so the entries of one thread share the calling
+// thread's workspace, and threads never share one (reentrancy under the OpenMP block loop).
+struct HostWorkspace {
+  void* dev = nullptr;
+  size_t bytes = 0;
+  hipStream_t stream = nullptr;
+  int device = -1;
+};
+// The calling thread's workspace on the current device, grown to at least `bytes`.  When the thread has moved to
+// another device, the stream and buffer held on the previous one are released under that device first; the new
+// device is recorded only once its stream exists.  `who` prefixes the error text.
+int thread_workspace(size_t bytes, const char* who, HostWorkspace** out);
 }  // namespace cloudsc_capi

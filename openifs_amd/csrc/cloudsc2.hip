@@ -795,15 +795,6 @@ hipError_t launch(const cloudsc_params_t& q, const cloudsc2_params_t& q2, bool l
   return hipSuccess;
 }
 
-// Per-block drop-in: a per-thread device workspace and stream, as the other per-block entries.
-struct Workspace {
-  int device = -1;
-  hipStream_t stream = nullptr;
-  void* dev = nullptr;
-  size_t bytes = 0;
-};
-thread_local Workspace g_ws;
-
 // One host array of a per-block call: copied to the workspace, and back to `out` when that is set.
 struct HostArr {
   const void* host;
@@ -821,24 +812,12 @@ int run_on_device(const char* what, const HostArr* arrs, int narr, F launch_fn) 
     off[i] = total;
     total += (arrs[i].n * sizeof(R) + 255) & ~(size_t)255;
   }
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: hipGetDevice", e);
-  Workspace& w = g_ws;
-  if (w.device != dev) {
-    w = Workspace{};
-    w.device = dev;
-    e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc2: hipStreamCreate", e);
-  }
-  if (w.bytes < total) {
-    if (w.dev) (void)hipFree(w.dev);
-    w.dev = nullptr;
-    w.bytes = 0;
-    e = hipMalloc(&w.dev, total);
-    if (e != hipSuccess) return fail(CLOUDSC_ENOMEM, "cloudsc2: hipMalloc workspace", e);
-    w.bytes = total;
-  }
+  // the calling thread's workspace (shared with the other per-block entries)
+  cloudsc_capi::HostWorkspace* wp = nullptr;
+  const int rc = cloudsc_capi::thread_workspace(total, "cloudsc2", &wp);
+  if (rc) return rc;
+  cloudsc_capi::HostWorkspace& w = *wp;
+  hipError_t e = hipSuccess;
   char* base = (char*)w.dev;
   for (int i = 0; i < narr && e == hipSuccess; ++i)
     if (arrs[i].host)

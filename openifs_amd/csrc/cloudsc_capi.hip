@@ -6,6 +6,7 @@
 //       positional drop-in for SUBROUTINE CLOUDSC (cloudsc.F90:14-44).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -18,17 +19,13 @@
 #include "capi_util.h"
 #include "cloudsc_kernel.h"
 
-namespace cloudsc_gpu {
-int tuned_block(int variant, int klev);
-void set_tuned_block(int variant, int klev, int block);
-}
-
 using cloudsc_gpu::KDims;
 using cloudsc_gpu::KPtrs;
 
 namespace cloudsc_capi {
 
 thread_local std::string g_last_error;
+thread_local int64_t g_last_warn[2] = {0, 0};
 
 int fail(int code, const char* what, hipError_t e) {
   g_last_error = what;
@@ -151,37 +148,48 @@ int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngp
   return CLOUDSC_OK;
 }
 
-// ---- per-block host entry: a per-thread device workspace and stream ----
-struct Workspace {
-  void* dev = nullptr;
-  size_t bytes = 0;
-  hipStream_t stream = nullptr;
-  int device = -1;
-};
-thread_local Workspace g_ws;
+}  // namespace
 
-int workspace(size_t bytes, Workspace** out) {
+namespace cloudsc_capi {
+// ---- per-block host entries: a per-thread device workspace and stream ----
+thread_local HostWorkspace g_ws;
+
+int thread_workspace(size_t bytes, const char* who, HostWorkspace** out) {
+  std::string msg(who);
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: hipGetDevice", e);
-  Workspace& w = g_ws;
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, (msg + ": hipGetDevice").c_str(), e);
+  HostWorkspace& w = g_ws;
   if (w.device != dev) {  // first call on this thread, or the thread switched devices
-    w = Workspace{};
+    if (w.device >= 0 && hipSetDevice(w.device) == hipSuccess) {
+      if (w.stream) (void)hipStreamDestroy(w.stream);
+      if (w.dev) (void)hipFree(w.dev);
+    }
+    w = HostWorkspace{};
+    e = hipSetDevice(dev);
+    if (e != hipSuccess) return fail(CLOUDSC_EHIP, (msg + ": hipSetDevice").c_str(), e);
+    hipStream_t st = nullptr;
+    e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(CLOUDSC_EHIP, (msg + ": hipStreamCreate").c_str(), e);
+    w.stream = st;
     w.device = dev;
-    e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: hipStreamCreate", e);
   }
   if (w.bytes < bytes) {
     if (w.dev) (void)hipFree(w.dev);
     w.dev = nullptr;
     w.bytes = 0;
     e = hipMalloc(&w.dev, bytes);
-    if (e != hipSuccess) return fail(CLOUDSC_ENOMEM, "cloudsc: hipMalloc workspace", e);
+    if (e != hipSuccess) return fail(CLOUDSC_ENOMEM, (msg + ": hipMalloc workspace").c_str(), e);
     w.bytes = bytes;
   }
   *out = &w;
   return CLOUDSC_OK;
 }
+}  // namespace cloudsc_capi
+
+namespace {
+using cloudsc_capi::HostWorkspace;
+using cloudsc_capi::thread_workspace;
 
 template <typename R, typename RD>
 int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int klev, R ptsphy, const R* pt,
@@ -264,11 +272,13 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
     off[i] = total;
     total += (arrs[i].n * arrs[i].esz + 255) & ~(size_t)255;
   }
-  Workspace* w = nullptr;
-  rc = workspace(total, &w);
+  const size_t warn_off = total;  // two counters of the negative-condensate warnings
+  HostWorkspace* w = nullptr;
+  rc = thread_workspace(total + 256, "cloudsc", &w);
   if (rc) return rc;
   char* base = (char*)w->dev;
-  hipError_t e = hipSuccess;
+  uint32_t* d_warn = (uint32_t*)(base + warn_off);
+  hipError_t e = hipMemsetAsync(d_warn, 0, 2 * sizeof(uint32_t), w->stream);
   for (int i = 0; i < narr && e == hipSuccess; ++i)
     if (arrs[i].n)
       e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * arrs[i].esz, hipMemcpyHostToDevice, w->stream);
@@ -285,6 +295,7 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
   p.pfplsn = D(39); p.pfhpsl = D(40); p.pfhpsn = D(41);
   // one block: KFDIA bounds the active columns, KIDIA-1 shifts the pressure section of the supersaturation check
   KDims d{kfdia, klon, klev, 1, kidia, kfdia};
+  d.nwarn = d_warn;
   cloudsc_gpu::KBudPtrs<R> bp{};
   if (budgets) {
     bp.pvfa = D(42); bp.pdyna = D(43); bp.pdynl = D(44); bp.pdyni = D(45); bp.pdyns = D(46); bp.phrsw = D(47);
@@ -305,8 +316,30 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
   for (int i = 0; i < narr && e == hipSuccess; ++i)
     if (arrs[i].host_out && arrs[i].n)
       e = hipMemcpyAsync(arrs[i].host_out, base + off[i], arrs[i].n * arrs[i].esz, hipMemcpyDeviceToHost, w->stream);
+  uint32_t warn[2] = {0, 0};
+  if (e == hipSuccess) e = hipMemcpyAsync(warn, d_warn, sizeof(warn), hipMemcpyDeviceToHost, w->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: copy to host", e);
+  cloudsc_capi::g_last_warn[0] = warn[0];
+  cloudsc_capi::g_last_warn[1] = warn[1];
+  // non-finite outputs in the reference's write set: columns KIDIA:KFDIA of the tendencies (species 1-4 of
+  // PTENDENCY_LOC_CLD), PCOVPTOT from NCLDTOP and the 14 flux profiles
+  int64_t nbad = 0;
+  auto scan = [&](const R* a, size_t ld, int k0, int k1) {
+    for (int k = k0; k < k1; ++k)
+      for (int jl = kidia - 1; jl < kfdia; ++jl) nbad += !std::isfinite(a[(size_t)k * ld + jl]);
+  };
+  const R* tend[] = {tloc_t, tloc_q, tloc_a};
+  for (const R* a : tend) scan(a, klon, 0, klev);
+  for (int m = 0; m < 4; ++m) scan(tloc_cld + (size_t)m * n2, klon, 0, klev);
+  scan(pcovptot, klon, q->ncldtop - 1, klev);
+  const R* flux[] = {pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng,
+                     pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn};
+  for (const R* a : flux) scan(a, klon, 0, klev + 1);
+  if (nbad) {
+    const std::string msg = "cloudsc: " + std::to_string(nbad) + " non-finite outputs";
+    return fail(CLOUDSC_ENONFINITE, msg.c_str());
+  }
   return CLOUDSC_OK;
 }
 
@@ -402,13 +435,59 @@ int cloudsc_blocked_mixed_c(CLOUDSC_BLOCKED_ARGS(float)) { BLOCKED_CALL(float, d
 
 const char* cloudsc_last_error(void) { return cloudsc_capi::g_last_error.c_str(); }
 
+int cloudsc_last_warnings(int64_t* n_negative_liquid, int64_t* n_negative_ice) {
+  if (!n_negative_liquid || !n_negative_ice)
+    return cloudsc_capi::fail(CLOUDSC_ENULL, "cloudsc_last_warnings: NULL output pointer");
+  *n_negative_liquid = cloudsc_capi::g_last_warn[0];
+  *n_negative_ice = cloudsc_capi::g_last_warn[1];
+  return CLOUDSC_OK;
+}
+
 }  // extern "C"
 
 extern "C" int cloudsc_tune_set_block(int variant, int klev, int block) {
   if (variant < 0 || variant > 2 || klev < 2 || !(block == 0 || block == 64 || block == 128 || block == 256 || block == 512))
     return cloudsc_capi::fail(CLOUDSC_EDIMS, "cloudsc_tune_set_block: variant 0..2, KLEV >= 2, block 0/64/128/256/512");
-  cloudsc_gpu::set_tuned_block(variant, klev, block);
+  if (block == 0) {
+    cloudsc_gpu::set_tuned_launch(variant, klev, nullptr);
+  } else {
+    cloudsc_gpu::LaunchCfg l;
+    l.block = block;
+    cloudsc_gpu::set_tuned_launch(variant, klev, &l);
+  }
   return CLOUDSC_OK;
 }
 
-extern "C" int cloudsc_tune_get_block(int variant, int klev) { return cloudsc_gpu::tuned_block(variant, klev); }
+extern "C" int cloudsc_tune_get_block(int variant, int klev) {
+  return cloudsc_gpu::tuned_launch(variant, klev).block;
+}
+
+extern "C" int cloudsc_tune_set_launch(int variant, int klev, int block, int waves_per_simd, int klev_compiled) {
+  if (variant < 0 || variant > 2 || klev < 2)
+    return cloudsc_capi::fail(CLOUDSC_EDIMS, "cloudsc_tune_set_launch: variant 0..2, KLEV >= 2");
+  if (block == 0) {
+    cloudsc_gpu::set_tuned_launch(variant, klev, nullptr);
+    return CLOUDSC_OK;
+  }
+  cloudsc_gpu::LaunchCfg l;
+  l.block = block;
+  l.wpe = waves_per_simd;
+  l.kspec = klev_compiled;
+  if (!cloudsc_gpu::valid_launch(variant, l))
+    return cloudsc_capi::fail(CLOUDSC_EDIMS,
+                              "cloudsc_tune_set_launch: block 64/128/256/512; waves per SIMD 0, or 2 (dp), 4 (sp), "
+                              "3 or 4 (mixed); klev_compiled 0 or 1; waves per SIMD and klev_compiled only with "
+                              "block 128 or 256");
+  cloudsc_gpu::set_tuned_launch(variant, klev, &l);
+  return CLOUDSC_OK;
+}
+
+extern "C" int cloudsc_tune_get_launch(int variant, int klev, int* block, int* waves_per_simd, int* klev_compiled) {
+  if (variant < 0 || variant > 2 || !block || !waves_per_simd || !klev_compiled)
+    return cloudsc_capi::fail(CLOUDSC_ENULL, "cloudsc_tune_get_launch: variant 0..2 and three output pointers");
+  const cloudsc_gpu::LaunchCfg l = cloudsc_gpu::tuned_launch(variant, klev);
+  *block = l.block;
+  *waves_per_simd = l.wpe;
+  *klev_compiled = l.kspec;
+  return CLOUDSC_OK;
+}

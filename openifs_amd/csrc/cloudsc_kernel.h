@@ -125,6 +125,16 @@ template <typename T>
 KConstDefault<T> make_kconst_default(const cloudsc_params_t& q, double ptsphy);
 bool is_default_config(const cloudsc_params_t& q);
 
+// Launch configuration of the blocked kernel (tuned per arithmetic variant and KLEV).
+struct LaunchCfg {
+  int block = 256;  // columns per work-group: 64, 128, 256 or 512
+  int wpe = 0;      // minimum waves per SIMD of the register allocation (0: compiler's choice)
+  int kspec = 0;    // 1: use the instantiation with KLEV compiled in (137 and 91)
+};
+bool valid_launch(int variant, const LaunchCfg& l);
+LaunchCfg tuned_launch(int variant, int klev);
+void set_tuned_launch(int variant, int klev, const LaunchCfg* l);  // NULL removes the setting
+
 struct KDims {
   int32_t ngptot;   // columns in total (blocked) or KFDIA (one block)
   int32_t nproma;   // KLON
@@ -132,6 +142,9 @@ struct KDims {
   int32_t ngpblks;
   int32_t kidia;    // first active column of every block (1-based)
   int32_t kfdia;    // last active column of a full block (1-based)
+  // NULL, or two device counters the launch adds to: points where the reference writes its negative-condensate
+  // warnings (cloudsc.F90:3794 liquid, :3802 ice; condensate below -RLMIN before the clean-up)
+  uint32_t* nwarn = nullptr;
 };
 
 // Blocked launch over all blocks (cloudsc_kernel.hip): splits the grid so that every array of one launch spans

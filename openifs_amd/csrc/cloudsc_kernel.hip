@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <type_traits>
 #include <mutex>
 #include <utility>
 
@@ -23,6 +24,18 @@ __device__ __forceinline__ T trunc_scale(T x, int jo, int a, int b, T zz) {
   return hit ? x * zz : x;
 }
 
+// The reference's negative-condensate warnings (cloudsc.F90:3794 liquid, :3802 ice) as counts: one atomic add per
+// wave and counter, by the wave's first active lane, only when some lane of the wave warns.
+__device__ __forceinline__ void count_warnings(uint32_t* n, bool wl, bool wi) {
+  const uint64_t bl = __ballot(wl), bi = __ballot(wi);
+  if ((bl | bi) == 0) return;
+  const uint64_t act = __ballot(1);
+  if (__lane_id() == (unsigned)(__ffsll((long long)act) - 1)) {
+    if (bl) atomicAdd(n, (uint32_t)__popcll(bl));
+    if (bi) atomicAdd(n + 1, (uint32_t)__popcll(bi));
+  }
+}
+
 // KC / KCD: constants of the working precision and of the FP64 islands, either the run-time
 // structure KConst or the compile-time default configuration KConstDefault (same member names).
 // UNI: every wave lies in one block (NPROMA a multiple of 64), so block-dependent offsets are wave-uniform.
@@ -31,12 +44,17 @@ __device__ __forceinline__ T trunc_scale(T x, int jo, int a, int b, T zz) {
 // :1278-1279), stores them as TENDENCY_SATADJ (STATE_COPY, :1289), and CLOUDSC then reads the incremented CML
 // tendencies from registers.
 // BUD: the process budgets (LCLDBUD*) are recorded per level and written into PEXTRA (cloudsc_budget.h).
-template <typename R, typename RD, typename KC, typename KCD, int BLOCK, bool UNI, bool SAT, bool BUD>
-__global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p,
-                                                                const KDims d, int ssc_off, const KSatPtrs<R> s,
-                                                                const KBudPtrs<R> bp) {
+// WPE: minimum waves per SIMD the register allocation must allow (0: the compiler's choice); fewer registers per
+// lane for more waves in flight, at the price of spills when the live state does not fit.
+// KL: number of levels fixed at compile time (0: d.klev at run time), so that the level loop and every level
+// offset are constants of the instantiation.
+template <typename R, typename RD, typename KC, typename KCD, int BLOCK, bool UNI, bool SAT, bool BUD, int WPE = 0,
+          int KL = 0>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WPE > 0 ? WPE : 1))) void
+cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p, const KDims d, int ssc_off, const KSatPtrs<R> s,
+                      const KBudPtrs<R> bp) {
   const uint32_t g = blockIdx.x * BLOCK + threadIdx.x;
-  const uint32_t nproma = (uint32_t)d.nproma, klev = (uint32_t)d.klev;
+  const uint32_t nproma = (uint32_t)d.nproma, klev = KL > 0 ? (uint32_t)KL : (uint32_t)d.klev;
   if (g >= (uint32_t)d.ngpblks * nproma) return;
   uint32_t b = g / nproma;
   if (UNI) b = __builtin_amdgcn_readfirstlane(b);
@@ -279,13 +297,16 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
 
       // Explicit (ZSOLQA) entries: vX = ZSOLQA(V,X) = -ZSOLQA(X,V); il = ZSOLQA(I,L);
       // ri = ZSOLQA(R,I); rs = ZSOLQA(R,S); dX = ZSOLQA(X,X).  Implicit (ZSOLQB) entries qbXY.
-      R vl = R(0), vi = R(0), vr = R(0), vs = R(0), il = R(0), ri = R(0), rs = R(0);
-      R dl = R(0), di = R(0), dr = R(0), ds = R(0);
+      // The level's water budget -- ZSOLQA, the first guesses ZQXFG and the sinks limited by them -- is kept in
+      // RD: FP64 in the mixed variant, so that a species a limited sink removes completely leaves an FP64-sized
+      // residue that the ZEPSEC clip after the solve removes, as in the FP64 build (R in the dp and sp variants).
+      RD vl = RD(0), vi = RD(0), vr = RD(0), vs = RD(0), il = RD(0), ri = RD(0), rs = RD(0);
+      RD dl = RD(0), di = RD(0), dr = RD(0), ds = RD(0);
       R qbll = R(0), qbii = R(0), qbrl = R(0), qbsi = R(0), qbsl = R(0);
       R zsolab = R(0), zsolac = R(0);
       R csrc_l = R(0), csrc_i = R(0), csrc_r = R(0), csrc_s = R(0), csink = R(0);
       R fsrc_i = R(0), fsrc_r = R(0), fsrc_s = R(0);
-      R qxfg_l = ql, qxfg_i = qi, qxfg_r = qr, qxfg_s = qs;
+      RD qxfg_l = ql, qxfg_i = qi, qxfg_r = qr, qxfg_s = qs;
       R zqpretot = R(0);
 
       if (ql < rlmin) vl = ql;  // :1334-1342
@@ -399,15 +420,15 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
         di = di + fsrc_i;
         bud.set(I12, fsrc_i * zqtmst);  // (:1864-1866)
         qxfg_i = qxfg_i + fsrc_i;
-        zqpretot = zqpretot + qxfg_i;
+        zqpretot = R(zqpretot + qxfg_i);
         fsrc_r = pf_r * zdtgdp;
         dr = dr + fsrc_r;
         qxfg_r = qxfg_r + fsrc_r;
-        zqpretot = zqpretot + qxfg_r;
+        zqpretot = R(zqpretot + qxfg_r);
         fsrc_s = pf_s * zdtgdp;
         ds = ds + fsrc_s;
         qxfg_s = qxfg_s + fsrc_s;
-        zqpretot = zqpretot + qxfg_s;
+        zqpretot = R(zqpretot + qxfg_s);
       }
       const R fsink_i = zdtgdp * zrho * vfall_i;
       const R fsink_r = zdtgdp * zrho * vfall_r;
@@ -474,7 +495,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
                            m_div(c.rcl_const2r * m_sqrt(zrho * zfallcorr2), m_sqrt(zcorr2) * m_pow(zlambda, c.rcl_const3r)));
           const R zdenom = R(1) + zbeta * ptsphy * zcorqsliq;
           const R zdpevap = m_div(zcovpclr * zbeta * ptsphy * zsubsat, zdenom);
-          const R zevap = m_min(zdpevap, qxfg_r);
+          const RD zevap = m_min(RD(zdpevap), qxfg_r);
           vr = vr + zevap;
           qxfg_r = qxfg_r - zevap;
           bud.set(L20, -zevap * zqtmst);  // (:2363)
@@ -498,7 +519,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           const R zcvds = m_div(R(7.8) * m_pow(m_div(zicenuclei, zrho), R(0.666)) * zsupersatice, R(8.87) * (zadd + zbdd));
           const R zice0 = m_max(zicecld, m_div(zicenuclei * c.riceinit, zrho));
           const R zinew = m_pow(R(0.666) * zcvds * ptsphy + m_pow(zice0, R(0.666)), R(1.5));
-          R zdepos = m_max(c.rcl_overlapliqice * za * (zinew - zice0), R(0));
+          RD zdepos = m_max(c.rcl_overlapliqice * za * (zinew - zice0), R(0));
           zdepos = m_min(zdepos, qxfg_l);
           const R zinfactor = m_min(m_div(zicenuclei, R(15000.)), R(1));
           zdepos = zdepos * m_min(zinfactor + (R(1) - zinfactor) * (c.rdepliqrefrate + m_div(zcldtopdist, c.rdepliqrefdepth)), R(1));
@@ -530,8 +551,8 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       }
 
       // revised in-cloud condensate (:2750-2754)
-      zliqcld = qxfg_l * ztmpa;
-      zicecld = qxfg_i * ztmpa;
+      zliqcld = R(qxfg_l * ztmpa);
+      zicecld = R(qxfg_i * ztmpa);
 
       // 4.8 ice -> snow autoconversion (:2769-2800)
       if (ztp1 <= c.rtt && zicecld > zepsec) {
@@ -544,7 +565,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
 
       // 4.12 melting of ice and snow (:2815-2862)
       {
-        const R zicetot = qxfg_i + qxfg_s;
+        const R zicetot = R(qxfg_i + qxfg_s);
         R zmeltmax = R(0);
         if (zicetot > zepsec && ztp1 > c.rtt) {
           const R zsubsat = m_max(zqsice - qv, R(0));
@@ -553,15 +574,15 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
           zmeltmax = m_max(ztdmtw0 * zcons1 * c.zrldcp, R(0));
         }
         if (zmeltmax > zepsec && zicetot > zepsec) {
-          R zalfa = m_div(qxfg_i, zicetot);
-          R zmelt = m_min(qxfg_i, zalfa * zmeltmax);
+          R zalfa = R(m_div(qxfg_i, RD(zicetot)));
+          RD zmelt = m_min(qxfg_i, RD(zalfa * zmeltmax));
           qxfg_i = qxfg_i - zmelt;
           qxfg_r = qxfg_r + zmelt;
           ri = ri + zmelt;  // ZSOLQA(R,I) += melt
           bud.set(I15, -zmelt * zqtmst);  // (:2856-2858)
           bud.set(L17, zmelt * zqtmst);
-          zalfa = m_div(qxfg_s, zicetot);
-          zmelt = m_min(qxfg_s, zalfa * zmeltmax);
+          zalfa = R(m_div(qxfg_s, RD(zicetot)));
+          zmelt = m_min(qxfg_s, RD(zalfa * zmeltmax));
           qxfg_s = qxfg_s - zmelt;
           qxfg_r = qxfg_r + zmelt;
           rs = rs + zmelt;  // ZSOLQA(R,S) += melt
@@ -599,7 +620,7 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       {
         const R zfrzmax = m_max((c.rthomo - ztp1) * c.zrldcp, R(0));
         if (zfrzmax > zepsec && qxfg_l > zepsec) {
-          const R zfrz = m_min(qxfg_l, zfrzmax);
+          const RD zfrz = m_min(qxfg_l, RD(zfrzmax));
           il = il + zfrz;
           bud.set(L19, -zfrz * zqtmst);  // (:2958)
         }
@@ -650,16 +671,16 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
         if (zcovpclr > zepsec && qxfg_s > zepsec && zqe < zzrh * zqsice) {
           const R dd = zcovptot * zdtgdp;
           const R den = m_max(m_abs(dd), c.zepsilon);
-          const R zpreclr = m_div(qxfg_s * zcovpclr, dd < R(0) ? -den : den);
+          const R zpreclr = R(m_div(qxfg_s * RD(zcovpclr), RD(dd < R(0) ? -den : den)));
           const R zbeta1 = m_div(m_div(m_sqrt(m_div(pap, paph_surf)), c.rvrfactor) * zpreclr, m_max(zcovpclr, zepsec));
           const R zbeta = c.rg * c.rpecons * m_pow(zbeta1, R(0.5777));
           const R zdenom = R(1) + zbeta * ptsphy * zcorqsice;
           const R zdpr = m_div(zcovpclr * zbeta * (zqsice - zqe), zdenom) * zdp * c.zrg_r;
           const R zdpevap = zdpr * zdtgdp;
-          const R zevap = m_min(zdpevap, qxfg_s);
+          const RD zevap = m_min(RD(zdpevap), qxfg_s);
           vs = vs + zevap;
           bud.set(I17, -zevap * zqtmst);  // (:3304)
-          zcovptot = m_max(c.rcovpmin, zcovptot - m_max(R(0), m_div((zcovptot - za) * zevap, qxfg_s)));
+          zcovptot = m_max(c.rcovpmin, zcovptot - m_max(R(0), R(m_div(RD(zcovptot - za) * zevap, qxfg_s))));
           qxfg_s = qxfg_s - zevap;
         }
       }
@@ -679,26 +700,26 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       // 5.2.1 truncation of explicit sinks (:3455-3536).  Row sums of ZSOLQA
       // (minus sign: sinks positive), in the reference's column order L,I,R,S,V.
       {
-        const R qxs[5] = {ql, qi, qr, qs, qv};
-        R rsum[5];
-        rsum[0] = ((R(0) - dl) + il) + vl;        // row L: (L,L)=dl (L,I)=-il (L,V)=-vl
-        rsum[1] = (((R(0) - il) - di) + ri) + vi; // row I: il, di, (I,R)=-ri, (I,V)=-vi
-        rsum[2] = (((R(0) - ri) - dr) - rs) + vr; // row R: ri, dr, rs, (R,V)=-vr
-        rsum[3] = ((R(0) + rs) - ds) + vs;        // row S: (S,R)=-rs, ds, (S,V)=-vs
-        rsum[4] = (((R(0) - vl) - vi) - vr) - vs; // row V: vl, vi, vr, vs
+        const RD qxs[5] = {ql, qi, qr, qs, qv};
+        RD rsum[5];
+        rsum[0] = ((RD(0) - dl) + il) + vl;        // row L: (L,L)=dl (L,I)=-il (L,V)=-vl
+        rsum[1] = (((RD(0) - il) - di) + ri) + vi; // row I: il, di, (I,R)=-ri, (I,V)=-vi
+        rsum[2] = (((RD(0) - ri) - dr) - rs) + vr; // row R: ri, dr, rs, (R,V)=-vr
+        rsum[3] = ((RD(0) + rs) - ds) + vs;        // row S: (S,R)=-rs, ds, (S,V)=-vs
+        rsum[4] = (((RD(0) - vl) - vi) - vr) - vs; // row V: vl, vi, vr, vs
         // ZRATIO = ZMAX/MAX(sinksum,ZMAX) is exactly 1 unless the sinks overshoot; with no
         // overshoot anywhere every scale factor below is 1 and the step is a no-op.
-        R zmax[5];
+        RD zmax[5];
         bool any_trunc = false;
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
-          zmax[m] = m_max(qxs[m], zepsec);
+          zmax[m] = m_max(qxs[m], RD(zepsec));
           any_trunc = any_trunc || (rsum[m] > zmax[m]);
         }
         if (any_trunc) {
-          R zratio[5];
+          RD zratio[5];
 #pragma unroll
-          for (int m = 0; m < 5; ++m) zratio[m] = rsum[m] > zmax[m] ? m_div(zmax[m], rsum[m]) : R(1);
+          for (int m = 0; m < 5; ++m) zratio[m] = rsum[m] > zmax[m] ? m_div(zmax[m], rsum[m]) : RD(1);
           // stable ascending order of zratio (insertion sort of :3479-3495)
           int rank[5];
 #pragma unroll
@@ -713,24 +734,24 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
             int jo = 0;
 #pragma unroll
             for (int m = 0; m < 5; ++m) jo = (rank[m] == s) ? m : jo;
-            R ssum;
+            RD ssum;
             if (jo == 0) {
-              ssum = ((R(0) - dl) + il) + vl;
+              ssum = ((RD(0) - dl) + il) + vl;
             } else if (jo == 1) {
-              ssum = (((R(0) - il) - di) + ri) + vi;
+              ssum = (((RD(0) - il) - di) + ri) + vi;
             } else if (jo == 2) {
-              ssum = (((R(0) - ri) - dr) - rs) + vr;
+              ssum = (((RD(0) - ri) - dr) - rs) + vr;
             } else if (jo == 3) {
-              ssum = ((R(0) + rs) - ds) + vs;
+              ssum = ((RD(0) + rs) - ds) + vs;
             } else {
-              ssum = (((R(0) - vl) - vi) - vr) - vs;
+              ssum = (((RD(0) - vl) - vi) - vr) - vs;
             }
-            R zmm = zmax[0];
+            RD zmm = zmax[0];
 #pragma unroll
             for (int m = 1; m < 5; ++m) zmm = (jo == m) ? zmax[m] : zmm;
             // ZZRATIO = ZMM/MAX(ZSINKSUM,ZMM) < 1 only on overshoot; otherwise the scaling is by 1
             if (ssum > zmm) {
-              const R zz = m_div(zmm, ssum);
+              const RD zz = m_div(zmm, ssum);
               // off-diagonal pairs (a,b) with stored x = ZSOLQA(a,b)
               vl = trunc_scale(vl, jo, 4, 0, zz);
               vi = trunc_scale(vi, jo, 4, 1, zz);
@@ -740,10 +761,10 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
               ri = trunc_scale(ri, jo, 2, 1, zz);
               rs = trunc_scale(rs, jo, 2, 3, zz);
               // a negative diagonal entry is scaled twice (:3531-3532)
-              if (jo == 0 && dl < R(0)) dl = (dl * zz) * zz;
-              if (jo == 1 && di < R(0)) di = (di * zz) * zz;
-              if (jo == 2 && dr < R(0)) dr = (dr * zz) * zz;
-              if (jo == 3 && ds < R(0)) ds = (ds * zz) * zz;
+              if (jo == 0 && dl < RD(0)) dl = (dl * zz) * zz;
+              if (jo == 1 && di < RD(0)) di = (di * zz) * zz;
+              if (jo == 2 && dr < RD(0)) dr = (dr * zz) * zz;
+              if (jo == 3 && ds < RD(0)) ds = (ds * zz) * zz;
             }
           }
         }
@@ -835,6 +856,8 @@ __global__ __launch_bounds__(BLOCK) void cloudsc_column_kernel(const KC c, const
       // negative clean-up (:3793-3807)
       {
         const R zl = ql0 + tcld_l * ptsphy;
+        const R zi0 = qi0 + tcld_i * ptsphy;
+        if (d.nwarn) count_warnings(d.nwarn, zl < -rlmin, zi0 < -rlmin);
         if (zl < R(0)) {
           tcld_l = tcld_l - zl * zqtmst;
           tq = tq + zl * zqtmst;
@@ -1047,10 +1070,12 @@ KPtrs<R> offset_ptrs(const KPtrs<R>& p, int64_t b0, int64_t nproma, int64_t klev
   return q;
 }
 
-// Work-group size per (arithmetic variant, KLEV), set by the autotuner (cloudsc_tune_set_block); 256 otherwise.
+// Launch configuration per (arithmetic variant, KLEV), set by the autotuner (cloudsc_tune_set_launch): work-group
+// size, minimum waves per SIMD of the register allocation, and whether the level count is compiled in.  Default:
+// 256 columns per group, the compiler's register allocation, run-time KLEV.
 namespace {
 std::mutex g_tune_mutex;
-std::map<std::pair<int, int>, int> g_tuned_block;
+std::map<std::pair<int, int>, LaunchCfg> g_tuned;
 }  // namespace
 
 template <typename R, typename RD>
@@ -1058,52 +1083,95 @@ constexpr int variant_id() {
   return sizeof(R) == 8 ? 0 : (sizeof(RD) == 8 ? 2 : 1);  // 0 dp, 1 sp, 2 mixed
 }
 
-int tuned_block(int variant, int klev) {
-  std::lock_guard<std::mutex> lk(g_tune_mutex);
-  const auto it = g_tuned_block.find({variant, klev});
-  return it == g_tuned_block.end() ? 256 : it->second;
+// Waves-per-SIMD bounds instantiated for each variant besides 0: dp holds its state in 256 registers (1 wave),
+// sp in 137 (3 waves), mixed in 177 (2 waves); each gets the next occupancy steps its state fits with few spills.
+bool valid_wpe(int variant, int wpe) {
+  if (wpe == 0) return true;
+  switch (variant) {
+    case 0: return wpe == 2;
+    case 1: return wpe == 4;
+    case 2: return wpe == 3 || wpe == 4;
+    default: return false;
+  }
 }
 
-void set_tuned_block(int variant, int klev, int block) {
+bool valid_launch(int variant, const LaunchCfg& l) {
+  const bool blk = l.block == 64 || l.block == 128 || l.block == 256 || l.block == 512;
+  if (!blk || !valid_wpe(variant, l.wpe) || (l.kspec != 0 && l.kspec != 1)) return false;
+  // register bounds and compiled-in level counts are instantiated for 128- and 256-column groups only
+  return (l.wpe == 0 && l.kspec == 0) || l.block == 128 || l.block == 256;
+}
+
+LaunchCfg tuned_launch(int variant, int klev) {
   std::lock_guard<std::mutex> lk(g_tune_mutex);
-  if (block <= 0)
-    g_tuned_block.erase({variant, klev});
+  const auto it = g_tuned.find({variant, klev});
+  return it == g_tuned.end() ? LaunchCfg{} : it->second;
+}
+
+void set_tuned_launch(int variant, int klev, const LaunchCfg* l) {
+  std::lock_guard<std::mutex> lk(g_tune_mutex);
+  if (!l)
+    g_tuned.erase({variant, klev});
   else
-    g_tuned_block[{variant, klev}] = block;
+    g_tuned[{variant, klev}] = *l;
+}
+
+template <int BLOCK, int WPE, int KL, bool SAT, typename R, typename RD, typename KC, typename KCD>
+hipError_t launch_one(unsigned ncols, const KC& c, const KCD& cd, const KPtrs<R>& p, const KDims& d, int ssc_off,
+                      const KSatPtrs<R>& s, hipStream_t stream) {
+  const KBudPtrs<R> nb{};
+  hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, BLOCK, true, SAT, false, WPE, KL>),
+                     dim3((ncols + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, stream, c, cd, p, d, ssc_off, s, nb);
+  return hipGetLastError();
+}
+
+// Level count of the tuned instantiation: the compiled-in 137 or 91 when asked for and matching, else run time.
+template <int BLOCK, int WPE, bool SAT, typename R, typename RD, typename KC, typename KCD>
+hipError_t launch_kl(int kspec, unsigned ncols, const KC& c, const KCD& cd, const KPtrs<R>& p, const KDims& d,
+                     int ssc_off, const KSatPtrs<R>& s, hipStream_t stream) {
+  if (kspec && d.klev == 137) return launch_one<BLOCK, WPE, 137, SAT, R, RD>(ncols, c, cd, p, d, ssc_off, s, stream);
+  if (kspec && d.klev == 91) return launch_one<BLOCK, WPE, 91, SAT, R, RD>(ncols, c, cd, p, d, ssc_off, s, stream);
+  return launch_one<BLOCK, WPE, 0, SAT, R, RD>(ncols, c, cd, p, d, ssc_off, s, stream);
+}
+
+template <int BLOCK, bool SAT, typename R, typename RD, typename KC, typename KCD>
+hipError_t launch_wpe(const LaunchCfg& l, unsigned ncols, const KC& c, const KCD& cd, const KPtrs<R>& p,
+                      const KDims& d, int ssc_off, const KSatPtrs<R>& s, hipStream_t stream) {
+  constexpr int v = variant_id<R, RD>();
+  constexpr int w1 = v == 0 ? 2 : (v == 1 ? 4 : 3);
+  constexpr int w2 = 4;  // mixed only
+  if (l.wpe == w1) return launch_kl<BLOCK, w1, SAT, R, RD>(l.kspec, ncols, c, cd, p, d, ssc_off, s, stream);
+  if (v == 2 && l.wpe == w2) return launch_kl<BLOCK, w2, SAT, R, RD>(l.kspec, ncols, c, cd, p, d, ssc_off, s, stream);
+  return launch_kl<BLOCK, 0, SAT, R, RD>(l.kspec, ncols, c, cd, p, d, ssc_off, s, stream);
 }
 
 template <bool SAT, typename R, typename RD, typename KC, typename KCD>
-hipError_t launch_variant(int block, bool uni, unsigned ncols, const KC& c, const KCD& cd, const KPtrs<R>& p,
-                          const KDims& d, int ssc_off, const KSatPtrs<R>& s, hipStream_t stream) {
+hipError_t launch_variant(const LaunchCfg& l, bool uni, unsigned ncols, const KC& c, const KCD& cd,
+                          const KPtrs<R>& p, const KDims& d, int ssc_off, const KSatPtrs<R>& s, hipStream_t stream) {
   const KBudPtrs<R> nb{};
   if (!uni) {  // blocks not aligned to waves: per-lane block offsets, 256-lane groups
     hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, false, SAT, false>), dim3((ncols + 255) / 256),
                        dim3(256), 0, stream, c, cd, p, d, ssc_off, s, nb);
     return hipGetLastError();
   }
-  const unsigned grid = (ncols + block - 1) / block;
-  switch (block) {
+  // the tuned instantiations (register bound, compiled-in KLEV) exist for the default configuration of CLOUDSC
+  constexpr bool tuned = !SAT && std::is_same<KC, KConstDefault<R>>::value;
+  switch (l.block) {
     case 64:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 64, true, SAT, false>), dim3(grid), dim3(64), 0,
-                         stream, c, cd, p, d, ssc_off, s, nb);
-      break;
+      return launch_one<64, 0, 0, SAT, R, RD>(ncols, c, cd, p, d, ssc_off, s, stream);
     case 128:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 128, true, SAT, false>), dim3(grid), dim3(128), 0,
-                         stream, c, cd, p, d, ssc_off, s, nb);
-      break;
+      if constexpr (tuned) return launch_wpe<128, SAT, R, RD>(l, ncols, c, cd, p, d, ssc_off, s, stream);
+      return launch_one<128, 0, 0, SAT, R, RD>(ncols, c, cd, p, d, ssc_off, s, stream);
     case 512:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 512, true, SAT, false>), dim3(grid), dim3(512), 0,
-                         stream, c, cd, p, d, ssc_off, s, nb);
-      break;
+      return launch_one<512, 0, 0, SAT, R, RD>(ncols, c, cd, p, d, ssc_off, s, stream);
     default:
-      hipLaunchKernelGGL((cloudsc_column_kernel<R, RD, KC, KCD, 256, true, SAT, false>), dim3((ncols + 255) / 256),
-                         dim3(256), 0, stream, c, cd, p, d, ssc_off, s, nb);
+      if constexpr (tuned) return launch_wpe<256, SAT, R, RD>(l, ncols, c, cd, p, d, ssc_off, s, stream);
+      return launch_one<256, 0, 0, SAT, R, RD>(ncols, c, cd, p, d, ssc_off, s, stream);
   }
-  return hipGetLastError();
 }
 
 template <typename R, typename RD, typename KC, typename KCD>
-hipError_t launch_either(bool sat, int block, bool uni, unsigned ncols, const KC& c, const KCD& cd,
+hipError_t launch_either(bool sat, const LaunchCfg& l, bool uni, unsigned ncols, const KC& c, const KCD& cd,
                          const KPtrs<R>& p, const KDims& d, int ssc_off, const KSatPtrs<R>& s,
                          const KBudPtrs<R>* bud, hipStream_t stream) {
   if (bud) {  // budget-instrumented kernel: one instantiation per variant (diagnostic runs); one wave per group
@@ -1111,8 +1179,8 @@ hipError_t launch_either(bool sat, int block, bool uni, unsigned ncols, const KC
                        dim3(64), 0, stream, c, cd, p, d, ssc_off, s, *bud);
     return hipGetLastError();
   }
-  return sat ? launch_variant<true, R, RD>(block, uni, ncols, c, cd, p, d, ssc_off, s, stream)
-             : launch_variant<false, R, RD>(block, uni, ncols, c, cd, p, d, ssc_off, s, stream);
+  return sat ? launch_variant<true, R, RD>(l, uni, ncols, c, cd, p, d, ssc_off, s, stream)
+             : launch_variant<false, R, RD>(l, uni, ncols, c, cd, p, d, ssc_off, s, stream);
 }
 
 template <typename R>
@@ -1146,7 +1214,7 @@ template <typename R, typename RD>
 hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<R>& p, const KDims& d, int ssc_off,
                           hipStream_t stream, const KSatPtrs<R>* sat, const KBudPtrs<R>* bud) {
   const bool def = is_default_config(q);
-  const int block = def ? tuned_block(variant_id<R, RD>(), d.klev) : 256;
+  const LaunchCfg tl = def ? tuned_launch(variant_id<R, RD>(), d.klev) : LaunchCfg{};
   // every wave inside one block (64-lane waves, 64 | NPROMA): block offsets are uniform across the wave
   const bool uni = d.nproma % 64 == 0;
   // blocks per launch: the largest array, (NPROMA,KLEV,5) or PEXTRA (NPROMA,KLEV,KFLDX), stays below 4 GiB
@@ -1166,13 +1234,13 @@ hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<
     if (def) {
       const KConstDefault<R> c = make_kconst_default<R>(q, ptsphy);
       const KConstDefault<RD> cd = make_kconst_default<RD>(q, ptsphy);
-      e = launch_either<R, RD>(sat != nullptr, block, uni, ncols, c, cd, pc, dc, ssc_off, sc, bud ? &bc : nullptr,
+      e = launch_either<R, RD>(sat != nullptr, tl, uni, ncols, c, cd, pc, dc, ssc_off, sc, bud ? &bc : nullptr,
                                stream);
     } else {
       const KConst<R> c = make_kconst<R>(q, ptsphy);
       const KConst<RD> cd = make_kconst<RD>(q, ptsphy);
-      e = launch_either<R, RD>(sat != nullptr, 256, uni, ncols, c, cd, pc, dc, ssc_off, sc, bud ? &bc : nullptr,
-                               stream);
+      e = launch_either<R, RD>(sat != nullptr, LaunchCfg{}, uni, ncols, c, cd, pc, dc, ssc_off, sc,
+                               bud ? &bc : nullptr, stream);
     }
     if (e != hipSuccess) return e;
   }

@@ -249,15 +249,6 @@ int blocked(const cloudsc_params_t* q, int ngptot, int nproma, int klev, int ngp
   return CLOUDSC_OK;
 }
 
-// Per-block drop-in: a per-thread device workspace and stream, as the CLOUDSC per-block entry.
-struct SatWorkspace {
-  void* dev = nullptr;
-  size_t bytes = 0;
-  hipStream_t stream = nullptr;
-  int device = -1;
-};
-thread_local SatWorkspace g_sws;
-
 template <typename R, typename RD>
 int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int klev, R ptsphy, const R* pap,
               const R* paph, const R* pt, const R* pq, const R* pa, const R* pl, const R* pi, const R* tt,
@@ -291,24 +282,12 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
     off[i] = total;
     total += (arrs[i].n * sizeof(R) + 255) & ~(size_t)255;
   }
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloud_satadj: hipGetDevice", e);
-  SatWorkspace& w = g_sws;
-  if (w.device != dev) {
-    w = SatWorkspace{};
-    w.device = dev;
-    e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloud_satadj: hipStreamCreate", e);
-  }
-  if (w.bytes < total) {
-    if (w.dev) (void)hipFree(w.dev);
-    w.dev = nullptr;
-    w.bytes = 0;
-    e = hipMalloc(&w.dev, total);
-    if (e != hipSuccess) return fail(CLOUDSC_ENOMEM, "cloud_satadj: hipMalloc workspace", e);
-    w.bytes = total;
-  }
+  // the calling thread's workspace (shared with the other per-block entries)
+  cloudsc_capi::HostWorkspace* wp = nullptr;
+  const int wrc = cloudsc_capi::thread_workspace(total, "cloud_satadj", &wp);
+  if (wrc) return wrc;
+  cloudsc_capi::HostWorkspace& w = *wp;
+  hipError_t e = hipSuccess;
   char* base = (char*)w.dev;
   for (int i = 0; i < narr && e == hipSuccess; ++i)
     e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * sizeof(R), hipMemcpyHostToDevice, w.stream);

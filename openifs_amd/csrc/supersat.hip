@@ -88,15 +88,6 @@ namespace {
 using cloudsc_capi::fail;
 using cloudsc_gpu::SltPtrs;
 
-// Per-level drop-in: a per-thread device workspace and stream
-struct SsWorkspace {
-  void* dev = nullptr;
-  size_t bytes = 0;
-  hipStream_t stream = nullptr;
-  int device = -1;
-};
-thread_local SsWorkspace g_ssw;
-
 template <typename R, typename RD>
 int points(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int klev, int kftliqice, const R* pt,
            const R* pq, const R* pa, const R* pp, const R* psurf, R* t_adj, R* q_adj, R* a_adj, R* l_adj,
@@ -112,24 +103,12 @@ int points(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int klev, 
   if (rc) return rc;
   const int n = kfdia - kidia + 1, off = kidia - 1;
   const size_t nb = (size_t)n * sizeof(R), slot = (nb + 255) & ~(size_t)255;
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloud_supersatcheck: hipGetDevice", e);
-  SsWorkspace& w = g_ssw;
-  if (w.device != dev) {
-    w = SsWorkspace{};
-    w.device = dev;
-    e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
-    if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloud_supersatcheck: hipStreamCreate", e);
-  }
-  if (w.bytes < 9 * slot) {
-    if (w.dev) (void)hipFree(w.dev);
-    w.dev = nullptr;
-    w.bytes = 0;
-    e = hipMalloc(&w.dev, 9 * slot);
-    if (e != hipSuccess) return fail(CLOUDSC_ENOMEM, "cloud_supersatcheck: hipMalloc workspace", e);
-    w.bytes = 9 * slot;
-  }
+  // the calling thread's workspace (shared with the other per-block entries)
+  cloudsc_capi::HostWorkspace* wp = nullptr;
+  rc = cloudsc_capi::thread_workspace(9 * slot, "cloud_supersatcheck", &wp);
+  if (rc) return rc;
+  cloudsc_capi::HostWorkspace& w = *wp;
+  hipError_t e = hipSuccess;
   char* base = (char*)w.dev;
   R* d[9];
   for (int i = 0; i < 9; ++i) d[i] = (R*)(base + i * slot);

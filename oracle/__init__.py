@@ -66,6 +66,13 @@ def cloudsc_block(precision, params, kidia, kfdia, klon, klev, ptsphy, arrays, k
     return arrays
 
 
+def last_warnings():
+    """(negative-liquid, negative-ice) warning counts of the calling thread's last cloudsc_block call."""
+    nl, ni = ctypes.c_int64(), ctypes.c_int64()
+    lib().oracle_last_warnings(ctypes.byref(nl), ctypes.byref(ni))
+    return nl.value, ni.value
+
+
 def cloudsc_blocked(precision, params, ngptot, nproma, klev, ptsphy, arrays, nthreads=0):
     """All blocks of a blocked field set (dict of C-order numpy arrays, see openifs_amd.cloudsc.blocked_shape)."""
     from openifs_amd.cloudsc import BLOCKED_ARRAYS

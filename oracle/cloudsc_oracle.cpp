@@ -62,6 +62,9 @@ int check_budget(const cloudsc_params_t* p, int klev, const B& b) {
   return CLOUDSC_OK;
 }
 
+// Negative-condensate warning counts of the calling thread's last per-block call (oracle_last_warnings).
+thread_local int64_t g_last_warn[2] = {0, 0};
+
 int check_dims(const cloudsc_params_t* p, int kidia, int kfdia, int klon, int klev) {
   if (klon < 1 || klev < 2 || kidia < 1 || kfdia < kidia || kfdia > klon) return CLOUDSC_EDIMS;
   // CLOUD_SUPERSATCHECK reads the pressure section from KIDIA (cloudsc.F90:3776): stay inside KLON.
@@ -91,6 +94,8 @@ int run_block(const cloudsc_params_t* p, int kidia, int kfdia, int klon, int kle
                     pclv,   pcovptot, prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf,
                     pfcqrng, pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn,
                     ((p->lcldbud & kBudOut) || p->lrad_cloud_inhomog || p->lcloud_inhomog) ? &bud : nullptr};
+  g_last_warn[0] = g_last_warn[1] = 0;
+  a.nwarn = g_last_warn;
   oracle::cloudsc<R, RD>(*p, a);
   return CLOUDSC_OK;
 }
@@ -537,6 +542,10 @@ int oracle_cloudsc2_blocked_dp(ORACLE_CLOUDSC2_BLOCKED_ARGS(double)) { ORACLE_CL
 int oracle_cloudsc2_blocked_sp(ORACLE_CLOUDSC2_BLOCKED_ARGS(float)) { ORACLE_CLOUDSC2_BLOCKED_CALL(float); }
 
 int oracle_cloudsc_dp(CLOUDSC_ARGS(double)) { ORACLE_BLOCK_CALL(double, double); }
+void oracle_last_warnings(int64_t* n_negative_liquid, int64_t* n_negative_ice) {
+  *n_negative_liquid = g_last_warn[0];
+  *n_negative_ice = g_last_warn[1];
+}
 int oracle_cloudsc_sp(CLOUDSC_ARGS(float)) { ORACLE_BLOCK_CALL(float, float); }
 int oracle_cloudsc_mixed(CLOUDSC_ARGS(float)) { ORACLE_BLOCK_CALL(float, double); }
 

@@ -20,9 +20,16 @@
 // driver rejects configurations that would need them.
 //
 // Template parameters: R = JPRB (storage and bulk arithmetic), RD = type of
-// the "FP64 islands" (CLOUD_SUPERSATCHECK and the 5x5 solve).  <double,double>
-// is the ifs_dp build, <float,float> the ifs_sp build, <float,double> the
-// mixed-precision variant.
+// the "FP64 islands": CLOUD_SUPERSATCHECK, the 5x5 solve, and the water
+// budget of a level that feeds the solve -- the first-guess amounts ZQXFG,
+// the explicit source/sink matrix ZSOLQA, the sink amounts limited by ZQXFG
+// (MIN(rate, ZQXFG)) and the sink truncation.  <double,double> is the ifs_dp
+// build and <float,float> the ifs_sp build (RD = R: the reference's
+// arithmetic unchanged); <float,double> is the mixed-precision variant.  In
+// the mixed variant a species that a limited sink removes completely leaves a
+// residue of FP64 size in the solve's right-hand side, which the reference's
+// ZEPSEC = 1E-14 clip removes as in the FP64 build; with FP32 sums the residue
+// is ~1E-13 and survives (e.g. fully evaporated rain keeps ZCOVPTOT alive below).
 //
 // Notational choices (documented, not reference behaviour): a real exponent
 // of 2 or 3 (e.g. ZTP1**3._JPRB) is evaluated as a product, a real exponent
@@ -223,7 +230,7 @@ void lu5_solve(RD lhs[NCLV][NCLV], RD x[NCLV]) {
 }
 
 template <typename R, typename RD>
-void implicit_solve(const R qa[NCLV][NCLV], const R qb[NCLV][NCLV], const R fallsink[NCLV], const R qx[NCLV],
+void implicit_solve(const RD qa[NCLV][NCLV], const R qb[NCLV][NCLV], const R fallsink[NCLV], const R qx[NCLV],
                     R qxn[NCLV]) {
   RD lhs[NCLV][NCLV], x[NCLV];
   for (int jm = 0; jm < NCLV; ++jm)
@@ -274,6 +281,8 @@ struct Args {
   R *pfsqlf, *pfsqif, *pfcqnng, *pfcqlng, *pfsqrf, *pfsqsf, *pfcqrng, *pfcqsng, *pfsqltur, *pfsqitur;
   R *pfplsl, *pfplsn, *pfhpsl, *pfhpsn;
   const BudArgs<R>* bud = nullptr;  // diagnostics: switched by params.lcldbud and lrad_cloud_inhomog, NULL = none
+  // NULL, or counts of the reference's negative-condensate warnings (liquid :3794, ice :3802) to add to
+  int64_t* nwarn = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -457,7 +466,8 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
   std::vector<R> zqxnm1(KL * NCLV, R(0)), zanewm1(KL, R(0)), zda(KL, R(0)), zcovpclr(KL, R(0));
   std::vector<R> zcovpmax(KL, R(0)), zcovptot(KL, R(0)), zcldtopdist(KL, R(0));
   // Per-level work arrays
-  std::vector<R> zqxfg(KL * NCLV), zsolqa(KL * NCLV * NCLV), zsolqb(KL * NCLV * NCLV);
+  std::vector<RD> zqxfg(KL * NCLV), zsolqa(KL * NCLV * NCLV);  // RD: water budget of the level
+  std::vector<R> zsolqb(KL * NCLV * NCLV);
   std::vector<R> zfallspeed(KL * NCLV), zfallsrce(KL * NCLV), zfallsink(KL * NCLV);
   std::vector<R> zconvsrce(KL * NCLV), zconvsink(KL * NCLV), zlcust(KL * NCLV), zqxn(KL * NCLV);
   std::vector<R> zfluxq(KL * NCLV);
@@ -469,9 +479,9 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
   std::vector<R> zsnowcld(KL), zsnowcldm1(KL), zmeltmax(KL), zfrzmax(KL), zpsdep(KL), zlamr(KL), zn0r(KL);
   std::vector<R> zlams(KL), zn0s(KL), zlami(KL), zn0i(KL), zpsacr(KL), zpiacr(KL);
   std::vector<RD> zq_upd(KL), zt_upd(KL), za_upd(KL), zt_adj(KL), zq_adj(KL), za_adj(KL), zl_adj(KL), zi_adj(KL);
-  auto QA = [&](int jl, int m, int n) -> R& { return zsolqa[((size_t)n * NCLV + m) * KL + (jl - 1)]; };
+  auto QA = [&](int jl, int m, int n) -> RD& { return zsolqa[((size_t)n * NCLV + m) * KL + (jl - 1)]; };
   auto QB = [&](int jl, int m, int n) -> R& { return zsolqb[((size_t)n * NCLV + m) * KL + (jl - 1)]; };
-  auto V = [&](std::vector<R>& v, int jl, int m) -> R& { return v[(size_t)m * KL + (jl - 1)]; };
+  auto V = [&](auto& v, int jl, int m) -> auto& { return v[(size_t)m * KL + (jl - 1)]; };
   // Budget arrays ZBUDCC(KLON,12), ZBUDL(KLON,22), ZBUDI(KLON,18) (:557-559), 1-based second index.
   const int lcldbud = a.bud ? p.lcldbud : 0;
   std::vector<R> zbudcc(KL * 12), zbudl(KL * 22), zbudi(KL * 18);
@@ -804,7 +814,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
                          RCL_CONST2R * std::sqrt(zrho[i] * zfallcorr) / (std::sqrt(zcorr2) * std::pow(zlambda, RCL_CONST3R)));
         const R zdenom = R(1) + zbeta * ptsphy * zcorqsliq[i];
         const R zdpevap = zcovpclr[i] * zbeta * ptsphy * zsubsat / zdenom;
-        const R zevap = std::min(zdpevap, V(zqxfg, jl, QR));
+        const RD zevap = std::min<RD>(zdpevap, V(zqxfg, jl, QR));
         QA(jl, QV, QR) = QA(jl, QV, QR) + zevap;
         QA(jl, QR, QV) = QA(jl, QR, QV) - zevap;
         BL(jl, 20) = -zevap * ZQTMST;  // :2363
@@ -834,8 +844,8 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         const R zcvds = R(7.8) * std::pow(zicenuclei / zrho[i], R(0.666)) * zsupersatice / (R(8.87) * (zadd + zbdd));
         const R zice0 = std::max(zicecld[i], zicenuclei * RICEINIT / zrho[i]);
         const R zinew = std::pow(R(0.666) * zcvds * ptsphy + std::pow(zice0, R(0.666)), R(1.5));
-        R zdepos = std::max(zoverlap_liqice * za[k] * (zinew - zice0), R(0));
-        zdepos = std::min(zdepos, V(zqxfg, jl, QL));
+        RD zdepos = std::max(zoverlap_liqice * za[k] * (zinew - zice0), R(0));
+        zdepos = std::min<RD>(zdepos, V(zqxfg, jl, QL));
         const R zinfactor = std::min(zicenuclei / R(15000.), R(1));
         zdepos = zdepos * std::min(zinfactor + (R(1) - zinfactor) * (RDEPLIQREFRATE + zcldtopdist[i] / RDEPLIQREFDEPTH), R(1));
         QA(jl, QI, QL) = QA(jl, QI, QL) + zdepos;
@@ -922,7 +932,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         const int i = jl - 1;
         if (zmeltmax[i] > ZEPSEC && zicetot[i] > ZEPSEC) {
           const R zalfa = V(zqxfg, jl, jm) / zicetot[i];
-          const R zmelt = std::min(V(zqxfg, jl, jm), zalfa * zmeltmax[i]);
+          const RD zmelt = std::min<RD>(V(zqxfg, jl, jm), zalfa * zmeltmax[i]);
           V(zqxfg, jl, jm) = V(zqxfg, jl, jm) - zmelt;
           V(zqxfg, jl, jn) = V(zqxfg, jl, jn) + zmelt;
           QA(jl, jn, jm) = QA(jl, jn, jm) + zmelt;
@@ -971,7 +981,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
     for (int jl = kidia; jl <= kfdia; ++jl) {
       const int i = jl - 1;
       if (zfrzmax[i] > ZEPSEC && V(zqxfg, jl, QL) > ZEPSEC) {
-        const R zfrz = std::min(V(zqxfg, jl, QL), zfrzmax[i]);
+        const RD zfrz = std::min<RD>(V(zqxfg, jl, QL), zfrzmax[i]);
         QA(jl, QI, QL) = QA(jl, QI, QL) + zfrz;
         QA(jl, QL, QI) = QA(jl, QL, QI) - zfrz;
         BL(jl, 19) = -zfrz * ZQTMST;  // :2958
@@ -1061,11 +1071,11 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
         const R zdenom = R(1) + zbeta * ptsphy * zcorqsice[i];
         const R zdpr = zcovpclr[i] * zbeta * (zqsice[k] - zqe) / zdenom * zdp[i] * ZRG_R;
         const R zdpevap = zdpr * zdtgdp[i];
-        const R zevap = std::min(zdpevap, V(zqxfg, jl, QS));
+        const RD zevap = std::min<RD>(zdpevap, V(zqxfg, jl, QS));
         QA(jl, QV, QS) = QA(jl, QV, QS) + zevap;
         QA(jl, QS, QV) = QA(jl, QS, QV) - zevap;
         BI(jl, 17) = -zevap * ZQTMST;  // :3304
-        zcovptot[i] = std::max(RCOVPMIN, zcovptot[i] - std::max(R(0), (zcovptot[i] - za[k]) * zevap / V(zqxfg, jl, QS)));
+        zcovptot[i] = std::max(RCOVPMIN, zcovptot[i] - std::max(R(0), R((zcovptot[i] - za[k]) * zevap / V(zqxfg, jl, QS))));
         V(zqxfg, jl, QS) = V(zqxfg, jl, QS) - zevap;
       }
     }
@@ -1093,7 +1103,8 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
 
     // 5.2.1 truncate explicit sinks (:3455-3536) and 5.2.2 solve (:3546-3628), per column
     for (int jl = kidia; jl <= kfdia; ++jl) {
-      R qa[NCLV][NCLV], qb[NCLV][NCLV], qx[NCLV], fs[NCLV], qxn[NCLV];
+      RD qa[NCLV][NCLV], qxd[NCLV];
+      R qb[NCLV][NCLV], qx[NCLV], fs[NCLV], qxn[NCLV];
       for (int m = 0; m < NCLV; ++m) {
         qx[m] = zqx[i3(jl, jk, m + 1)];
         fs[m] = V(zfallsink, jl, m);
@@ -1102,7 +1113,8 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
           qb[m][n] = QB(jl, m, n);
         }
       }
-      truncate_sinks<R>(qa, qx, ZEPSEC);
+      for (int m = 0; m < NCLV; ++m) qxd[m] = RD(qx[m]);
+      truncate_sinks<RD>(qa, qxd, RD(ZEPSEC));
       implicit_solve<R, RD>(qa, qb, fs, qx, qxn);
       for (int m = 0; m < NCLV; ++m) {
         V(zqxn, jl, m) = qxn[m];
@@ -1190,12 +1202,14 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
       BL(jl, 2) = R(zl_adj[i]) * ZQTMST;
       BI(jl, 2) = R(zi_adj[i]) * ZQTMST;
       const R zl = zqx0[i3(jl, jk, QL + 1)] + a.tloc_cld[i3(jl, jk, QL + 1)] * ptsphy;
+      if (a.nwarn && zl < -RLMIN) ++a.nwarn[0];  // WRITE(NULOUT,*) 'CLOUDSC: WARNING Negative cloud liquid!' (:3794)
       if (zl < R(0)) {
         a.tloc_cld[i3(jl, jk, QL + 1)] = a.tloc_cld[i3(jl, jk, QL + 1)] - zl * ZQTMST;
         a.tloc_q[k] = a.tloc_q[k] + zl * ZQTMST;
         a.tloc_t[k] = a.tloc_t[k] - RALVDCP * zl * ZQTMST;
       }
       const R zi = zqx0[i3(jl, jk, QI + 1)] + a.tloc_cld[i3(jl, jk, QI + 1)] * ptsphy;
+      if (a.nwarn && zi < -RLMIN) ++a.nwarn[1];  // 'CLOUDSC: WARNING Negative cloud ice!' (:3802)
       if (zi < R(0)) {
         a.tloc_cld[i3(jl, jk, QI + 1)] = a.tloc_cld[i3(jl, jk, QI + 1)] - zi * ZQTMST;
         a.tloc_q[k] = a.tloc_q[k] + zi * ZQTMST;

@@ -86,16 +86,19 @@ def test_satur_lphylin_form_equals_foeewm(params137, case):
     np.testing.assert_allclose(a1["pqsat"][real], a0["pqsat"][real], rtol=1e-13)
 
 
-@pytest.mark.parametrize("lphylin,ktdia", [(0, 1), (1, 1), (0, 5)])
-def test_layer_is_cloudsc2_on_its_pqsat(params137, case, lphylin, ktdia):
+@pytest.mark.parametrize("lphylin,ktdia,ldrain1d", [(0, 1, False), (1, 1, False), (0, 5, False), (0, 1, True)])
+def test_layer_is_cloudsc2_on_its_pqsat(params137, case, lphylin, ktdia, ldrain1d):
+    """The layer runs CLOUDSC2 on the PQSAT it computes; with LDRAIN1D both take CUADJTQS, so the water closure
+    and the no-supersaturation pins of CLOUDSC2 with LDRAIN1D (tests/test_cloudsc2.py) carry over."""
     p, q2 = params137, params2(params137, lphylin)
-    lay = run_layer("dp", p, q2, case, 500, 32, 137, ktdia)
+    lay = run_layer("dp", p, q2, case, 500, 32, 137, ktdia, ldrain1d=ldrain1d)
     c2 = {c2n: case[ln].copy() for ln, c2n in C2.LAYER_FROM_CLOUDSC2.items()}  # noqa: F841
     c2["pqs"] = np.where(np.isnan(lay["pqsat"]), 0.3, lay["pqsat"])  # levels above KTDIA are not read
     for name, kind, role in C2.CLOUDSC2_ARRAYS:
         if role == "o":
             c2[name] = np.full(C2.blocked_shape(kind, 32, 137, 16), np.nan)
-    oracle.cloudsc2_blocked("dp", p, q2, C2.reference_eta(137), 500, 32, ktdia, 137, False, PTSPHY, c2, nthreads=8)
+    oracle.cloudsc2_blocked("dp", p, q2, C2.reference_eta(137), 500, 32, ktdia, 137, ldrain1d, PTSPHY, c2,
+                            nthreads=8)
     for ln, c2n in [("ptendency_loc_t", "ptent"), ("ptendency_loc_q", "ptenq"), ("ptendency_loc_l", "ptenl"),
                     ("ptendency_loc_i", "pteni"), ("pclc", "pclc"), ("pfplsl", "pfplsl"), ("pfplsn", "pfplsn"),
                     ("pfhpsl", "pfhpsl"), ("pfhpsn", "pfhpsn"), ("pcovptot", "pcovptot")]:
@@ -132,13 +135,13 @@ def test_optional_outputs_may_be_null(params137, case):
 # ----------------------------------------------------------------------------------------------------------------
 # GPU parity
 # ----------------------------------------------------------------------------------------------------------------
-def gpu_layer(p, q2, host, ngptot, nproma, klev, prec, ktdia=1, drop=()):
+def gpu_layer(p, q2, host, ngptot, nproma, klev, prec, ktdia=1, drop=(), ldrain1d=False):
     import torch
 
     dt = torch.float64 if prec == "dp" else torch.float32
     f = {k: torch.from_numpy(np.ascontiguousarray(v)).to("cuda", dt) for k, v in host.items() if k not in drop}
     ceta = torch.from_numpy(C2.reference_eta(klev)).to("cuda", dt)
-    C2.cloud_s_layer_blocked(f, p, q2, ceta, ngptot, nproma, klev, PTSPHY, prec, ktdia)
+    C2.cloud_s_layer_blocked(f, p, q2, ceta, ngptot, nproma, klev, PTSPHY, prec, ktdia, ldrain1d=ldrain1d)
     torch.cuda.synchronize()
     return {n: f[n].double().cpu().numpy() for n, k, r, o in C2.LAYER_ARRAYS if r == "o" and n in f}
 
@@ -147,15 +150,17 @@ OUTS = [n for n, k, r, o in C2.LAYER_ARRAYS if r == "o"]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lphylin,ktdia,ngptot,nproma,klev", [(0, 1, 2048, 64, 137), (1, 1, 2048, 64, 137),
-                                                                (0, 5, 777, 32, 91)])
-def test_gpu_layer_fp64_matches_oracle(params137, params91, lphylin, ktdia, ngptot, nproma, klev):
+@pytest.mark.parametrize("lphylin,ktdia,ngptot,nproma,klev,ldrain1d",
+                         [(0, 1, 2048, 64, 137, False), (1, 1, 2048, 64, 137, False), (0, 5, 777, 32, 91, False),
+                          (0, 1, 2048, 64, 137, True), (0, 5, 777, 32, 91, True)])
+def test_gpu_layer_fp64_matches_oracle(params137, params91, lphylin, ktdia, ngptot, nproma, klev, ldrain1d):
+    """LDRAIN1D with LPHYLIN off: CUADJTQS with the FOEEWM form of SATUR in the layer."""
     p = params137 if klev == 137 else params91
     q2 = params2(p, lphylin, 1)
     host = layer_case(p, 3, ngptot, nproma, klev, "dp")
-    print(f"cloud_s_layer dp lphylin={lphylin} ktdia={ktdia} {ngptot}x{klev}", flush=True)
-    out = gpu_layer(p, q2, host, ngptot, nproma, klev, "dp", ktdia)
-    ref = run_layer("dp", p, q2, host, ngptot, nproma, klev, ktdia)
+    print(f"cloud_s_layer dp lphylin={lphylin} ktdia={ktdia} ldrain1d={ldrain1d} {ngptot}x{klev}", flush=True)
+    out = gpu_layer(p, q2, host, ngptot, nproma, klev, "dp", ktdia, ldrain1d=ldrain1d)
+    ref = run_layer("dp", p, q2, host, ngptot, nproma, klev, ktdia, ldrain1d=ldrain1d)
     for n in OUTS:
         np.testing.assert_array_equal(np.isnan(out[n]), np.isnan(ref[n]), err_msg=n)
         assert relrms(out[n], ref[n]) <= 1e-12, n
@@ -212,10 +217,13 @@ def st_case(params137):
     return st_inputs(C2.synth_host(params137, 23, 512, 32, 137))
 
 
-@pytest.mark.parametrize("lphylin,levap", [(0, 0), (0, 1), (1, 0)])
-def test_cloudst_no_supersaturation_left(params137, st_case, lphylin, levap):
+@pytest.mark.parametrize("lphylin,levap,ldrain1d", [(0, 0, False), (0, 1, False), (1, 0, False), (0, 0, True)])
+def test_cloudst_no_supersaturation_left(params137, st_case, lphylin, levap, ldrain1d):
+    """LDRAIN1D takes the LPHYLIN paths (CUADJTQS, the tanh phase split) and the evaporation of LEVAPLS2
+    (cloudst.F90:359,464,481,508,574)."""
     p = params137
-    a = run_cloudst("dp", p, params2(p, lphylin, levap, 0), st_case, 512, 32, 137)
+    a = run_cloudst("dp", p, params2(p, lphylin, levap, 0), st_case, 512, 32, 137, ldrain1d=ldrain1d)
+    lphylin = lphylin or ldrain1d
     if lphylin:
         # CUADJTQS takes its phase from the temperature it is given: exclude warm points that the evaporating
         # detrainment cools across RTT before the adjustment (the bookkeeping of final_rh covers the rest)
@@ -295,15 +303,18 @@ def test_layer_cloudst_branch(params137, case, ktdia):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("lphylin,levap,ktdia,ngptot,nproma,klev", [(0, 0, 1, 2048, 64, 137), (0, 1, 1, 2048, 64, 137),
-                                                                    (1, 0, 5, 777, 32, 91)])
-def test_gpu_cloudst_fp64_matches_oracle(params137, params91, lphylin, levap, ktdia, ngptot, nproma, klev):
+@pytest.mark.parametrize("lphylin,levap,ktdia,ngptot,nproma,klev,ldrain1d",
+                         [(0, 0, 1, 2048, 64, 137, False), (0, 1, 1, 2048, 64, 137, False),
+                          (1, 0, 5, 777, 32, 91, False), (0, 0, 1, 2048, 64, 137, True), (0, 0, 5, 777, 32, 91, True)])
+def test_gpu_cloudst_fp64_matches_oracle(params137, params91, lphylin, levap, ktdia, ngptot, nproma, klev, ldrain1d):
+    """LDRAIN1D with LPHYLIN off: the tanh phase function, CUADJTQS and the evaporation all on."""
     import torch
 
     p = params137 if klev == 137 else params91
     q2 = params2(p, lphylin, levap, 0)
     inp = st_inputs(C2.synth_host(p, 29, ngptot, nproma, klev, "dp"))
-    print(f"cloudst dp lphylin={lphylin} levap={levap} ktdia={ktdia} {ngptot}x{klev}", flush=True)
+    print(f"cloudst dp lphylin={lphylin} levap={levap} ktdia={ktdia} ldrain1d={ldrain1d} {ngptot}x{klev}",
+          flush=True)
     ngpblks = (ngptot + nproma - 1) // nproma
     f = {k: torch.from_numpy(v).to("cuda") for k, v in inp.items()}
     for n, kind, role in C2.CLOUDST_ARRAYS:
@@ -311,9 +322,9 @@ def test_gpu_cloudst_fp64_matches_oracle(params137, params91, lphylin, levap, kt
             f[n] = torch.full(C2.blocked_shape(kind, nproma, klev, ngpblks), float("nan"), dtype=torch.float64,
                               device="cuda")
     ceta = torch.from_numpy(C2.reference_eta(klev)).to("cuda")
-    C2.cloudst_blocked(f, p, q2, ceta, ngptot, nproma, klev, PTSPHY, "dp", ktdia)
+    C2.cloudst_blocked(f, p, q2, ceta, ngptot, nproma, klev, PTSPHY, "dp", ktdia, ldrain1d=ldrain1d)
     torch.cuda.synchronize()
-    ref = run_cloudst("dp", p, q2, inp, ngptot, nproma, klev, ktdia)
+    ref = run_cloudst("dp", p, q2, inp, ngptot, nproma, klev, ktdia, ldrain1d=ldrain1d)
     for n, kind, role in C2.CLOUDST_ARRAYS:
         if role == "o":
             g = f[n].cpu().numpy()

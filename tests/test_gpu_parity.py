@@ -2,12 +2,25 @@
 
 FP64: bit-level agreement is expected (differences only from FMA contraction
 and the device's libm): per-field relative RMS <= 1e-12 and an identical
-write set.  FP32 and mixed: CLOUDSC's near-zero thresholds (ZEPSEC = 1e-14
-against FP32 cancellation residues of ~1e-12, the supersaturation switch)
-make FP32 results depend chaotically on rounding, so the FP32 oracle itself
-is ~1e-2 (relative RMS) away from FP64.  The criterion is that the GPU is as
-accurate as the FP32 oracle: its relative RMS against the FP64 oracle must
-not exceed twice the FP32 oracle's, field by field.
+write set.
+
+sp (the ifs_sp build, all arithmetic in FP32): CLOUDSC's near-zero thresholds
+(ZEPSEC = 1e-14 against FP32 cancellation residues of ~1e-13, the
+supersaturation switch) make FP32 results depend chaotically on rounding, so
+the FP32 oracle itself is ~1e-2 (relative RMS) away from FP64.  The criterion
+is that the GPU is as accurate as the FP32 oracle: its relative RMS against
+the FP64 oracle must not exceed twice the FP32 oracle's, field by field.
+
+mixed (FP32 storage and bulk arithmetic; FP64 in CLOUD_SUPERSATCHECK, the 5x5
+solve and the level's water budget ZQXFG/ZSOLQA with the limited sinks and the
+sink truncation): the target tolerance of 1e-4 relative RMS against the FP64
+oracle on identical (FP32-representable) inputs, for every output except the
+cloud-cover tendency.  The cloud-cover tendency jumps by up to 1/dt where
+CLOUD_SUPERSATCHECK compares an updated humidity with its limit and the two
+agree to a few 1e-7 (after a sink limited to reach saturation); FP32 state
+decides ~1e-5 of the points differently from FP64.  It is checked as: at most
+1e-4 of the points differ by more than 1e-4 of the field's maximum, and the
+relative RMS over the others is <= 1e-4.
 """
 import ctypes
 
@@ -111,13 +124,13 @@ def test_per_block_kidia_window_fp64(params137):
     assert np.all(a_gpu["prainfrac_toprfz"][outside] == 0.0)
 
 
-@pytest.mark.parametrize("prec,klev", [("sp", 137), ("mixed", 137), ("sp", 91), ("mixed", 91)])
-def test_fp32_as_accurate_as_fp32_oracle(prec, klev, params137, params91):
+@pytest.mark.parametrize("klev", [137, 91])
+def test_sp_as_accurate_as_fp32_oracle(klev, params137, params91):
     params = params137 if klev == 137 else params91
     ngptot, nproma = 8192, 64
-    inp, out = gpu_run(params, prec, ngptot, nproma, klev, 51)
+    inp, out = gpu_run(params, "sp", ngptot, nproma, klev, 51)
     ref64 = oracle_blocked("dp", params, inp, ngptot, nproma, klev, nthreads=16)
-    ref32 = oracle_blocked(prec, params, inp, ngptot, nproma, klev, nthreads=16)
+    ref32 = oracle_blocked("sp", params, inp, ngptot, nproma, klev, nthreads=16)
     report = []
     for n in FIELDS:
         g, r64, r32 = (columns(x[n], nproma) for x in (out, ref64, ref32))
@@ -126,6 +139,51 @@ def test_fp32_as_accurate_as_fp32_oracle(prec, klev, params137, params91):
         report.append((n, e_gpu, e_cpu))
         assert e_gpu <= 2.0 * e_cpu + 1e-6, f"{n}: GPU {e_gpu:.3e} vs FP32 oracle {e_cpu:.3e}"
     print("\n".join(f"{n:20s} gpu {a:.3e}  fp32-oracle {b:.3e}" for n, a, b in report))
+
+
+MIXED_TOL = 1e-4  # relative RMS against the FP64 oracle
+
+
+def check_mixed_against_fp64(out, ref64, nproma, report=None):
+    """The mixed variant's tolerance: MIXED_TOL on every output but the cloud-cover tendency, whose
+    supersaturation-check ties are bounded in number (see the module docstring)."""
+    for n in FIELDS:
+        g, r = columns(out[n], nproma), columns(ref64[n], nproma)
+        assert_same_write_set(g, r, n)
+        if n == "ptendency_loc_a":
+            m = ~np.isnan(r)
+            d = np.abs(g[m].astype(np.float64) - r[m])
+            flip = d > 1e-4 * np.abs(r[m]).max()
+            assert flip.mean() <= 1e-4, f"{n}: {flip.sum()} of {flip.size} points differ"
+            e = relrms(np.where(flip, r[m], g[m]), r[m])
+            if report is not None:
+                report.append((n, e, int(flip.sum())))
+            assert e <= MIXED_TOL, f"{n}: {e:.3e} outside the tie points"
+            continue
+        e = relrms(g, r)
+        if report is not None:
+            report.append((n, e, 0))
+        assert e <= MIXED_TOL, f"{n}: GPU mixed {e:.3e} against FP64"
+    sp = lambda x: columns(x["pfplsl"], nproma)[:, -1].astype(np.float64) + columns(x["pfplsn"], nproma)[:, -1]
+    e = relrms(sp(out), sp(ref64))
+    if report is not None:
+        report.append(("surface precipitation", e, 0))
+    assert e <= MIXED_TOL
+
+
+@pytest.mark.parametrize("klev", [137, 91])
+def test_mixed_within_1e4_of_fp64(klev, params137, params91):
+    params = params137 if klev == 137 else params91
+    ngptot, nproma = 8192, 64
+    inp, out = gpu_run(params, "mixed", ngptot, nproma, klev, 51)
+    ref64 = oracle_blocked("dp", params, inp, ngptot, nproma, klev, nthreads=16)
+    report = []
+    check_mixed_against_fp64(out, ref64, nproma, report)
+    # the GPU's mixed variant follows the oracle's: same policy, same arithmetic up to transcendentals
+    refm = oracle_blocked("mixed", params, inp, ngptot, nproma, klev, nthreads=16)
+    for n in ("ptendency_loc_t", "ptendency_loc_q", "ptendency_loc_cld", "pfplsl", "pfplsn"):
+        assert relrms(columns(out[n], nproma), columns(refm[n], nproma)) <= 1e-4, n
+    print("\n".join(f"{n:22s} {e:.3e}  ties {t}" for n, e, t in report))
 
 
 def _budgets_torch(params, f, k0):
