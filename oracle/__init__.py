@@ -14,19 +14,23 @@ Parity status of each oracle function (what pins it; see tests/test_oracle_*.py)
   truncate_sinks                 pinned: brute-force positivity and ordering properties
   cloudsc (whole column)         pinned by column water/enthalpy conservation, the reference's
                                  write set and special cases (dry, clear-sky, warm, saturated)
-  section-4 processes            pinned: snow sedimentation (implicit upwind flux form), KK2000
-                                 autoconversion, homogeneous freezing (tests/test_oracle_processes.py);
-                                 parity unpinned: rain evaporation, ice and snow deposition, ice->snow
-                                 autoconversion, melting, rain freezing, riming, PSACR/PIACR, snow
-                                 sublimation, erosion, accretion (no reference run or golden vector exists)
+  section-4 processes            pinned by single-process columns against the published rate laws with the
+                                 SUCLDP constants, size-distribution integrals by quadrature and the reference's
+                                 time discretisation (tests/test_oracle_processes.py, test_oracle_processes2.py),
+                                 each checked to fail against a planted mistake: snow sedimentation, KK2000
+                                 autoconversion and accretion, homogeneous freezing, turbulent erosion (rate and
+                                 saturation limit), ice deposition (Meyers nuclei, Rotstayn growth), snow deposition,
+                                 ice->snow autoconversion, melting, rain freezing (Wisner, and the heat-limited
+                                 refreezing), riming, PSACR, PIACR, rain evaporation, snow sublimation
   budget diagnostics (LCLDBUD*)  pinned (tests/test_budget.py): liquid, ice and cloud-cover closure, the
                                  temperature budget against the liquid/ice fields, vertical integrals against the
                                  integrated 3D fields, counter closed forms, process regimes, TIMEINT/LBUD23/
                                  accumulation; the rain-only rates are pinned only through those identities
   condensate FSD (PFSD)          ice FSD pinned (tests/test_fsd.py): structure recovered from two PGAW values
                                  (detrainment ratio, layer-depth/total-water factor, overcast/broken branch),
-                                 defaults and bounds; liquid FSD (Boutle et al. 2013, IFS parameters): parity
-                                 unpinned
+                                 defaults and bounds; liquid FSD pinned by the structure of the Boutle et al. (2013)
+                                 form (power laws in the grid length, linear in cover, overcast ratio;
+                                 tests/test_oracle_processes2.py)
   LCLOUD_INHOMOG factors         pinned (tests/test_inhomog.py): autoconversion and accretion enhancement equal the
                                  lognormal and bivariate-lognormal moment ratios by Gauss-Hermite quadrature in an
                                  overcast warm layer; no change without liquid

@@ -12,7 +12,7 @@ CPU pins of the oracle:
     where ZANEWP > 0.95 and 1.5 below.
 GPU: the diagnostics entry against the oracle (FP64 1e-12, FP32 and mixed as accurate as the FP32 oracle) and the
 per-block entry against the blocked one bit for bit.  The liquid FSD (Boutle et al. 2013 with the IFS
-parameters) is compared with the oracle but has no independent pin: parity unpinned."""
+parameters) is pinned by its structure in tests/test_oracle_processes2.py."""
 import numpy as np
 import pytest
 
