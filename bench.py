@@ -56,16 +56,23 @@ def read_traffic(nproma, name="cloudsc_sp_l137_655360_traffic.md"):
     return float(m.group(2))
 
 
-def cpu_baseline(params, sample_cols, nthreads, min_seconds):
-    """FP32 CPU oracle over `sample_cols` columns of the workload, repeated until `min_seconds` have
-    elapsed (PLUDE restored before each pass).  Returns (columns/s, seconds, passes)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_arrays(params, sample_cols, nproma):
     import numpy as np
     import torch
 
-    import oracle
     from openifs_amd import cloudsc as C
 
-    nproma = 32
     f = C.BlockedFields(sample_cols, nproma, KLEV, "sp", "cuda", names=set(C.INPUT_NAMES))
     C.synth_device(f, params, 1234)
     arr = {n: f[n].cpu().numpy() for n in C.INPUT_NAMES}
@@ -75,18 +82,38 @@ def cpu_baseline(params, sample_cols, nthreads, min_seconds):
         if role == "o":
             arr[name] = np.empty(C.blocked_shape(kind, nproma, KLEV, (sample_cols + nproma - 1) // nproma),
                                  dtype=np.float32)
+    return arr
+
+
+def cpu_baseline(params, sample_cols, nthreads, min_seconds, npromas=(8, 16, 64, 128), sweep_cols=16384):
+    """FP32 CPU oracle over `sample_cols` columns of the workload in NPROMA = 32 blocks: one warm-up pass, then
+    passes (PLUDE restored before each) until `min_seconds` have elapsed and at least 5 passes ran; then one pass of
+    `sweep_cols` columns per other NPROMA.  Returns (columns/s of the median NPROMA-32 pass, seconds, passes,
+    {nproma: columns/s})."""
+    import oracle
+
+    nproma = 32
+    arr = _cpu_arrays(params, sample_cols, nproma)
     plude0 = arr["plude"].copy()
     small = 4096
     warm = {k: (v[: small // nproma].copy()) for k, v in arr.items()}
     oracle.cloudsc_blocked("sp", params, small, nproma, KLEV, 900.0, warm, nthreads=nthreads)
-    passes, dt = 0, 0.0
-    while dt < min_seconds and passes < 200:
+    times = []
+    while (sum(times) < min_seconds or len(times) < 5) and len(times) < 200:
         arr["plude"][...] = plude0
         t0 = time.perf_counter()
         oracle.cloudsc_blocked("sp", params, sample_cols, nproma, KLEV, 900.0, arr, nthreads=nthreads)
-        dt += time.perf_counter() - t0
-        passes += 1
-    return sample_cols * passes / dt, dt, passes
+        times.append(time.perf_counter() - t0)
+    dt, passes = sum(times), len(times)
+    per_nproma = {nproma: sample_cols / sorted(times)[passes // 2]}
+    for npr in npromas:
+        a = _cpu_arrays(params, sweep_cols, npr)
+        oracle.cloudsc_blocked("sp", params, small, npr, KLEV, 900.0,
+                               {k: v[: max(1, small // npr)].copy() for k, v in a.items()}, nthreads=nthreads)
+        t0 = time.perf_counter()
+        oracle.cloudsc_blocked("sp", params, sweep_cols, npr, KLEV, 900.0, a, nthreads=nthreads)
+        per_nproma[npr] = sweep_cols / (time.perf_counter() - t0)
+    return per_nproma[nproma], dt, passes, per_nproma
 
 
 def mlp_flops_per_column(din, dout):
@@ -503,7 +530,26 @@ def cloudsc2_bench(params, nproma, steps, cpu_threads, cpu_seconds):
     return res
 
 
+def host_cpus():
+    """CPUs this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
+
+
+# CPU threads one GPU's share of a GPU host provides to its process (OMP_NUM_THREADS is set to it there)
+CPU_SHARE = 16
+
+
+def cpu_threads(args):
+    return args.cpu_threads or min(CPU_SHARE, int(os.environ.get("OMP_NUM_THREADS", CPU_SHARE)), host_cpus())
+
+
 def main():
+    # OpenMP placement of the CPU oracle's threads, before its runtime starts: one thread per core, packed
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
@@ -605,7 +651,7 @@ def main():
     satadj = None
     if not args.no_satadj:
         satadj = satadj_bench(params, args.nproma, max(5, min(args.steps, 20)), 2,
-                              args.cpu_threads or min(16, os.cpu_count() or 1), 0.0 if args.no_cpu else 3.0)
+                              cpu_threads(args), 0.0 if args.no_cpu else 3.0)
 
     budget = None
     if not args.no_budget:
@@ -614,16 +660,22 @@ def main():
     cloudsc2 = None
     if not args.no_cloudsc2:
         cloudsc2 = cloudsc2_bench(params, args.nproma, max(5, min(args.steps, 20)),
-                                  args.cpu_threads or min(16, os.cpu_count() or 1), 0.0 if args.no_cpu else 3.0)
+                                  cpu_threads(args), 0.0 if args.no_cpu else 3.0)
 
     cpu = None
     if not args.no_cpu:
-        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cps, cdt, passes = cpu_baseline(params, args.cpu_sample, nthreads, args.cpu_seconds)
+        nthreads = cpu_threads(args)
+        cps, cdt, passes, per_nproma = cpu_baseline(params, args.cpu_sample, nthreads, args.cpu_seconds)
+        best = max(per_nproma, key=per_nproma.get)
         cpu = {"value": cps, "unit": "columns/s", "cores": nthreads, "kind": "port",
-               "sample": f"{args.cpu_sample} columns of the same synthetic distributions x {passes} passes "
-                         f"({cdt:.1f} s), FP32 oracle (C++/OpenMP restatement of CLOUDSC), NPROMA=32 blocks, "
-                         f"SCHEDULE(DYNAMIC,1)"}
+               "sample": f"{args.cpu_sample} columns of the same synthetic distributions, median of {passes} passes "
+                         f"({cdt:.1f} s) after one warm-up, FP32 oracle (C++/OpenMP restatement of CLOUDSC), "
+                         f"NPROMA=32 blocks, SCHEDULE(DYNAMIC,1)",
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "host_cpus_available": host_cpus(),
+               "threads": nthreads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+               "omp_places": os.environ.get("OMP_PLACES"),
+               "columns_per_s_by_nproma": {str(k): v for k, v in sorted(per_nproma.items())},
+               "best_nproma": best}
 
     line = {
         "metric": METRIC,

@@ -7,7 +7,14 @@
 
 namespace cloudsc_gpu {
 
+// CLOUDSC_FAST_TRANSCENDENTALS selects all three hardware forms below (exp/pow/log, division, square root);
+// CLOUDSC_FAST_EXP, CLOUDSC_FAST_DIV and CLOUDSC_FAST_SQRT select them one by one.
 #ifdef CLOUDSC_FAST_TRANSCENDENTALS
+#define CLOUDSC_FAST_EXP
+#define CLOUDSC_FAST_DIV
+#define CLOUDSC_FAST_SQRT
+#endif
+#ifdef CLOUDSC_FAST_EXP
 // FP32 exp and pow on the hardware base-2 instructions (v_exp_f32, v_log_f32):
 // exp(x) = 2^(x log2 e), pow(x,y) = 2^(y log2 x).  Relative error ~ (1 + |y log2 x|) ulp.
 __device__ __forceinline__ float m_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
@@ -21,7 +28,7 @@ __device__ __forceinline__ float m_log(float x) { return logf(x); }
 __device__ __forceinline__ double m_exp(double x) { return exp(x); }
 __device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
 __device__ __forceinline__ double m_log(double x) { return log(x); }
-#ifdef CLOUDSC_FAST_TRANSCENDENTALS
+#ifdef CLOUDSC_FAST_DIV
 // FP32 division as reciprocal (v_rcp_f32, 1 ulp) times numerator instead of the ~10-instruction
 // IEEE sequence.  Measured against the FP64 oracle on 8192 L137 columns, the FP32 kernel's
 // relative RMS error per output field stays within 7% of the IEEE build's (both are dominated
@@ -31,7 +38,13 @@ __device__ __forceinline__ float m_div(float a, float b) { return a * __builtin_
 __device__ __forceinline__ float m_div(float a, float b) { return a / b; }
 #endif
 __device__ __forceinline__ double m_div(double a, double b) { return a / b; }
-#ifdef CLOUDSC_FAST_TRANSCENDENTALS
+// Quotients that are exactly 1, or exactly the numerator's factor, when the denominator is the numerator's own
+// sum or factor (the liquid share of the condensate, the rain share of the precipitation, the precipitation
+// cover's overlap factor when the cloud cover does not grow): IEEE division.  a * rcp(a) can be 1 - 2^-24
+// instead, which leaves a share of ~6e-8 to the other phase and switches on its processes.
+template <typename T>
+__device__ __forceinline__ T m_share(T a, T b) { return a / b; }
+#ifdef CLOUDSC_FAST_SQRT
 // square roots occur only inside rate expressions: v_sqrt_f32 (1 ulp)
 __device__ __forceinline__ float m_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 #else

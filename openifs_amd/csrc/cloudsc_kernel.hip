@@ -241,7 +241,7 @@ cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p, const KDims d,
     const R zli = ql + qi;
     R zliqfrac = R(0), zicefrac = R(0);
     if (zli > rlmin) {
-      zliqfrac = m_div(ql, zli);
+      zliqfrac = m_share(ql, zli);
       zicefrac = R(1) - zliqfrac;
     }
 
@@ -437,8 +437,8 @@ cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p, const KDims d,
       // 4.2 precipitation fraction (:1898-1919)
       R zcovpclr = R(0), zraincld = R(0), zraincldm1 = R(0), zsnowcld = R(0);
       if (zqpretot > zepsec) {
-        zcovptot = R(1) - m_div((R(1) - zcovptot) * (R(1) - m_max(za, za_prev)),
-                                 R(1) - m_min(za_prev, R(1) - R(1.E-06)));
+        zcovptot = R(1) - m_share((R(1) - zcovptot) * (R(1) - m_max(za, za_prev)),
+                                   R(1) - m_min(za_prev, R(1) - R(1.E-06)));
         zcovptot = m_max(zcovptot, c.rcovpmin);
         zcovpclr = m_max(R(0), zcovptot - za);
         zraincld = m_div(R(0.5) * (qr + qr_prev), zcovptot);
@@ -594,7 +594,7 @@ cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p, const KDims d,
       if (qr > zepsec) {
         if (ztp1 <= c.rtt && t_prev > c.rtt) {
           const R zqp = m_max(qs + qr, zepsec);
-          prainfrac = m_div(qr, zqp);
+          prainfrac = m_share(qr, zqp);
           llrainliq = prainfrac > R(0.8f);
         }
         if (ztp1 < c.rtt) {
@@ -770,27 +770,29 @@ cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p, const KDims d,
         }
       }
 
-      // 5.2.2 implicit solve (:3546-3628) in RD; ZQLHS is lower triangular here.
+      // 5.2.2 implicit solve (:3546-3628); ZQLHS is lower triangular here.  The right-hand side -- the level's
+      // water budget -- is summed in RD and the elimination runs in R: a species removed completely by a limited
+      // sink has a right-hand side of RD size, which the ZEPSEC clip below removes.
       R qxn_v_solve;
       {
-        const RD one = RD(1);
-        const RD dL = ((one + RD(qbll)) + RD(qbrl)) + RD(qbsl);
-        const RD dI = ((one + RD(fsink_i)) + RD(qbii)) + RD(qbsi);
-        const RD dR = one + RD(fsink_r);
-        const RD dS = one + RD(fsink_s);
-        const RD xL = RD(ql) + ((RD(dl) - RD(il)) - RD(vl));
-        const RD xI = RD(qi) + (((RD(il) + RD(di)) - RD(ri)) - RD(vi));
-        const RD xR = RD(qr) + (((RD(ri) + RD(dr)) + RD(rs)) - RD(vr));
-        const RD xS = RD(qs) + ((-RD(rs) + RD(ds)) - RD(vs));
-        const RD xV = RD(qv) + (((RD(vl) + RD(vi)) + RD(vr)) + RD(vs));
-        const RD lRL = m_div(-RD(qbrl), dL), lSL = m_div(-RD(qbsl), dL), lSI = m_div(-RD(qbsi), dI);
-        const RD yR = xR - lRL * xL;
-        const RD yS = (xS - lSL * xL) - lSI * xI;
-        qxn_l = R(m_div(xL, dL));
-        qxn_i = R(m_div(xI, dI));
-        qxn_r = R(m_div(yR, dR));
-        qxn_s = R(m_div(yS, dS));
-        qxn_v_solve = R(xV);  // row V of ZQLHS is the identity
+        const R one = R(1);
+        const R dL = ((one + qbll) + qbrl) + qbsl;
+        const R dI = ((one + fsink_i) + qbii) + qbsi;
+        const R dR = one + fsink_r;
+        const R dS = one + fsink_s;
+        const R xL = R(RD(ql) + ((dl - il) - vl));
+        const R xI = R(RD(qi) + (((il + di) - ri) - vi));
+        const R xR = R(RD(qr) + (((ri + dr) + rs) - vr));
+        const R xS = R(RD(qs) + ((-rs + ds) - vs));
+        const R xV = R(RD(qv) + (((vl + vi) + vr) + vs));
+        const R lRL = m_div(-qbrl, dL), lSL = m_div(-qbsl, dL), lSI = m_div(-qbsi, dI);
+        const R yR = xR - lRL * xL;
+        const R yS = (xS - lSL * xL) - lSI * xI;
+        qxn_l = m_div(xL, dL);
+        qxn_i = m_div(xI, dI);
+        qxn_r = m_div(yR, dR);
+        qxn_s = m_div(yS, dS);
+        qxn_v_solve = xV;  // row V of ZQLHS is the identity
       }
       R qxn_v = qxn_v_solve;
       // clip small values (:3633-3640)
@@ -836,14 +838,14 @@ cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p, const KDims d,
         ta = zda * zqtmst;
       }
 
-      // 6.4 CLOUD_SUPERSATCHECK on the updated state, FP64 islands in RD (:3765-3777)
+      // 6.4 CLOUD_SUPERSATCHECK on the updated state (:3765-3777), in R
       {
-        const RD zq_upd = RD(pq) + RD(ptsphy) * (RD(tq) + RD(tcml_q));
-        const RD zt_upd = RD(pt) + RD(ptsphy) * (RD(tt) + RD(tcml_t));
-        const RD za_upd = RD(pa) + RD(ptsphy) * (RD(ta) + RD(tcml_a));
+        const R zq_upd = pq + ptsphy * (tq + tcml_q);
+        const R zt_upd = pt + ptsphy * (tt + tcml_t);
+        const R za_upd = pa + ptsphy * (ta + tcml_a);
         const R pp = ssc_off ? LD(p.pap, i2 + (uint32_t)ssc_off * E, jo) : pap;
-        RD t_adj, q_adj, a_adj, l_adj, i_adj;
-        supersatcheck<RD, KCD>(cd, zt_upd, zq_upd, za_upd, RD(pp), t_adj, q_adj, a_adj, l_adj, i_adj);
+        R t_adj, q_adj, a_adj, l_adj, i_adj;
+        supersatcheck<R, KC>(c, zt_upd, zq_upd, za_upd, pp, t_adj, q_adj, a_adj, l_adj, i_adj);
         tq = tq + R(q_adj) * zqtmst;
         tt = tt + R(t_adj) * zqtmst;
         ta = ta + R(a_adj) * zqtmst;
@@ -877,7 +879,7 @@ cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p, const KDims d,
           const R a_end = pa + (zanewm1 - zaorig);
           if (a_end > R(0.001)) fsd_anewp = m_max(m_max(a_end, zcovptot), R(0.1));
           // detrained share of the condensate (:4112-4121), from PLUDE as rescaled above
-          if (ql0 + qi0 + plude > rlmin) fsd_rat = m_max(R(0), m_min(R(1), m_div(plude, plude + ql0 + qi0)));
+          if (ql0 + qi0 + plude > rlmin) fsd_rat = m_max(R(0), m_min(R(1), m_share(plude, plude + ql0 + qi0)));
         }
       }
       // section 7 (:3826-4101), only when a budget writes PEXTRA: otherwise the budget inputs may be NULL (a run
@@ -1053,7 +1055,6 @@ bool is_default_config(const cloudsc_params_t& q) {
       CLOUDSC_SAME(rcl_const8s) CLOUDSC_SAME(rmfadvw) CLOUDSC_SAME(rmfadvwdd) (q.lmfdsnow == lmfdsnow);
 #undef CLOUDSC_SAME
 }
-
 template <typename R>
 KPtrs<R> offset_ptrs(const KPtrs<R>& p, int64_t b0, int64_t nproma, int64_t klev) {
   const int64_t s1 = nproma, s2 = nproma * klev, sh = nproma * (klev + 1);

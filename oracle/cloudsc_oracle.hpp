@@ -20,16 +20,19 @@
 // driver rejects configurations that would need them.
 //
 // Template parameters: R = JPRB (storage and bulk arithmetic), RD = type of
-// the "FP64 islands": CLOUD_SUPERSATCHECK, the 5x5 solve, and the water
-// budget of a level that feeds the solve -- the first-guess amounts ZQXFG,
-// the explicit source/sink matrix ZSOLQA, the sink amounts limited by ZQXFG
-// (MIN(rate, ZQXFG)) and the sink truncation.  <double,double> is the ifs_dp
-// build and <float,float> the ifs_sp build (RD = R: the reference's
-// arithmetic unchanged); <float,double> is the mixed-precision variant.  In
-// the mixed variant a species that a limited sink removes completely leaves a
-// residue of FP64 size in the solve's right-hand side, which the reference's
-// ZEPSEC = 1E-14 clip removes as in the FP64 build; with FP32 sums the residue
-// is ~1E-13 and survives (e.g. fully evaporated rain keeps ZCOVPTOT alive below).
+// the water budget of a level that feeds the implicit solve -- the first-guess
+// amounts ZQXFG, the explicit source/sink matrix ZSOLQA, the sink amounts
+// limited by ZQXFG (MIN(rate, ZQXFG)), the sink truncation and the solve's
+// right-hand side.  <double,double> is the ifs_dp build and <float,float> the
+// ifs_sp build (RD = R: the reference's arithmetic unchanged); <float,double>
+// is the mixed-precision variant.  In the mixed variant a species that a
+// limited sink removes completely leaves a residue of FP64 size in the
+// right-hand side, which the reference's ZEPSEC = 1E-14 clip removes as in the
+// FP64 build; with FP32 sums the residue is ~1E-13 and survives (fully
+// evaporated rain, for instance, keeps ZCOVPTOT alive below).  The elimination
+// and CLOUD_SUPERSATCHECK run in R: in FP64 they change no output by more
+// than FP32 rounding.  The stand-alone CLOUD_SUPERSATCHECK entry
+// (supersatcheck<R, RD>) keeps RD.
 //
 // Notational choices (documented, not reference behaviour): a real exponent
 // of 2 or 3 (e.g. ZTP1**3._JPRB) is evaluated as a product, a real exponent
@@ -232,24 +235,25 @@ void lu5_solve(RD lhs[NCLV][NCLV], RD x[NCLV]) {
 template <typename R, typename RD>
 void implicit_solve(const RD qa[NCLV][NCLV], const R qb[NCLV][NCLV], const R fallsink[NCLV], const R qx[NCLV],
                     R qxn[NCLV]) {
-  RD lhs[NCLV][NCLV], x[NCLV];
+  // the right-hand side (the level's water budget) is summed in RD, the elimination runs in R
+  R lhs[NCLV][NCLV], x[NCLV];
   for (int jm = 0; jm < NCLV; ++jm)
     for (int jn = 0; jn < NCLV; ++jn) {
       if (jn == jm) {
-        RD d = RD(1) + RD(fallsink[jm]);
-        for (int jo = 0; jo < NCLV; ++jo) d = d + RD(qb[jo][jn]);
+        R d = R(1) + fallsink[jm];
+        for (int jo = 0; jo < NCLV; ++jo) d = d + qb[jo][jn];
         lhs[jn][jm] = d;
       } else {
-        lhs[jn][jm] = -RD(qb[jn][jm]);
+        lhs[jn][jm] = -qb[jn][jm];
       }
     }
   for (int jm = 0; jm < NCLV; ++jm) {
     RD zexplicit = RD(0);
-    for (int jn = 0; jn < NCLV; ++jn) zexplicit = zexplicit + RD(qa[jm][jn]);
-    x[jm] = RD(qx[jm]) + zexplicit;
+    for (int jn = 0; jn < NCLV; ++jn) zexplicit = zexplicit + qa[jm][jn];
+    x[jm] = R(RD(qx[jm]) + zexplicit);
   }
-  lu5_solve<RD>(lhs, x);
-  for (int m = 0; m < NCLV; ++m) qxn[m] = R(x[m]);
+  lu5_solve<R>(lhs, x);
+  for (int m = 0; m < NCLV; ++m) qxn[m] = x[m];
 }
 
 // Inputs and outputs of the diagnostics for one call, Fortran layout: the budgets (LCLDBUD*,
@@ -478,7 +482,7 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
   std::vector<R> zliqcld(KL), zicecld(KL), zmf(KL), zdqs(KL), zldifdt(KL), zraincld(KL), zraincldm1(KL);
   std::vector<R> zsnowcld(KL), zsnowcldm1(KL), zmeltmax(KL), zfrzmax(KL), zpsdep(KL), zlamr(KL), zn0r(KL);
   std::vector<R> zlams(KL), zn0s(KL), zlami(KL), zn0i(KL), zpsacr(KL), zpiacr(KL);
-  std::vector<RD> zq_upd(KL), zt_upd(KL), za_upd(KL), zt_adj(KL), zq_adj(KL), za_adj(KL), zl_adj(KL), zi_adj(KL);
+  std::vector<R> zq_upd(KL), zt_upd(KL), za_upd(KL), zt_adj(KL), zq_adj(KL), za_adj(KL), zl_adj(KL), zi_adj(KL);
   auto QA = [&](int jl, int m, int n) -> RD& { return zsolqa[((size_t)n * NCLV + m) * KL + (jl - 1)]; };
   auto QB = [&](int jl, int m, int n) -> R& { return zsolqb[((size_t)n * NCLV + m) * KL + (jl - 1)]; };
   auto V = [&](auto& v, int jl, int m) -> auto& { return v[(size_t)m * KL + (jl - 1)]; };
@@ -1180,14 +1184,14 @@ void cloudsc(const cloudsc_params_t& p, const Args<R>& a) {
       const size_t k = i2(jl, jk);
       a.tloc_q[k] = a.tloc_q[k] + (V(zqxn, jl, QV) - zqx[i3(jl, jk, QV + 1)]) * ZQTMST;
       a.tloc_a[k] = a.tloc_a[k] + zda[i] * ZQTMST;
-      zq_upd[i] = RD(a.pq[k]) + RD(ptsphy) * (RD(a.tloc_q[k]) + RD(a.tcml_q[k]));
-      zt_upd[i] = RD(a.pt[k]) + RD(ptsphy) * (RD(a.tloc_t[k]) + RD(a.tcml_t[k]));
-      za_upd[i] = RD(a.pa[k]) + RD(ptsphy) * (RD(a.tloc_a[k]) + RD(a.tcml_a[k]));
+      zq_upd[i] = a.pq[k] + ptsphy * (a.tloc_q[k] + a.tcml_q[k]);
+      zt_upd[i] = a.pt[k] + ptsphy * (a.tloc_t[k] + a.tcml_t[k]);
+      za_upd[i] = a.pa[k] + ptsphy * (a.tloc_a[k] + a.tcml_a[k]);
     }
     // 6.4 CLOUD_SUPERSATCHECK (:3772-3777).  The sections PAP(KIDIA:KFDIA,JK) and
     // PAPH(KIDIA:KFDIA,KLEV+1) are indexed from KIDIA by the callee, so column JL
     // reads the pressures of column KIDIA+JL-1 (identical when KIDIA = 1).
-    supersatcheck<R, RD>(p, kidia, kfdia, zt_upd.data(), zq_upd.data(), za_upd.data(), a.pap + i2(kidia, jk),
+    supersatcheck<R, R>(p, kidia, kfdia, zt_upd.data(), zq_upd.data(), za_upd.data(), a.pap + i2(kidia, jk),
                          a.paph + i2(kidia, klev + 1), zt_adj.data(), zq_adj.data(), za_adj.data(), zl_adj.data(),
                          zi_adj.data());
     for (int jl = kidia; jl <= kfdia; ++jl) {  // (:3779-3809)

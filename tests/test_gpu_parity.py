@@ -11,11 +11,13 @@ the FP32 oracle itself is ~1e-2 (relative RMS) away from FP64.  The criterion
 is that the GPU is as accurate as the FP32 oracle: its relative RMS against
 the FP64 oracle must not exceed twice the FP32 oracle's, field by field.
 
-mixed (FP32 storage and bulk arithmetic; FP64 in CLOUD_SUPERSATCHECK, the 5x5
-solve and the level's water budget ZQXFG/ZSOLQA with the limited sinks and the
-sink truncation): the target tolerance of 1e-4 relative RMS against the FP64
-oracle on identical (FP32-representable) inputs, for every output except the
-cloud-cover tendency.  The cloud-cover tendency jumps by up to 1/dt where
+mixed (FP32 storage and arithmetic, on the same hardware FP32 exp/pow, square
+root and reciprocal-based division as sp; FP64 for the level's water budget --
+ZQXFG, ZSOLQA, the sinks limited by the available amount, the sink truncation
+and the implicit solve's right-hand side): the
+target tolerance of 1e-4 relative RMS against the FP64 oracle on identical
+(FP32-representable) inputs, for every output except the cloud-cover
+tendency.  The cloud-cover tendency jumps by up to 1/dt where
 CLOUD_SUPERSATCHECK compares an updated humidity with its limit and the two
 agree to a few 1e-7 (after a sink limited to reach saturation); FP32 state
 decides ~1e-5 of the points differently from FP64.  It is checked as: at most
