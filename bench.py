@@ -1,18 +1,21 @@
 """Benchmark of the plain FP32 CLOUDSC kernel on one MI355X.
 
-Workload: plain CLOUDSC in FP32, NLEV = 137, 655,360 columns (a TCo399-sized
-grid) in IFS blocked layout, seeded synthetic atmospheres generated on the
-device.  One step = one launch of cloudsc_blocked_sp_c over all columns,
-preceded by the device copy that restores PLUDE (CLOUDSC rescales it in place,
-so every step starts from the same inputs).  Inputs are resident in HBM
-before the timed region starts.
+Workload: plain CLOUDSC with FP32 fields, NLEV = 137, 655,360 columns (a
+TCo399-sized grid) in IFS blocked layout, seeded synthetic atmospheres
+generated on the device.  One step = one launch of cloudsc_blocked_mixed_c
+(FP32 state and arithmetic, FP64 water budget: within 1e-5 of the FP64 build;
+--variant sp times cloudsc_blocked_sp_c, all FP32, ~1e-2 from FP64) over all
+columns, preceded by the device copy that restores PLUDE (CLOUDSC rescales it
+in place, so every step starts from the same inputs).  Inputs are resident in
+HBM before the timed region starts.  The other variant is timed the same way
+and reported under its name.
 
 Reported besides the throughput:
   roofline      HBM-bound: algorithmic bytes per launch (4*(49*NLEV+20) per
                 column) / average kernel duration from HIP events on the
                 kernel's stream; traffic from the rocprofv3 FETCH_SIZE /
                 WRITE_SIZE summary committed under profiles/ (null if absent)
-  err_vs_fp64   per-field relative RMS of this FP32 result against the FP64
+  err_vs_fp64   per-field relative RMS of the headline result against the FP64
                 CPU oracle on the first 2048 columns
   cpu_baseline  the CPU oracle (C++/OpenMP restatement, FP32) on a sample of
                 the same workload, OpenMP over NPROMA=32 blocks, SCHEDULE(DYNAMIC,1)
@@ -67,53 +70,111 @@ def cpu_model():
     return "unknown"
 
 
-def _cpu_arrays(params, sample_cols, nproma):
+def _reblock(a, nproma_from, nproma_to):
+    """A blocked array (NGPBLKS, ..., NPROMA) re-cut into blocks of `nproma_to` columns."""
+    import numpy as np
+
+    nb = a.shape[0]
+    cols = np.moveaxis(a, -1, 1).reshape((nb * nproma_from,) + a.shape[1:-1])
+    n = cols.shape[0] // nproma_to * nproma_to
+    return np.ascontiguousarray(np.moveaxis(cols[:n].reshape((n // nproma_to, nproma_to) + a.shape[1:-1]), 1, -1))
+
+
+def cpu_child(workdir):
+    """The CPU baseline proper, in a process of its own (no torch, no HIP): OpenMP binds its threads to cores
+    (OMP_PROC_BIND=close, OMP_PLACES=cores from the parent) over the whole CPU share, which a process whose
+    OpenMP runtime already started could not do.  Reads the inputs and settings the parent wrote to `workdir`,
+    prints one JSON line."""
+    affinity = len(os.sched_getaffinity(0))  # before the OpenMP runtime binds this thread to its first core
+    import numpy as np
+
+    import oracle
+    from openifs_amd import cloudsc as C
+
+    with open(os.path.join(workdir, "settings.json")) as fh:
+        cfg = json.load(fh)
+    with open(os.path.join(workdir, "params.bin"), "rb") as fh:
+        params = C.CloudscParams.from_buffer_copy(fh.read())
+    base = {n: np.load(os.path.join(workdir, n + ".npy")) for n in C.INPUT_NAMES}
+    nthreads, nproma0 = cfg["threads"], cfg["nproma"]
+
+    def arrays(nproma, ncols):
+        arr = {n: _reblock(v, nproma0, nproma)[: ncols // nproma] for n, v in base.items()}
+        for name, kind, role in C.BLOCKED_ARRAYS:
+            if role == "o":
+                arr[name] = np.empty(C.blocked_shape(kind, nproma, KLEV, ncols // nproma), dtype=np.float32)
+        return arr
+
+    def run(arr, nproma, ncols):
+        oracle.cloudsc_blocked("sp", params, ncols, nproma, KLEV, 900.0, arr, nthreads=nthreads)
+
+    ncols = cfg["sample_cols"]
+    arr = arrays(nproma0, ncols)
+    plude0 = arr["plude"].copy()
+    run(arr, nproma0, ncols)  # warm-up
+    times = []
+    while (sum(times) < cfg["min_seconds"] or len(times) < 5) and len(times) < 200:
+        arr["plude"][...] = plude0
+        t0 = time.perf_counter()
+        run(arr, nproma0, ncols)
+        times.append(time.perf_counter() - t0)
+    per_nproma = {nproma0: ncols / sorted(times)[len(times) // 2]}
+    for npr in cfg["sweep_npromas"]:
+        n = cfg["sweep_cols"] // npr * npr
+        a = arrays(npr, n)
+        p0 = a["plude"].copy()
+        run(a, npr, n)
+        ts = []
+        for _ in range(3):
+            a["plude"][...] = p0
+            t0 = time.perf_counter()
+            run(a, npr, n)
+            ts.append(time.perf_counter() - t0)
+        per_nproma[npr] = n / sorted(ts)[1]
+    print(json.dumps({"per_nproma": {str(k): v for k, v in per_nproma.items()}, "seconds": sum(times),
+                      "passes": len(times), "affinity": affinity}))
+
+
+def cpu_baseline(params, sample_cols, nthreads, min_seconds, npromas=(8, 16, 64, 128), sweep_cols=16384):
+    """FP32 CPU oracle over `sample_cols` columns of the workload in NPROMA = 32 blocks, in a child process with
+    OMP_PROC_BIND=close, OMP_PLACES=cores and `nthreads` threads: one warm-up pass, then passes (PLUDE restored
+    before each) until `min_seconds` have elapsed and at least 5 passes ran; then `sweep_cols` columns per other
+    NPROMA (median of 3).  Returns (columns/s of the median NPROMA-32 pass, seconds, passes, {nproma: columns/s},
+    CPUs the child could run on)."""
+    import shutil
+    import subprocess
+    import tempfile
+
     import numpy as np
     import torch
 
     from openifs_amd import cloudsc as C
 
+    nproma = 32
     f = C.BlockedFields(sample_cols, nproma, KLEV, "sp", "cuda", names=set(C.INPUT_NAMES))
     C.synth_device(f, params, 1234)
-    arr = {n: f[n].cpu().numpy() for n in C.INPUT_NAMES}
-    del f
-    torch.cuda.empty_cache()
-    for name, kind, role in C.BLOCKED_ARRAYS:
-        if role == "o":
-            arr[name] = np.empty(C.blocked_shape(kind, nproma, KLEV, (sample_cols + nproma - 1) // nproma),
-                                 dtype=np.float32)
-    return arr
-
-
-def cpu_baseline(params, sample_cols, nthreads, min_seconds, npromas=(8, 16, 64, 128), sweep_cols=16384):
-    """FP32 CPU oracle over `sample_cols` columns of the workload in NPROMA = 32 blocks: one warm-up pass, then
-    passes (PLUDE restored before each) until `min_seconds` have elapsed and at least 5 passes ran; then one pass of
-    `sweep_cols` columns per other NPROMA.  Returns (columns/s of the median NPROMA-32 pass, seconds, passes,
-    {nproma: columns/s})."""
-    import oracle
-
-    nproma = 32
-    arr = _cpu_arrays(params, sample_cols, nproma)
-    plude0 = arr["plude"].copy()
-    small = 4096
-    warm = {k: (v[: small // nproma].copy()) for k, v in arr.items()}
-    oracle.cloudsc_blocked("sp", params, small, nproma, KLEV, 900.0, warm, nthreads=nthreads)
-    times = []
-    while (sum(times) < min_seconds or len(times) < 5) and len(times) < 200:
-        arr["plude"][...] = plude0
-        t0 = time.perf_counter()
-        oracle.cloudsc_blocked("sp", params, sample_cols, nproma, KLEV, 900.0, arr, nthreads=nthreads)
-        times.append(time.perf_counter() - t0)
-    dt, passes = sum(times), len(times)
-    per_nproma = {nproma: sample_cols / sorted(times)[passes // 2]}
-    for npr in npromas:
-        a = _cpu_arrays(params, sweep_cols, npr)
-        oracle.cloudsc_blocked("sp", params, small, npr, KLEV, 900.0,
-                               {k: v[: max(1, small // npr)].copy() for k, v in a.items()}, nthreads=nthreads)
-        t0 = time.perf_counter()
-        oracle.cloudsc_blocked("sp", params, sweep_cols, npr, KLEV, 900.0, a, nthreads=nthreads)
-        per_nproma[npr] = sweep_cols / (time.perf_counter() - t0)
-    return per_nproma[nproma], dt, passes, per_nproma
+    workdir = tempfile.mkdtemp(prefix="cloudsc_cpu_", dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    try:
+        for n in C.INPUT_NAMES:
+            np.save(os.path.join(workdir, n + ".npy"), f[n].cpu().numpy())
+        del f
+        torch.cuda.empty_cache()
+        with open(os.path.join(workdir, "params.bin"), "wb") as fh:
+            fh.write(bytes(params))
+        with open(os.path.join(workdir, "settings.json"), "w") as fh:
+            json.dump({"threads": nthreads, "nproma": nproma, "sample_cols": sample_cols,
+                       "min_seconds": min_seconds, "sweep_npromas": list(npromas),
+                       "sweep_cols": min(sweep_cols, sample_cols)}, fh)
+        env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores", OMP_NUM_THREADS=str(nthreads))
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", workdir], env=env,
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=1200)
+        if r.returncode != 0:
+            raise RuntimeError(f"CPU baseline child failed ({r.returncode}): {r.stderr[-2000:]}")
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+    finally:
+        shutil.rmtree(workdir, ignore_errors=True)
+    per_nproma = {int(k): v for k, v in res["per_nproma"].items()}
+    return per_nproma[nproma], res["seconds"], res["passes"], per_nproma, res["affinity"]
 
 
 def mlp_flops_per_column(din, dout):
@@ -530,6 +591,72 @@ def cloudsc2_bench(params, nproma, steps, cpu_threads, cpu_seconds):
     return res
 
 
+def plain_leg(C, params, variant, nproma, steps, warmup, cpu_threads, dev):
+    """Plain CLOUDSC in `variant` (sp or mixed) on the C2 grid: per-field error of the first 2048 columns against
+    the FP64 CPU oracle, then `steps` timed steps (PLUDE restore + one blocked launch), kernel time from HIP events
+    on the launch stream."""
+    import numpy as np
+    import torch
+
+    import oracle
+
+    f = C.BlockedFields(NGPTOT, nproma, KLEV, variant, dev)
+    C.synth_device(f, params, 20261202)
+    plude0 = f["plude"].clone()
+    nb = 2048 // nproma
+    inp = {n: f[n][:nb].cpu().numpy() for n in C.INPUT_NAMES}
+    C.cloudsc_blocked(f, params, 900.0)
+    torch.cuda.synchronize()
+    out = {n: f[n][:nb].cpu().numpy() for n in C.OUTPUT_NAMES}
+    ref = {k: (v.astype(np.float64) if v.dtype.kind == "f" else v.copy()) for k, v in inp.items()}
+    for name, kind, role in C.BLOCKED_ARRAYS:
+        if role == "o":
+            ref[name] = np.full(C.blocked_shape(kind, nproma, KLEV, nb), np.nan)
+    oracle.cloudsc_blocked("dp", params, nb * nproma, nproma, KLEV, 900.0, ref, nthreads=cpu_threads)
+
+    def rr(a, r):
+        a = a.astype(np.float64)
+        m = ~np.isnan(r)
+        return float(np.sqrt(np.sum((a[m] - r[m]) ** 2) / np.sum(r[m] ** 2)))
+
+    err = {n: rr(out[n], ref[n]) for n in ("ptendency_loc_t", "ptendency_loc_q", "ptendency_loc_a", "pfplsl",
+                                           "pfplsn", "pcovptot")}
+    for s, nm in enumerate(("l", "i", "r", "s")):
+        err[f"ptendency_loc_cld_{nm}"] = rr(out["ptendency_loc_cld"][:, s], ref["ptendency_loc_cld"][:, s])
+    sp = out["pfplsl"][:, -1].astype(np.float64) + out["pfplsn"][:, -1]
+    sr = ref["pfplsl"][:, -1] + ref["pfplsn"][:, -1]
+    err["surface_precip"] = float(np.sqrt(np.sum((sp - sr) ** 2) / np.sum(sr ** 2)))
+
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        f["plude"].copy_(plude0, non_blocking=True)
+        C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        f["plude"].copy_(plude0, non_blocking=True)
+        ev[i][0].record(stream)
+        C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    launch = None
+    try:
+        from openifs_amd import autotune
+
+        launch = dict(zip(("block", "waves_per_simd", "klev_compiled"), autotune.get_launch(variant, KLEV)))
+    except Exception:  # noqa: BLE001 - informational
+        pass
+    del f, plude0
+    torch.cuda.empty_cache()
+    ms = wall * 1e3 / steps
+    gbs = bytes_per_column(KLEV) * NGPTOT / (kern_ms / 1e3) / 1e9
+    return {"variant": variant, "ms_per_step": ms, "kernel_ms": kern_ms, "columns_per_s": NGPTOT / (ms / 1e3),
+            "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS, "launch": launch, "err_vs_fp64": err}
+
+
 def host_cpus():
     """CPUs this process may run on."""
     try:
@@ -542,20 +669,25 @@ def host_cpus():
 CPU_SHARE = 16
 
 
+HOST_CPUS = host_cpus()  # taken at import, before any OpenMP runtime of this process binds its threads
+
+
 def cpu_threads(args):
-    return args.cpu_threads or min(CPU_SHARE, int(os.environ.get("OMP_NUM_THREADS", CPU_SHARE)), host_cpus())
+    return args.cpu_threads or min(CPU_SHARE, int(os.environ.get("OMP_NUM_THREADS", CPU_SHARE)), HOST_CPUS)
 
 
 def main():
-    # OpenMP placement of the CPU oracle's threads, before its runtime starts: one thread per core, packed
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
+    if len(sys.argv) == 3 and sys.argv[1] == "--cpu-child":
+        cpu_child(sys.argv[2])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nproma", type=int, default=128)
-    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--variant", choices=("mixed", "sp"), default="mixed",
+                    help="headline variant: mixed (FP32, FP64 water budget; within 1e-5 of FP64) or sp (all FP32)")
+    ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -582,67 +714,15 @@ def main():
     dev = "cuda:0"
     torch.cuda.set_device(0)
     params = C.default_params(900.0, KLEV)
-    f = C.BlockedFields(NGPTOT, args.nproma, KLEV, "sp", dev)
-    C.synth_device(f, params, 20261202)
-    plude0 = f["plude"].clone()
-
-    # accuracy on a sample, against the FP64 CPU oracle, from the inputs the benchmark uses
-    import oracle
-
-    nproma = args.nproma
-    nb = 2048 // nproma
-    inp = {n: f[n][:nb].cpu().numpy() for n in C.INPUT_NAMES}
-    C.cloudsc_blocked(f, params, 900.0)
-    torch.cuda.synchronize()
-    out = {n: f[n][:nb].cpu().numpy() for n in C.OUTPUT_NAMES}
-    ref = {k: (v.astype(np.float64) if v.dtype.kind == "f" else v.copy()) for k, v in inp.items()}
-    for name, kind, role in C.BLOCKED_ARRAYS:
-        if role == "o":
-            ref[name] = np.full(C.blocked_shape(kind, nproma, KLEV, nb), np.nan)
-    oracle.cloudsc_blocked("dp", params, nb * nproma, nproma, KLEV, 900.0, ref, nthreads=args.cpu_threads)
-
-    def rr(a, r):
-        a = a.astype(np.float64)
-        m = ~np.isnan(r)
-        return float(np.sqrt(np.sum((a[m] - r[m]) ** 2) / np.sum(r[m] ** 2)))
-
-    err = {n: rr(out[n], ref[n]) for n in ("ptendency_loc_t", "ptendency_loc_q", "ptendency_loc_a", "pfplsl",
-                                           "pfplsn")}
-    for s, nm in enumerate(("l", "i", "r", "s")):
-        err[f"ptendency_loc_cld_{nm}"] = rr(out["ptendency_loc_cld"][:, s], ref["ptendency_loc_cld"][:, s])
-    sp = out["pfplsl"][:, -1].astype(np.float64) + out["pfplsn"][:, -1]
-    sr = ref["pfplsl"][:, -1] + ref["pfplsn"][:, -1]
-    err["surface_precip"] = float(np.sqrt(np.sum((sp - sr) ** 2) / np.sum(sr ** 2)))
-
-    stream = torch.cuda.current_stream()
-
-    def step():
-        f["plude"].copy_(plude0, non_blocking=True)
-        C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        f["plude"].copy_(plude0, non_blocking=True)
-        ev[i][0].record(stream)
-        C.cloudsc_blocked(f, params, 900.0, stream=stream.cuda_stream)
-        ev[i][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
-    ms_per_step = wall * 1e3 / args.steps
+    head = plain_leg(C, params, args.variant, args.nproma, args.steps, args.warmup, args.cpu_threads, dev)
+    other = "sp" if args.variant == "mixed" else "mixed"
+    second = plain_leg(C, params, other, args.nproma, max(5, min(args.steps, 20)), 2, args.cpu_threads, dev)
+    ms_per_step, kern_ms = head["ms_per_step"], head["kernel_ms"]
     value = NGPTOT / (ms_per_step / 1e3)
-
     bpc = bytes_per_column(KLEV)
-    alg_bytes = bpc * NGPTOT
-    achieved = alg_bytes / (kern_ms / 1e3) / 1e9
-    traffic = read_traffic(args.nproma)
+    achieved = bpc * NGPTOT / (kern_ms / 1e3) / 1e9
+    traffic = read_traffic(args.nproma, f"cloudsc_{args.variant}_l137_655360_traffic.md")
 
-    del f, plude0
-    torch.cuda.empty_cache()
     hybrid = None
     if not args.no_hybrid:
         hybrid = surrogate_bench(params, args.train_cols, args.hybrid_cols, args.rerun_fraction, args.gate_fraction,
@@ -665,15 +745,14 @@ def main():
     cpu = None
     if not args.no_cpu:
         nthreads = cpu_threads(args)
-        cps, cdt, passes, per_nproma = cpu_baseline(params, args.cpu_sample, nthreads, args.cpu_seconds)
+        cps, cdt, passes, per_nproma, child_cpus = cpu_baseline(params, args.cpu_sample, nthreads, args.cpu_seconds)
         best = max(per_nproma, key=per_nproma.get)
         cpu = {"value": cps, "unit": "columns/s", "cores": nthreads, "kind": "port",
                "sample": f"{args.cpu_sample} columns of the same synthetic distributions, median of {passes} passes "
                          f"({cdt:.1f} s) after one warm-up, FP32 oracle (C++/OpenMP restatement of CLOUDSC), "
                          f"NPROMA=32 blocks, SCHEDULE(DYNAMIC,1)",
-               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "host_cpus_available": host_cpus(),
-               "threads": nthreads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
-               "omp_places": os.environ.get("OMP_PLACES"),
+               "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "host_cpus_available": child_cpus,
+               "threads": nthreads, "omp_proc_bind": "close", "omp_places": "cores",
                "columns_per_s_by_nproma": {str(k): v for k, v in sorted(per_nproma.items())},
                "best_nproma": best}
 
@@ -689,15 +768,19 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
+        "precision": {"mixed": "FP32 storage and arithmetic, FP64 water budget of each level",
+                      "sp": "FP32 throughout (the ifs_sp build)"}[args.variant],
         "data": "synthetic (seeded atmospheres generated on the device; distributions in include/cloudsc_synth.h)",
-        "config": {"workload": f"plain CLOUDSC FP32, NLEV={KLEV}, {NGPTOT} columns (TCo399-sized), "
+        "config": {"workload": f"plain CLOUDSC FP32 ({args.variant}), NLEV={KLEV}, {NGPTOT} columns (TCo399-sized), "
                                f"NPROMA={args.nproma}, dt=900 s", "columns": NGPTOT, "nlev": KLEV,
-                   "nproma": args.nproma, "parallelism": "single GPU"},
+                   "nproma": args.nproma, "variant": args.variant, "launch": head["launch"],
+                   "parallelism": "single GPU"},
         "kernel_ms": kern_ms,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_column": bpc},
-        "err_vs_fp64": err,
+        "err_vs_fp64": head["err_vs_fp64"],
+        other: second,
         "hybrid": hybrid,
         "satadj": satadj,
         "budget": budget,

@@ -145,7 +145,9 @@ int cloudsc_params_default(cloudsc_params_t* p, double ptsphy, int nflevg, const
 int cloudsc_dp_c(CLOUDSC_ARGS(double));
 /* Replaces CLOUDSC of the ifs_sp build (JPRB = FP32). */
 int cloudsc_sp_c(CLOUDSC_ARGS(float));
-/* Replaces CLOUDSC of the ifs_sp build, FP64 islands (supersaturation check, 5x5 solve). */
+/* Replaces CLOUDSC of the ifs_sp build with FP64 for the water budget of each level (the first-guess amounts,
+ * the source/sink matrix, the limited sinks, the sink truncation and the solve's right-hand side): within 1e-5
+ * relative RMS of the ifs_dp build, where the ifs_sp build is ~1e-2 away. */
 int cloudsc_mixed_c(CLOUDSC_ARGS(float));
 
 /* ---------------------------------------------------------------------------

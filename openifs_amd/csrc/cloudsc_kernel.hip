@@ -36,7 +36,7 @@ __device__ __forceinline__ void count_warnings(uint32_t* n, bool wl, bool wi) {
   }
 }
 
-// KC / KCD: constants of the working precision and of the FP64 islands, either the run-time
+// KC / KCD: constants of the working precision R and of the water-budget precision RD, either the run-time
 // structure KConst or the compile-time default configuration KConstDefault (same member names).
 // UNI: every wave lies in one block (NPROMA a multiple of 64), so block-dependent offsets are wave-uniform.
 // SAT: the fused CALLPAR sequence (callpar.F90:1253-1293): on every level at or below NCLDTOP the column first runs
