@@ -149,6 +149,15 @@ int cloudsc_sp_c(CLOUDSC_ARGS(float));
  * the source/sink matrix, the limited sinks, the sink truncation and the solve's right-hand side): within 1e-5
  * relative RMS of the ifs_dp build, where the ifs_sp build is ~1e-2 away. */
 int cloudsc_mixed_c(CLOUDSC_ARGS(float));
+/* cloudsc_mixed_c without device allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (hosts that forbid hipMalloc, tight serving loops).  The block is staged in a fixed
+ * 32 MiB device array of the library (one per GPU, held until the process ends) instead of the calling thread's
+ * workspace, and runs on `stream` (a hipStream_t; NULL = the null stream) of the current device.  A block needs
+ * 4 B x KLON x (38 KLEV + 15 (KLEV+1) + 4) plus the budget inputs and KFLDX PEXTRA fields when budgets are on;
+ * one that needs more than the array returns CLOUDSC_EDIMS (at KLEV 137 without budgets: KLON up to about 1,150).
+ * Returns after `stream` has finished the block.  All calls on one GPU share its array: calls that overlap on one
+ * GPU race (other threads must use cloudsc_mixed_c or wait for each other). */
+int cloudsc_mixed_noalloc_c(CLOUDSC_ARGS(float), void *stream);
 
 /* ---------------------------------------------------------------------------
  * Blocked, device-resident entry: one launch over all NGPBLKS blocks of the

@@ -187,9 +187,50 @@ int thread_workspace(size_t bytes, const char* who, HostWorkspace** out) {
 }
 }  // namespace cloudsc_capi
 
+// ---- device memory of the allocation-free per-block entry (cloudsc_mixed_noalloc_c) ----
+// For callers that may not allocate device memory (hosts that forbid hipMalloc, tight serving loops): the staging
+// buffer of one block lives in this fixed array instead of the per-thread workspace.  Each GPU the library runs on
+// holds its own copy until the process ends.  Every call on a GPU shares that GPU's copy and leaves its contents
+// behind (the warning counters are cleared at the start of each call): calls on one GPU that overlap race.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
 namespace {
 using cloudsc_capi::HostWorkspace;
 using cloudsc_capi::thread_workspace;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each GPU
+
+// The current device's copy of `scratch`, looked up once per device with that device current.  Host code uses only
+// this address: the array's name is a host-side stand-in, not a device address.
+int scratch_on_current_device(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_mixed_noalloc_c: hipGetDevice", e);
+  if (dev < 0 || dev >= kMaxDevices) return fail(CLOUDSC_EDIMS, "cloudsc_mixed_noalloc_c: device index above 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(CLOUDSC_EHIP, "cloudsc_mixed_noalloc_c: hipGetSymbolAddress(scratch)", e);
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return CLOUDSC_OK;
+}
+
+// Clears the two warning counters: a kernel rather than hipMemsetAsync, which stays correct in captured graphs.
+__global__ void zero_counters(uint32_t* n) {
+  if (threadIdx.x < 2) n[threadIdx.x] = 0;
+}
+
+// Where per_block stages the block: the calling thread's workspace (nullptr) or the fixed array on `stream`.
+struct NoAlloc {
+  hipStream_t stream;
+};
 
 template <typename R, typename RD>
 int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int klev, R ptsphy, const R* pt,
@@ -200,7 +241,8 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
               R* prainfrac_toprfz, R* pfsqlf, R* pfsqif, R* pfcqnng, R* pfcqlng, R* pfsqrf, R* pfsqsf, R* pfcqrng,
               R* pfcqsng, R* pfsqltur, R* pfsqitur, R* pfplsl, R* pfplsn, R* pfhpsl, R* pfhpsn, const R* pvfa,
               const R* pdyna, const R* pdynl, const R* pdyni, const R* pdyns, const R* phrsw, const R* phrlw,
-              const int32_t* kpbltype, const int32_t* kctop, R* pextra, int kfldx, const R* pgaw, R* pfsd) {
+              const int32_t* kpbltype, const int32_t* kctop, R* pextra, int kfldx, const R* pgaw, R* pfsd,
+              const NoAlloc* na = nullptr) {
   int rc = check_config(q, true);
   if (rc) return rc;
   if (klon < 1 || klev < 2 || kidia < 1 || kfdia < kidia || kfdia > klon || kidia + kfdia - 1 > klon)
@@ -273,15 +315,31 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
     total += (arrs[i].n * arrs[i].esz + 255) & ~(size_t)255;
   }
   const size_t warn_off = total;  // two counters of the negative-condensate warnings
-  HostWorkspace* w = nullptr;
-  rc = thread_workspace(total + 256, "cloudsc", &w);
-  if (rc) return rc;
-  char* base = (char*)w->dev;
+  char* base = nullptr;
+  hipStream_t stream = nullptr;
+  if (na) {
+    if (total + 256 > kScratchBytes)
+      return fail(CLOUDSC_EDIMS, "cloudsc_mixed_noalloc_c: the block needs more than the 32 MiB fixed device array "
+                                 "(KLON x KLEV too large, or too many PEXTRA fields)");
+    void* fixed = nullptr;
+    rc = scratch_on_current_device(&fixed);
+    if (rc) return rc;
+    base = (char*)fixed;
+    stream = na->stream;
+  } else {
+    HostWorkspace* w = nullptr;
+    rc = thread_workspace(total + 256, "cloudsc", &w);
+    if (rc) return rc;
+    base = (char*)w->dev;
+    stream = w->stream;
+  }
   uint32_t* d_warn = (uint32_t*)(base + warn_off);
-  hipError_t e = hipMemsetAsync(d_warn, 0, 2 * sizeof(uint32_t), w->stream);
+  (void)hipGetLastError();  // a stale error of an earlier call is not this launch's
+  zero_counters<<<1, 64, 0, stream>>>(d_warn);
+  hipError_t e = hipGetLastError();
   for (int i = 0; i < narr && e == hipSuccess; ++i)
     if (arrs[i].n)
-      e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * arrs[i].esz, hipMemcpyHostToDevice, w->stream);
+      e = hipMemcpyAsync(base + off[i], arrs[i].host, arrs[i].n * arrs[i].esz, hipMemcpyHostToDevice, stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: copy to device", e);
   auto D = [&](int i) { return (R*)(base + off[i]); };
   KPtrs<R> p;
@@ -310,15 +368,15 @@ int per_block(const cloudsc_params_t* q, int kidia, int kfdia, int klon, int kle
   bp.pvfl = p.pvfl; bp.pvfi = p.pvfi;
   bp.reisthsc = R(q->reisthsc); bp.ralfdcp = R(q->ralfdcp);
   bp.ra = R(q->ra); bp.rpi = R(q->rpi); bp.rcloud_frac_std = R(q->rcloud_frac_std);
-  e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, kidia - 1, w->stream, nullptr,
+  e = cloudsc_gpu::launch_cloudsc<R, RD>(*q, (double)ptsphy, p, d, kidia - 1, stream, nullptr,
                                          (budgets || fsd || inh) ? &bp : nullptr);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: kernel launch", e);
   for (int i = 0; i < narr && e == hipSuccess; ++i)
     if (arrs[i].host_out && arrs[i].n)
-      e = hipMemcpyAsync(arrs[i].host_out, base + off[i], arrs[i].n * arrs[i].esz, hipMemcpyDeviceToHost, w->stream);
+      e = hipMemcpyAsync(arrs[i].host_out, base + off[i], arrs[i].n * arrs[i].esz, hipMemcpyDeviceToHost, stream);
   uint32_t warn[2] = {0, 0};
-  if (e == hipSuccess) e = hipMemcpyAsync(warn, d_warn, sizeof(warn), hipMemcpyDeviceToHost, w->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(warn, d_warn, sizeof(warn), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc: copy to host", e);
   cloudsc_capi::g_last_warn[0] = warn[0];
   cloudsc_capi::g_last_warn[1] = warn[1];
@@ -428,6 +486,19 @@ int cloudsc_blocked_diag_mixed_c(CLOUDSC_BLOCKED_DIAG_ARGS(float)) { DIAG_CALL(f
 int cloudsc_dp_c(CLOUDSC_ARGS(double)) { PER_BLOCK_CALL(double, double); }
 int cloudsc_sp_c(CLOUDSC_ARGS(float)) { PER_BLOCK_CALL(float, float); }
 int cloudsc_mixed_c(CLOUDSC_ARGS(float)) { PER_BLOCK_CALL(float, double); }
+
+int cloudsc_mixed_noalloc_c(CLOUDSC_ARGS(float), void* stream) {
+  (void)lddiag, (void)pdynr, (void)pvervel, (void)pgeom, (void)pgp2dspp, (void)ldslphy, (void)plcrit_aer,
+      (void)picrit_aer, (void)pre_ice, (void)pccn, (void)pnice;
+  const NoAlloc na{(hipStream_t)stream};
+  return per_block<float, double>(ydparams, kidia, kfdia, klon, klev, ptsphy, pt, pq, ptendency_cml_t,
+                                  ptendency_cml_q, ptendency_cml_a, ptendency_cml_cld, ptendency_loc_t,
+                                  ptendency_loc_q, ptendency_loc_a, ptendency_loc_cld, pvfl, pvfi, pap, paph, plsm,
+                                  ldcum, ktype, peis, plu, plude, pludeli, psnde, pmfu, pmfd, pa, pclv, pcovptot,
+                                  prainfrac_toprfz, pfsqlf, pfsqif, pfcqnng, pfcqlng, pfsqrf, pfsqsf, pfcqrng,
+                                  pfcqsng, pfsqltur, pfsqitur, pfplsl, pfplsn, pfhpsl, pfhpsn, pvfa, pdyna, pdynl,
+                                  pdyni, pdyns, phrsw, phrlw, kpbltype, kctop, pextra, kfldx, pgaw, pfsd, &na);
+}
 
 int cloudsc_blocked_dp_c(CLOUDSC_BLOCKED_ARGS(double)) { BLOCKED_CALL(double, double); }
 int cloudsc_blocked_sp_c(CLOUDSC_BLOCKED_ARGS(float)) { BLOCKED_CALL(float, float); }
