@@ -11,18 +11,10 @@
 #include "cloudsc_device.h"
 #include "cloudsc_kernel.h"
 #include "cloudsc_budget.h"
+#include "cloudsc_truncation.h"
 #include "satadj_device.h"
 
 namespace cloudsc_gpu {
-
-// Scale factor selection for the explicit-sink truncation (:3500-3536):
-// a stored off-diagonal value x = ZSOLQA(a,b) = -ZSOLQA(b,a) is scaled when
-// the row being processed is a and x < 0, or the row is b and x > 0.
-template <typename T>
-__device__ __forceinline__ T trunc_scale(T x, int jo, int a, int b, T zz) {
-  const bool hit = (jo == a && x < T(0)) || (jo == b && x > T(0));
-  return hit ? x * zz : x;
-}
 
 // The reference's negative-condensate warnings (cloudsc.F90:3794 liquid, :3802 ice) as counts: one atomic add per
 // wave and counter, by the wave's first active lane, only when some lane of the wave warns.
@@ -697,77 +689,11 @@ cloudsc_column_kernel(const KC c, const KCD cd, const KPtrs<R> p, const KDims d,
       const R zda = zanew - zaorig;
       zanewm1 = zanew;
 
-      // 5.2.1 truncation of explicit sinks (:3455-3536).  Row sums of ZSOLQA
-      // (minus sign: sinks positive), in the reference's column order L,I,R,S,V.
+      // 5.2.1 truncation of explicit sinks (:3455-3536)
       {
         const RD qxs[5] = {ql, qi, qr, qs, qv};
-        RD rsum[5];
-        rsum[0] = ((RD(0) - dl) + il) + vl;        // row L: (L,L)=dl (L,I)=-il (L,V)=-vl
-        rsum[1] = (((RD(0) - il) - di) + ri) + vi; // row I: il, di, (I,R)=-ri, (I,V)=-vi
-        rsum[2] = (((RD(0) - ri) - dr) - rs) + vr; // row R: ri, dr, rs, (R,V)=-vr
-        rsum[3] = ((RD(0) + rs) - ds) + vs;        // row S: (S,R)=-rs, ds, (S,V)=-vs
-        rsum[4] = (((RD(0) - vl) - vi) - vr) - vs; // row V: vl, vi, vr, vs
-        // ZRATIO = ZMAX/MAX(sinksum,ZMAX) is exactly 1 unless the sinks overshoot; with no
-        // overshoot anywhere every scale factor below is 1 and the step is a no-op.
-        RD zmax[5];
-        bool any_trunc = false;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          zmax[m] = m_max(qxs[m], RD(zepsec));
-          any_trunc = any_trunc || (rsum[m] > zmax[m]);
-        }
-        if (any_trunc) {
-          RD zratio[5];
-#pragma unroll
-          for (int m = 0; m < 5; ++m) zratio[m] = rsum[m] > zmax[m] ? m_div(zmax[m], rsum[m]) : RD(1);
-          // stable ascending order of zratio (insertion sort of :3479-3495)
-          int rank[5];
-#pragma unroll
-          for (int m = 0; m < 5; ++m) {
-            int r = 0;
-#pragma unroll
-            for (int n = 0; n < 5; ++n) r += (zratio[n] < zratio[m] || (n < m && zratio[n] == zratio[m])) ? 1 : 0;
-            rank[m] = r;
-          }
-#pragma unroll
-          for (int s = 0; s < 5; ++s) {
-            int jo = 0;
-#pragma unroll
-            for (int m = 0; m < 5; ++m) jo = (rank[m] == s) ? m : jo;
-            RD ssum;
-            if (jo == 0) {
-              ssum = ((RD(0) - dl) + il) + vl;
-            } else if (jo == 1) {
-              ssum = (((RD(0) - il) - di) + ri) + vi;
-            } else if (jo == 2) {
-              ssum = (((RD(0) - ri) - dr) - rs) + vr;
-            } else if (jo == 3) {
-              ssum = ((RD(0) + rs) - ds) + vs;
-            } else {
-              ssum = (((RD(0) - vl) - vi) - vr) - vs;
-            }
-            RD zmm = zmax[0];
-#pragma unroll
-            for (int m = 1; m < 5; ++m) zmm = (jo == m) ? zmax[m] : zmm;
-            // ZZRATIO = ZMM/MAX(ZSINKSUM,ZMM) < 1 only on overshoot; otherwise the scaling is by 1
-            if (ssum > zmm) {
-              const RD zz = m_div(zmm, ssum);
-              // off-diagonal pairs (a,b) with stored x = ZSOLQA(a,b)
-              vl = trunc_scale(vl, jo, 4, 0, zz);
-              vi = trunc_scale(vi, jo, 4, 1, zz);
-              vr = trunc_scale(vr, jo, 4, 2, zz);
-              vs = trunc_scale(vs, jo, 4, 3, zz);
-              il = trunc_scale(il, jo, 1, 0, zz);
-              ri = trunc_scale(ri, jo, 2, 1, zz);
-              rs = trunc_scale(rs, jo, 2, 3, zz);
-              // a negative diagonal entry is scaled twice (:3531-3532)
-              if (jo == 0 && dl < RD(0)) dl = (dl * zz) * zz;
-              if (jo == 1 && di < RD(0)) di = (di * zz) * zz;
-              if (jo == 2 && dr < RD(0)) dr = (dr * zz) * zz;
-              if (jo == 3 && ds < RD(0)) ds = (ds * zz) * zz;
-            }
-          }
-        }
+        truncate_sinks(dl, di, dr, ds, il, ri, rs, vl, vi, vr, vs, qxs, RD(zepsec),
+                       [](RD a, RD b) { return m_div(a, b); }, [](RD a, RD b) { return m_max(a, b); });
       }
 
       // 5.2.2 implicit solve (:3546-3628); ZQLHS is lower triangular here.  The right-hand side -- the level's

@@ -1,0 +1,165 @@
+"""Full-size GPU cases: the TCo1279-sized grid (6,599,680 columns, L137) for the plain kernel and the surrogate
+hybrid, and surrogate training on a 20 M-column workload.
+
+At this size (about 180 GB of blocked FP32 fields) the fields cannot be copied for comparison, so the checks are
+size-independent properties computed on the device in slices of blocks:
+  * plain, sp and mixed: every written output finite, the reference's write set, PLUDE untouched where the
+    reference leaves it, and the column water and enthalpy budgets closed to FP32 rounding;
+  * hybrid: per-column checksums of the bit patterns of all outputs, taken after the plain run, identify the
+    columns the hybrid took from the plain kernel -- they must be exactly as many as it reports, and every other
+    column must hold the network's result (finite, same write set).
+Training: 20 M fresh columns labelled by the plain kernel (batch 4,096, Adam 3e-4), then the held-out error
+against plain CLOUDSC on fresh columns, which must beat the climatological mean of every target.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from openifs_amd import cloudsc as C
+from tests.test_gpu_parity import _budgets_torch
+
+pytestmark = pytest.mark.gpu
+
+C5 = 6_599_680
+NPROMA = 128
+SLICE = 4096  # blocks per slice of the device-side checks (524,288 columns)
+
+
+class _View:
+    """Blocks b0:b1 of a BlockedFields, for the budget check."""
+
+    def __init__(self, f, b0, b1):
+        self.t = {k: v[b0:b1] for k, v in f.t.items()}
+        self.klev, self.nproma = f.klev, f.nproma
+
+    def __getitem__(self, k):
+        return self.t[k]
+
+
+def _column_checksums(f, names):
+    """Per column, for each output: the sum of the 32-bit patterns of its values over levels (int64)."""
+    out = torch.empty((len(names), f.ngpblks, f.nproma), dtype=torch.int64, device=f.device)
+    for i, n in enumerate(names):
+        x = f[n]
+        dims = tuple(range(1, x.dim() - 1))
+        for b0 in range(0, f.ngpblks, SLICE):
+            v = x[b0:b0 + SLICE].view(torch.int32).to(torch.int64)
+            out[i, b0:b0 + SLICE] = v.sum(dim=dims) if dims else v
+    return out
+
+
+def _check_written(f, params):
+    k0 = params.ncldtop - 1
+    for n in C.OUTPUT_NAMES:
+        for b0 in range(0, f.ngpblks, SLICE):
+            x = f[n][b0:b0 + SLICE]
+            if n == "ptendency_loc_cld":
+                assert torch.isfinite(x[:, :4]).all() and torch.isnan(x[:, 4]).all(), n
+            elif n == "pcovptot":
+                assert torch.isfinite(x[:, k0:]).all() and torch.isnan(x[:, :k0]).all(), n
+            else:
+                assert torch.isfinite(x).all(), n
+
+
+@pytest.mark.parametrize("prec", ["sp", "mixed"])
+def test_plain_c5_properties(prec, params137):
+    params = params137
+    print(f"plain {prec} at {C5} columns", flush=True)
+    f = C.BlockedFields(C5, NPROMA, 137, prec, "cuda", fill=float("nan"))
+    C.synth_device(f, params, 65)
+    plude_in = f["plude"].clone()
+    C.cloudsc_blocked(f, params, 900.0)
+    torch.cuda.synchronize()
+    _check_written(f, params)
+    k0 = params.ncldtop - 1
+    assert torch.equal(f["plude"][:, :k0], plude_in[:, :k0]) and torch.equal(f["plude"][:, -1], plude_in[:, -1])
+    del plude_in
+    med_w, med_h, max_w, max_h = [], [], 0.0, 0.0
+    for b0 in range(0, f.ngpblks, SLICE):
+        ew, eh = _budgets_torch(params, _View(f, b0, b0 + SLICE), k0)
+        med_w.append(float(ew.median()))
+        med_h.append(float(eh.median()))
+        max_w, max_h = max(max_w, float(ew.max())), max(max_h, float(eh.max()))
+    print(f"budget closure: water median {np.median(med_w):.2e} max {max_w:.2e}, "
+          f"enthalpy median {np.median(med_h):.2e} max {max_h:.2e}")
+    assert np.median(med_w) < 1e-5 and np.median(med_h) < 1e-5
+    assert max_w < 1e-2 and max_h < 1e-2
+    del f
+    torch.cuda.empty_cache()
+
+
+def test_hybrid_c5_takes_plain_columns_bit_for_bit(params137):
+    from openifs_amd import surrogate as S
+
+    params = params137
+    print(f"hybrid at {C5} columns", flush=True)
+    sur = S.Surrogate(137, hidden=1024, hidden_layers=3, max_batch=65536, seed=3)
+    S.train_surrogate(sur, params, 262144, chunk=262144, lr=3e-4, seed0=777)  # statistics and a first epoch
+    f = C.BlockedFields(C5, NPROMA, 137, "sp", "cuda", fill=float("nan"))
+    C.synth_device(f, params, 66)
+    plude0 = f["plude"].clone()
+    scores = sur.gate_scores(f, params)
+    thr = float(torch.quantile(scores[: 1 << 24].float().cpu(), 0.98))
+    del scores
+    names = [n for n, _, r in C.BLOCKED_ARRAYS if r == "o"] + ["plude"]
+    C.cloudsc_blocked(f, params, 900.0)
+    torch.cuda.synchronize()
+    ref = _column_checksums(f, names)
+    f["plude"].copy_(plude0)
+    del plude0
+    n_plain = sur.hybrid(f, params, rerun_fraction=0.01, rerun_seed=8, gate_threshold=thr)
+    torch.cuda.synchronize()
+    got = _column_checksums(f, names)
+    same = (got == ref).all(dim=0).reshape(-1)[:C5]
+    n_same = int(same.sum())
+    print(f"hybrid: {n_plain} plain columns reported, {n_same} columns equal to the plain run bit for bit")
+    assert 0.01 * C5 <= n_plain <= 0.04 * C5
+    assert n_same == n_plain
+    _check_written(f, params)
+    del f, ref, got
+    sur.close()
+    torch.cuda.empty_cache()
+
+
+def test_training_on_20m_columns_beats_the_mean_state(params137):
+    from openifs_amd import surrogate as S
+    from tests.helpers import columns
+
+    params = params137
+    klev = 137
+    sur = S.Surrogate(klev, hidden=1024, hidden_layers=3, max_batch=4096, seed=6)
+    print("training on 20,185,088 columns", flush=True)
+    losses = S.train_surrogate(sur, params, 77 * 262144, chunk=262144, lr=3e-4, seed0=20000)
+    assert len(losses) == 77 and all(np.isfinite(losses))
+    held = C.BlockedFields(65536, NPROMA, klev, "sp", "cuda")
+    C.synth_device(held, params, 99991)
+    plude0 = held["plude"].clone()
+    C.cloudsc_blocked(held, params, 900.0)
+    ref = {n: columns(held[n].cpu().numpy(), NPROMA) for n in ("ptendency_loc_t", "ptendency_loc_q", "pfplsl",
+                                                                 "pfplsn")}
+    held["plude"].copy_(plude0)
+    sur.predict(held, params)
+    torch.cuda.synchronize()
+    got = {n: columns(held[n].cpu().numpy(), NPROMA) for n in ref}
+    mean_only = S.Surrogate(klev, hidden=1024, hidden_layers=3, max_batch=4096, seed=6)
+    st = sur.state()
+    mean_only.load_state({k: (np.zeros_like(v) if k[0] in "wb" and k[1:].isdigit() else v) for k, v in st.items()})
+    held["plude"].copy_(plude0)
+    mean_only.predict(held, params)
+    torch.cuda.synchronize()
+    mean = {n: columns(held[n].cpu().numpy(), NPROMA) for n in ref}
+    sp = lambda d: d["pfplsl"][:, -1].astype(np.float64) + d["pfplsn"][:, -1]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    e = {"t": rel(got["ptendency_loc_t"], ref["ptendency_loc_t"]), "q": rel(got["ptendency_loc_q"],
+                                                                             ref["ptendency_loc_q"]),
+         "surface_precip": rel(sp(got), sp(ref))}
+    e_mean = {"t": rel(mean["ptendency_loc_t"], ref["ptendency_loc_t"]), "q": rel(mean["ptendency_loc_q"],
+                                                                                   ref["ptendency_loc_q"]),
+              "surface_precip": rel(sp(mean), sp(ref))}
+    print(f"held-out relative RMS against plain CLOUDSC after 20 M columns: {e}; mean state: {e_mean}; "
+          f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    for k in e:
+        assert e[k] < 0.9 * e_mean[k], k
+    sur.close()
+    mean_only.close()
