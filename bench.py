@@ -184,6 +184,10 @@ def mlp_flops_per_column(din, dout):
     return fwd, fwd + bwd
 
 
+# surrogate training recipe of the benchmark (and of the full-size training test)
+TRAIN = {"lr": 1e-3, "epochs": 2, "flux_weight": 4.0, "surface_weight": 50.0}
+
+
 def surrogate_bench(params, train_cols, hybrid_cols, rerun_fraction, gate_fraction, infer_batch):
     """Trains the surrogate (fresh seeded columns, the plain FP32 kernel as teacher) and times the hybrid
     against the plain kernel on the same `hybrid_cols` columns."""
@@ -197,10 +201,13 @@ def surrogate_bench(params, train_cols, hybrid_cols, rerun_fraction, gate_fracti
     sur = S.Surrogate(KLEV, hidden=1024, hidden_layers=3, max_batch=infer_batch, seed=7)
     fwd_flop, train_flop = mlp_flops_per_column(sur.mlp.din, sur.mlp.dout)
 
-    # training: batches of 4096 columns, Adam lr 3e-4
+    # training: batches of 4096 columns, Adam, two epochs over `train_cols` columns, cosine learning rate from 1e-3,
+    # precipitation-flux targets weighted x4 and the surface flux x200 (x4 as a flux, x50 on top)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    losses = S.train_surrogate(sur, params, train_cols, chunk=262144, lr=3e-4, seed0=5000, batch=4096, device=dev)
+    losses = S.train_surrogate(sur, params, train_cols, chunk=262144, lr=TRAIN["lr"], seed0=5000, batch=4096,
+                               device=dev, epochs=TRAIN["epochs"], cosine=True, flux_weight=TRAIN["flux_weight"],
+                               surface_weight=TRAIN["surface_weight"])
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     trained_cols = len(losses) * 262144
@@ -263,9 +270,11 @@ def surrogate_bench(params, train_cols, hybrid_cols, rerun_fraction, gate_fracti
         "surface_precip_relrms_network_only": err_net,
         "tendency_t_relrms_vs_plain": rel(f["ptendency_loc_t"], tt_ref),
         "tendency_q_relrms_vs_plain": rel(f["ptendency_loc_q"], tq_ref),
-        "training": {"columns": trained_cols, "seconds": t_train, "columns_per_s": trained_cols / t_train,
-                     "tflops": train_flop * trained_cols / t_train / 1e12, "batch": 4096, "lr": 3e-4,
-                     "final_loss": losses[-1] if losses else None, "first_loss": losses[0] if losses else None},
+        "training": {"columns": trained_cols, "distinct_columns": trained_cols // TRAIN["epochs"],
+                     "seconds": t_train, "columns_per_s": trained_cols / t_train,
+                     "tflops": train_flop * trained_cols / t_train / 1e12, "batch": 4096, **TRAIN,
+                     "schedule": "cosine to lr/50", "final_loss": losses[-1] if losses else None,
+                     "first_loss": losses[0] if losses else None},
     }
     del f, plude0, sp_ref, tq_ref, tt_ref
     sur.close()
@@ -677,6 +686,7 @@ def cpu_threads(args):
 
 
 def main():
+    global NGPTOT, KLEV
     if len(sys.argv) == 3 and sys.argv[1] == "--cpu-child":
         cpu_child(sys.argv[2])
         return
@@ -687,6 +697,10 @@ def main():
     ap.add_argument("--nproma", type=int, default=128)
     ap.add_argument("--variant", choices=("mixed", "sp"), default="mixed",
                     help="headline variant: mixed (FP32, FP64 water budget; within 1e-5 of FP64) or sp (all FP32)")
+    ap.add_argument("--no-second", action="store_true", help="do not time the other variant")
+    ap.add_argument("--columns", type=int, default=NGPTOT,
+                    help="columns of the plain workload (profiling other grid sizes; the benchmark is 655,360)")
+    ap.add_argument("--klev", type=int, default=KLEV, help="levels of the plain workload (the benchmark is 137)")
     ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -711,17 +725,19 @@ def main():
 
     from openifs_amd import cloudsc as C
 
+    NGPTOT, KLEV = args.columns, args.klev
     dev = "cuda:0"
     torch.cuda.set_device(0)
     params = C.default_params(900.0, KLEV)
     head = plain_leg(C, params, args.variant, args.nproma, args.steps, args.warmup, args.cpu_threads, dev)
     other = "sp" if args.variant == "mixed" else "mixed"
-    second = plain_leg(C, params, other, args.nproma, max(5, min(args.steps, 20)), 2, args.cpu_threads, dev)
+    second = None if args.no_second else plain_leg(C, params, other, args.nproma, max(5, min(args.steps, 20)), 2,
+                                                   args.cpu_threads, dev)
     ms_per_step, kern_ms = head["ms_per_step"], head["kernel_ms"]
     value = NGPTOT / (ms_per_step / 1e3)
     bpc = bytes_per_column(KLEV)
     achieved = bpc * NGPTOT / (kern_ms / 1e3) / 1e9
-    traffic = read_traffic(args.nproma, f"cloudsc_{args.variant}_l137_655360_traffic.md")
+    traffic = read_traffic(args.nproma, f"cloudsc_{args.variant}_l{KLEV}_{NGPTOT}_traffic.md")
 
     hybrid = None
     if not args.no_hybrid:
@@ -771,7 +787,8 @@ def main():
         "precision": {"mixed": "FP32 storage and arithmetic, FP64 water budget of each level",
                       "sp": "FP32 throughout (the ifs_sp build)"}[args.variant],
         "data": "synthetic (seeded atmospheres generated on the device; distributions in include/cloudsc_synth.h)",
-        "config": {"workload": f"plain CLOUDSC FP32 ({args.variant}), NLEV={KLEV}, {NGPTOT} columns (TCo399-sized), "
+        "config": {"workload": f"plain CLOUDSC FP32 ({args.variant}), NLEV={KLEV}, {NGPTOT} columns"
+                               f"{' (TCo399-sized)' if NGPTOT == 655360 else ''}, "
                                f"NPROMA={args.nproma}, dt=900 s", "columns": NGPTOT, "nlev": KLEV,
                    "nproma": args.nproma, "variant": args.variant, "launch": head["launch"],
                    "parallelism": "single GPU"},

@@ -8,6 +8,7 @@ train_surrogate -- the training loop: fresh seeded synthetic atmospheres, the pl
 There is no CPU fallback: every call goes to libcloudsc_hip.so and raises CloudscError on failure.
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -263,27 +264,38 @@ class Surrogate:
 
 
 def train_surrogate(sur, params, n_columns, chunk=262144, lr=3e-4, seed0=1000, nproma=128, batch=4096, log=None,
-                    device="cuda"):
-    """Trains ``sur`` on ``n_columns`` fresh synthetic columns (seeds seed0, seed0+1, ...) labelled by the plain
-    FP32 kernel, ``chunk`` columns per generated field set.  Statistics are fitted on the first chunk.
-    Returns the list of per-chunk mean training losses."""
+                    device="cuda", epochs=1, cosine=False, flux_weight=1.0, surface_weight=1.0):
+    """Trains ``sur`` on ``n_columns`` synthetic columns (seeds seed0, seed0+1, ...) labelled by the plain FP32
+    kernel, ``chunk`` columns per generated field set, ``epochs`` passes over the same seeds.  Statistics are
+    fitted on the first chunk; then the loss weights of the precipitation-flux targets are multiplied by
+    ``flux_weight`` and that of the surface flux by ``surface_weight`` as well (the surface precipitation is one
+    of the 6 x KLEV targets).  With ``cosine`` the learning rate follows a cosine from ``lr`` down to lr/50 over
+    all chunks.  Returns the list of per-chunk mean training losses."""
     import torch
 
     f = C.BlockedFields(chunk, nproma, sur.klev, "sp", device)
     plude = torch.empty_like(f["plude"])
     losses = []
-    done, seed = 0, seed0
-    while done < n_columns:
-        C.synth_device(f, params, seed)
+    nchunks = max(1, (n_columns + chunk - 1) // chunk)
+    total = nchunks * epochs
+    for i in range(total):
+        C.synth_device(f, params, seed0 + i % nchunks)
         plude.copy_(f["plude"])
         C.cloudsc_blocked(f, params, 900.0)
         f["plude"].copy_(plude)  # the features read the detrained liquid as CLOUDSC receives it
-        if done == 0 and not sur.fitted:
+        if i == 0 and not sur.fitted:
             sur.fit_norm(f, params)
-        losses.append(sur.train(f, params, lr, batch=batch))
-        done += chunk
-        seed += 1
+            if flux_weight != 1.0 or surface_weight != 1.0:
+                nrm = sur.norm()
+                k = sur.klev
+                nrm["wgt_y"][5 * k:6 * k] *= flux_weight
+                nrm["wgt_y"][6 * k - 1] *= surface_weight
+                sur.norm(nrm)
+        rate = lr
+        if cosine:
+            rate = max(lr * 0.5 * (1.0 + math.cos(math.pi * i / max(total - 1, 1))), lr / 50.0)
+        losses.append(sur.train(f, params, rate, batch=batch))
         if log:
-            log(f"columns {done}: loss {losses[-1]:.4f}")
+            log(f"columns {(i + 1) * chunk} (epoch {i // nchunks + 1}): loss {losses[-1]:.4f}, lr {rate:.2e}")
     del f, plude
     return losses

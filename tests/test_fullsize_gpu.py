@@ -8,8 +8,9 @@ size-independent properties computed on the device in slices of blocks:
   * hybrid: per-column checksums of the bit patterns of all outputs, taken after the plain run, identify the
     columns the hybrid took from the plain kernel -- they must be exactly as many as it reports, and every other
     column must hold the network's result (finite, same write set).
-Training: 20 M fresh columns labelled by the plain kernel (batch 4,096, Adam 3e-4), then the held-out error
-against plain CLOUDSC on fresh columns, which must beat the climatological mean of every target.
+Training: two epochs over 20 M columns labelled by the plain kernel (batch 4,096, Adam, cosine learning rate from
+1e-3, precipitation fluxes weighted up), then the held-out error against plain CLOUDSC on fresh columns, which
+must stay below 0.7 of the climatological mean's on the T and q tendencies and the surface precipitation.
 """
 import numpy as np
 import pytest
@@ -129,9 +130,11 @@ def test_training_on_20m_columns_beats_the_mean_state(params137):
     params = params137
     klev = 137
     sur = S.Surrogate(klev, hidden=1024, hidden_layers=3, max_batch=4096, seed=6)
-    print("training on 20,185,088 columns", flush=True)
-    losses = S.train_surrogate(sur, params, 77 * 262144, chunk=262144, lr=3e-4, seed0=20000)
-    assert len(losses) == 77 and all(np.isfinite(losses))
+    print("training: 2 epochs over 20,185,088 columns", flush=True)
+    # the benchmark's recipe: cosine learning rate from 1e-3, precipitation-flux targets weighted up
+    losses = S.train_surrogate(sur, params, 77 * 262144, chunk=262144, lr=1e-3, seed0=20000, epochs=2, cosine=True,
+                               flux_weight=4.0, surface_weight=50.0)
+    assert len(losses) == 154 and all(np.isfinite(losses))
     held = C.BlockedFields(65536, NPROMA, klev, "sp", "cuda")
     C.synth_device(held, params, 99991)
     plude0 = held["plude"].clone()
@@ -160,6 +163,6 @@ def test_training_on_20m_columns_beats_the_mean_state(params137):
     print(f"held-out relative RMS against plain CLOUDSC after 20 M columns: {e}; mean state: {e_mean}; "
           f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
     for k in e:
-        assert e[k] < 0.9 * e_mean[k], k
+        assert e[k] < 0.7 * e_mean[k], k
     sur.close()
     mean_only.close()
