@@ -152,6 +152,134 @@ __global__ __launch_bounds__(256) void gemm_tn(const bf16* __restrict__ A, int l
     }
 }
 
+// The large-product kernel: 256 x 256 block tile, K-step 64, NW waves as 2 x (NW/2), each computing 128 rows x
+// (512/NW) columns in 32x32 fragments whose accumulators live in the AGPRs.  The two LDS buffers are filled straight
+// from global memory (global_load_lds_dwordx4: no staging registers), the next tile while the current one
+// multiplies.  One such load writes 1 KB of LDS contiguously, lane l at 16*l: 8 rows of 128 B.  Chunk c (16 B) of
+// row r is stored in slot c ^ ((r >> 1) & 7) of its row: a fragment read (16 lanes = 16 consecutive rows, one
+// chunk, 256 B across the 64 banks) then finds the 8 even rows in 8 distinct slots of one 128-byte half of the banks
+// and the 8 odd rows likewise in the other half -- no bank conflict.  Each lane of a load fetches the global chunk
+// that belongs in its LDS slot.
+template <int EPI, int NW>
+__global__ __launch_bounds__(64 * NW) void gemm_tn_dl(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
+                                                       int ldb, int K, EpiArgs e) {
+  constexpr int TBM = 256, TBN = 256, TBK = 64, WN = NW / 2, WCOL = TBN / WN, FM = 4, FN = WCOL / 32;
+  constexpr int TILE = TBM * TBK;                    // bf16 per operand tile (32 KB)
+  constexpr int NLD = 32 / NW;                       // 1 KB loads per operand, thread and K-step
+  __shared__ __attribute__((aligned(1024))) bf16 lds[2][2][TILE];  // [buffer][A|B]
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int wm = w / WN, wn = w % WN;
+  // Tile mapping: the dispatcher hands consecutive workgroups to the 8 XCDs in turn, each with its own L2.  With the
+  // M-tile count a multiple of 8, workgroup L goes to XCD L % 8 and takes, in that XCD's order, all N-tiles of one
+  // M-tile before the next, so the A tile they share is fetched into one L2 only.
+  int mt = blockIdx.y, nt = blockIdx.x;
+  if ((gridDim.y & 7) == 0) {
+    const int L = blockIdx.y * gridDim.x + blockIdx.x, slot = L >> 3;
+    nt = slot % gridDim.x;
+    mt = (slot / gridDim.x) * 8 + (L & 7);
+  }
+  const int m0 = mt * TBM, n0 = nt * TBN;
+  // loads: wave w, instruction i fills row group g = NW i + w (rows 8g .. 8g+7) of each operand; g has the parity of
+  // w, so (row >> 1) & 7 = (4 (w & 1) + (lrow >> 1)) & 7
+  const int lrow = lane >> 3, lchunk = (lane & 7) ^ ((4 * (w & 1) + (lrow >> 1)) & 7);
+  const bf16* ag = A + (size_t)(m0 + 8 * w + lrow) * lda + 8 * lchunk;
+  const bf16* bg = B + (size_t)(n0 + 8 * w + lrow) * ldb + 8 * lchunk;
+  const size_t astep = (size_t)(8 * NW) * lda, bstep = (size_t)(8 * NW) * ldb;
+  auto gload = [&](int buf, int k0) {
+    typedef bf16 __attribute__((address_space(3))) lbf16;
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      __builtin_amdgcn_global_load_lds((const void*)(ag + astep * i + k0),
+                                       (lbf16*)&lds[buf][0][(NW * i + w) * 512], 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((const void*)(bg + bstep * i + k0),
+                                       (lbf16*)&lds[buf][1][(NW * i + w) * 512], 16, 0, 0);
+    }
+  };
+  f32x16 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+  const int nk = K / TBK;
+  gload(0, 0);
+  // fragment reads: lane supplies row (lane & 31) of a 32-row block and k = 8*(lane >> 5) .. +7 of a 16-wide step
+  const int fr = lane & 31, fh = lane >> 5;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    __builtin_amdgcn_s_waitcnt(0);  // this thread's loads of the current tile have landed
+    __syncthreads();                // everyone's; and everyone is done reading the other buffer
+    if (kt + 1 < nk) gload(cur ^ 1, (kt + 1) * TBK);
+    const bf16* as = lds[cur][0];
+    const bf16* bs = lds[cur][1];
+#pragma unroll
+    for (int ks = 0; ks < TBK / 16; ++ks) {
+      const int c = 2 * ks + fh;  // logical 16-byte chunk of the row
+      bf16x8 a[FM], b[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int r = wm * 128 + i * 32 + fr;
+        a[i] = *reinterpret_cast<const bf16x8*>(as + r * TBK + 8 * (c ^ ((r >> 1) & 7)));
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int r = wn * WCOL + j * 32 + fr;
+        b[j] = *reinterpret_cast<const bf16x8*>(bs + r * TBK + 8 * (c ^ ((r >> 1) & 7)));
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
+    }
+  }
+
+  // The product is formed as C^T = B A^T, so register r of lane l holds C[m][n] with m = lane & 31 of the fragment's
+  // rows and n = 8*(r>>2) + 4*(l>>5) + (r&3) of its columns: each lane owns 4 runs of 4 consecutive columns of one
+  // row, stored as one 16-byte (fp32) or 8-byte (bf16) write each.
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int row = m0 + wm * 128 + i * 32 + (lane & 31);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int col = n0 + wn * WCOL + j * 32 + 8 * q + 4 * (lane >> 5);
+        const size_t o = (size_t)row * e.ldc + col;
+        f32x4 v = {acc[i][j][4 * q], acc[i][j][4 * q + 1], acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]};
+        if (EPI == EPI_HIDDEN || EPI == EPI_HIDDEN_INF || EPI == EPI_OUT)
+          v = v + *reinterpret_cast<const f32x4*>(e.bias + col);
+        if (EPI == EPI_F32 || EPI == EPI_OUT) {
+          *reinterpret_cast<f32x4*>(e.c + o) = v;
+        } else if (EPI == EPI_HIDDEN || EPI == EPI_HIDDEN_INF) {
+          bf16x4 a;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) a[t] = (bf16)(v[t] * sigmoid(v[t]));
+          if (EPI == EPI_HIDDEN) {
+            *reinterpret_cast<f32x4*>(e.c + o) = v;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) e.actT[(size_t)(col + t) * e.ldt + row] = a[t];
+          }
+          *reinterpret_cast<bf16x4*>(e.act + o) = a;
+        } else {  // EPI_DHIDDEN
+          const f32x4 z = *reinterpret_cast<const f32x4*>(e.z + o);
+          bf16x4 g;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const float sg = sigmoid(z[t]);
+            g[t] = (bf16)(v[t] * (sg * (1.0f + z[t] * (1.0f - sg))));
+          }
+          *reinterpret_cast<bf16x4*>(e.act + o) = g;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) e.actT[(size_t)(col + t) * e.ldt + row] = g[t];
+        }
+      }
+    }
+}
+
 // Output-layer loss: L = sum_{m,n} wgt[n] (y - t)^2 / M.  dY = 2 wgt[n] (y - t) / M (bf16, both layouts);
 // loss partial sums are accumulated into *loss (fp32 atomics).
 __global__ void mse_grad(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ wgt,
@@ -252,8 +380,8 @@ hipError_t gemm(const bf16* A, int lda, const bf16* B, int ldb, int M, int N, in
   const long groups256x128 = (M % 256 == 0) ? (long)(M / 256) * (N / 128) : 0;
   const bool k64 = K % 64 == 0;
   if (groups256 >= 256 && k64) {
-    hipLaunchKernelGGL((mlp::gemm_tn<EPI, 256, 256, 64>), dim3(N / 256, M / 256), dim3(256), 0, s, A, lda, B, ldb, K,
-                       e);
+    hipLaunchKernelGGL((mlp::gemm_tn_dl<EPI, 8>), dim3(N / 256, M / 256), dim3(512), 0, s, A, lda,
+                       B, ldb, K, e);
   } else if (groups256x128 >= 256 && k64) {
     hipLaunchKernelGGL((mlp::gemm_tn<EPI, 256, 128, 64>), dim3(N / 128, M / 256), dim3(256), 0, s, A, lda, B, ldb, K,
                        e);

@@ -36,6 +36,7 @@ constexpr int TILE = 64;  // columns x matrix entries per transpose tile
 constexpr float CLIP = 10.0f;
 
 inline int pad128(int n) { return (n + 127) / 128 * 128; }
+inline int pad256(int n) { return (n + 255) / 256 * 256; }
 
 // Saturation constants (TTHF, fcttre.func.h) for the relative-humidity feature and the gate.
 struct Sat {
@@ -417,7 +418,7 @@ int cloudsc_surrogate_create(cloudsc_surrogate_t** out, int klev, int n_hidden, 
   cloudsc_surrogate* s = new cloudsc_surrogate();
   s->klev = klev;
   s->nin = surr::pad128(surr::NF * klev);
-  s->nout = surr::pad128(surr::NT * klev);
+  s->nout = surr::pad256(surr::NT * klev);  // the output product then runs on the 256 x 256-tile GEMM
   s->max_batch = max_batch;
   int rc = cloudsc_mlp_create(&s->net, s->nin, n_hidden, n_hidden_layers, s->nout, max_batch, seed);
   auto F = [&](size_t n, float** p) { return salloc(s, n * sizeof(float), (void**)p); };

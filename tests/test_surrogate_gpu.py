@@ -22,9 +22,9 @@ def _torch():
     return torch
 
 
-# tile paths: 128x128 (small), 256x128 (M=32768, N=896), 256x256 (M=16384, N=1024)
+# tile paths: 128x128 (small), 256x128 (M=32768, N=896), 256x256 with LDS filled straight from memory (M=16384, N=1024)
 @pytest.mark.parametrize("M,N,K", [(128, 128, 32), (256, 384, 1920), (512, 896, 1024), (1024, 1024, 4096),
-                                   (32768, 896, 32), (16384, 1024, 64)])
+                                   (32768, 896, 32), (16384, 1024, 64), (16384, 1024, 1920), (65536, 1024, 128)])
 def test_gemm_tn_matches_float64(M, N, K):
     torch = _torch()
     from openifs_amd import surrogate as S
