@@ -766,8 +766,11 @@ hipError_t launch(const cloudsc_params_t& q, const cloudsc2_params_t& q2, bool l
   // blocks per launch: every (NPROMA,KLEV+1) array of one launch stays below 4 GiB (32-bit byte offsets)
   const int64_t per_block = (int64_t)d.nproma * (d.klev + 1) * (int64_t)sizeof(R);
   const int64_t max_blocks = std::max<int64_t>(1, ((int64_t)1 << 32) / per_block - 1);
-  for (int64_t b0 = 0; b0 < d.ngpblks; b0 += max_blocks) {
-    const int64_t nb = std::min<int64_t>(max_blocks, d.ngpblks - b0);
+  // launches of equal size: a short last launch would leave most of the GPU idle while it runs
+  const int64_t nlaunch = (d.ngpblks + max_blocks - 1) / max_blocks;
+  const int64_t step = (d.ngpblks + nlaunch - 1) / nlaunch;
+  for (int64_t b0 = 0; b0 < d.ngpblks; b0 += step) {
+    const int64_t nb = std::min<int64_t>(step, d.ngpblks - b0);
     C2Dims dc = d;
     dc.ngpblks = (int)nb;
     dc.ngptot = (int)(d.ngptot - b0 * d.nproma);

@@ -1148,8 +1148,11 @@ hipError_t launch_cloudsc(const cloudsc_params_t& q, double ptsphy, const KPtrs<
   const int64_t nfld = bud ? std::max<int64_t>(5, bud->kfldx) : 5;
   const int64_t per_block = (int64_t)d.nproma * d.klev * nfld * (int64_t)sizeof(R);
   const int64_t max_blocks = std::max<int64_t>(1, ((int64_t)1 << 32) / per_block - 1);
-  for (int64_t b0 = 0; b0 < d.ngpblks; b0 += max_blocks) {
-    const int64_t nb = std::min<int64_t>(max_blocks, d.ngpblks - b0);
+  // launches of equal size: a short last launch would leave most of the GPU idle while it runs
+  const int64_t nlaunch = (d.ngpblks + max_blocks - 1) / max_blocks;
+  const int64_t step = (d.ngpblks + nlaunch - 1) / nlaunch;
+  for (int64_t b0 = 0; b0 < d.ngpblks; b0 += step) {
+    const int64_t nb = std::min<int64_t>(step, d.ngpblks - b0);
     KDims dc = d;
     dc.ngpblks = (int32_t)nb;
     dc.ngptot = (int32_t)(d.ngptot - b0 * d.nproma);

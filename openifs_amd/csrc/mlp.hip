@@ -281,19 +281,32 @@ __global__ __launch_bounds__(64 * NW) void gemm_tn_dl(const bf16* __restrict__ A
 }
 
 // Output-layer loss: L = sum_{m,n} wgt[n] (y - t)^2 / M.  dY = 2 wgt[n] (y - t) / M (bf16, both layouts);
-// loss partial sums are accumulated into *loss (fp32 atomics).
-__global__ void mse_grad(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ wgt,
-                         int M, int N, bf16* __restrict__ dy, bf16* __restrict__ dyT, float* __restrict__ loss) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// loss partial sums are accumulated into *loss (fp32 atomics).  A 64 x 64 tile per 256-lane group: rows of dY read
+// and written along n, the transposed copy written along m through LDS, so that every store is coalesced.
+__global__ __launch_bounds__(256) void mse_grad(const float* __restrict__ y, const float* __restrict__ t,
+                                                const float* __restrict__ wgt, int M, int N, bf16* __restrict__ dy,
+                                                bf16* __restrict__ dyT, float* __restrict__ loss) {
+  __shared__ bf16 tile[64][66];
+  const int n0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   float l = 0.0f;
-  if (idx < (size_t)M * N) {
-    const int m = (int)(idx / N), n = (int)(idx - (size_t)m * N);
-    const float d = y[idx] - t[idx];
-    const float wd = wgt[n] * d;
-    l = wd * d;
-    const bf16 g = (bf16)(2.0f * wd / (float)M);
-    dy[idx] = g;
-    dyT[(size_t)n * M + m] = g;
+  for (int r = ty; r < 64; r += 4) {  // lanes along n
+    const int m = m0 + r, n = n0 + tx;
+    bf16 g = (bf16)0.0f;
+    if (m < M && n < N) {
+      const size_t idx = (size_t)m * N + n;
+      const float d = y[idx] - t[idx];
+      const float wd = wgt[n] * d;
+      l += wd * d;
+      g = (bf16)(2.0f * wd / (float)M);
+      dy[idx] = g;
+    }
+    tile[r][tx] = g;
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {  // lanes along m
+    const int n = n0 + r, m = m0 + tx;
+    if (m < M && n < N) dyT[(size_t)n * M + m] = tile[tx][r];
   }
   // wave reduction, one atomic per wave
   for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
@@ -570,7 +583,7 @@ int cloudsc_mlp_gradients(cloudsc_mlp_t* m, const uint16_t* x, const uint16_t* x
   if (e == hipSuccess) {
     const int n = m->dout[m->nl - 1];
     hipLaunchKernelGGL(mlp::fill_f32, dim3(1), dim3(64), 0, s, m->loss, (size_t)1, 0.0f);
-    hipLaunchKernelGGL(mlp::mse_grad, dim3(blocks((size_t)M * n, 256)), dim3(256), 0, s, m->y, t, wgt, M, n, m->g0,
+    hipLaunchKernelGGL(mlp::mse_grad, dim3((n + 63) / 64, (M + 63) / 64), dim3(256), 0, s, m->y, t, wgt, M, n, m->g0,
                        m->g0T, m->loss);
     e = hipGetLastError();
   }

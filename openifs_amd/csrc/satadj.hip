@@ -153,8 +153,11 @@ hipError_t launch_satadj_c(const KC& c, const KCD& cd, const SatPtrs<R>& p, cons
   const bool vec = d.nproma % VW == 0 && aligned_to(p, VW * sizeof(R));
   const int64_t per_block = (int64_t)d.nproma * d.klev * std::max(4, d.ncs) * (int64_t)sizeof(R);
   const int64_t max_blocks = std::max<int64_t>(1, ((int64_t)1 << 32) / per_block - 1);
-  for (int64_t b0 = 0; b0 < d.ngpblks; b0 += max_blocks) {
-    const int64_t nb = std::min<int64_t>(max_blocks, d.ngpblks - b0);
+  // launches of equal size: a short last launch would leave most of the GPU idle while it runs
+  const int64_t nlaunch = (d.ngpblks + max_blocks - 1) / max_blocks;
+  const int64_t step = (d.ngpblks + nlaunch - 1) / nlaunch;
+  for (int64_t b0 = 0; b0 < d.ngpblks; b0 += step) {
+    const int64_t nb = std::min<int64_t>(step, d.ngpblks - b0);
     SatDims dc = d;
     dc.ngpblks = (int32_t)nb;
     dc.ngptot = (int32_t)(d.ngptot - b0 * d.nproma);
