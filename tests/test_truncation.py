@@ -1,12 +1,14 @@
 """The kernel's explicit-sink truncation (openifs_amd/csrc/cloudsc_truncation.h) against the oracle's
 (oracle/cloudsc_oracle.hpp truncate_sinks, which follows cloudsc.F90:3455-3536 line by line), bit for bit.
 
-The kernel keeps ZSOLQA as 4 diagonal entries and 7 antisymmetric pairs, ranks the species with one comparison
-per pair, and skips the steps of species whose row no scaling has touched.  Both are compiled here with g++ (the
-header's device attributes compile away on the host) and run on seeded cases drawn so that ties between ratios,
-two species overshooting at once, negative diagonals, and cascades -- a species that overshoots only after
-another species' scaling has cut one of its sources -- all occur many times.  The harness counts those cases
-itself and the test requires each to be present.
+The kernel keeps ZSOLQA as 4 diagonal entries and 7 antisymmetric pairs and skips the steps of species whose row
+no scaling has touched.  When no two species below ratio 1 share a pair, their steps commute and it runs them in
+species order with their first quotients (the commuting path); otherwise it ranks the species with one comparison
+per pair (the sorted path).  Both are compiled here with g++ (the header's device attributes compile away on the
+host) and run on seeded cases drawn so that ties between ratios, two species overshooting at once, negative
+diagonals, cascades -- a species that overshoots only after another species' scaling has cut one of its sources
+-- and both paths all occur many times.  The harness counts those cases itself and the test requires each to be
+present.
 """
 import os
 import shutil
@@ -28,7 +30,7 @@ HARNESS = r"""
 static const int PA[7][2] = {{1, 0}, {2, 1}, {2, 3}, {4, 0}, {4, 1}, {4, 2}, {4, 3}};
 
 template <typename T>
-struct Stats { long cases = 0, bad = 0, trunc = 0, two = 0, ties = 0, cascade = 0, negdiag = 0; };
+struct Stats { long cases = 0, bad = 0, trunc = 0, two = 0, ties = 0, cascade = 0, negdiag = 0, partners = 0; };
 
 // the reference loop once more, only to classify the case (which steps scale a species that did not overshoot)
 template <typename T>
@@ -82,13 +84,17 @@ static void run(long n, uint64_t seed, Stats<T>& st) {
       std::memcpy(tmp, qa, sizeof tmp);
       T zr[5];
       int nov = 0;
+      unsigned first = 0;
       for (int m = 0; m < 5; ++m) {
         T s = T(0);
         for (int k = 0; k < 5; ++k) s = s - qa[m][k];
         const T zmax = std::max(qx[m], zepsec);
         if (s > zmax) ++nov;
         zr[m] = zmax / std::max(s, zmax);
+        if (zr[m] < T(1)) first |= 1u << m;
       }
+      // species at ratio < 1 that share a pair: the kernel takes its sorted path, otherwise its commuting path
+      st.partners += (cloudsc_gpu::truncation_partner_sets() >> first) & 1u;
       bool tie = false;
       for (int a = 0; a < 5; ++a)
         for (int b = a + 1; b < 5; ++b) tie = tie || (zr[a] == zr[b] && zr[a] < T(1));
@@ -120,10 +126,10 @@ int main() {
   Stats<float> sf;
   run<double>(400000, 11, sd);
   run<float>(400000, 12, sf);
-  std::printf("double %ld %ld %ld %ld %ld %ld %ld\n", sd.cases, sd.bad, sd.trunc, sd.two, sd.ties, sd.cascade,
-              sd.negdiag);
-  std::printf("float %ld %ld %ld %ld %ld %ld %ld\n", sf.cases, sf.bad, sf.trunc, sf.two, sf.ties, sf.cascade,
-              sf.negdiag);
+  std::printf("double %ld %ld %ld %ld %ld %ld %ld %ld\n", sd.cases, sd.bad, sd.trunc, sd.two, sd.ties, sd.cascade,
+              sd.negdiag, sd.partners);
+  std::printf("float %ld %ld %ld %ld %ld %ld %ld %ld\n", sf.cases, sf.bad, sf.trunc, sf.two, sf.ties, sf.cascade,
+              sf.negdiag, sf.partners);
   return 0;
 }
 """
@@ -145,11 +151,11 @@ def harness(tmp_path_factory):
 
 @pytest.mark.parametrize("prec", ["double", "float"])
 def test_kernel_truncation_equals_oracle(harness, prec):
-    cases, bad, trunc, two, ties, cascade, negdiag = harness[prec]
+    cases, bad, trunc, two, ties, cascade, negdiag, partners = harness[prec]
     print(f"{prec}: {cases} cases, truncating {trunc}, two species {two}, ties {ties}, cascades {cascade}, "
-          f"negative diagonal {negdiag}")
+          f"negative diagonal {negdiag}, sorted path {partners}, commuting path {trunc - partners}")
     assert bad == 0, f"{bad} of {cases} cases differ from the oracle"
     # the cases that exercise every branch of the kernel's version are all present
     for name, n in (("truncating", trunc), ("two species", two), ("ties", ties), ("cascades", cascade),
-                    ("negative diagonal", negdiag)):
+                    ("negative diagonal", negdiag), ("sorted-path", partners), ("commuting-path", trunc - partners)):
         assert n >= 100, f"only {n} {name} cases"
