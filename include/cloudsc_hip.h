@@ -34,9 +34,12 @@
  * Three arithmetic variants share one argument list:
  *   _dp    : FP64 storage and arithmetic (the ifs_dp build).
  *   _sp    : FP32 storage and arithmetic (the ifs_sp build).
- *   _mixed : FP32 storage and bulk arithmetic; FP64 for CLOUD_SUPERSATCHECK
- *            (cloud_supersatcheck.F90) and for the no-pivot 5x5 elimination
- *            with its back-substitution (cloudsc.F90:3546-3628).
+ *   _mixed : FP32 storage and bulk arithmetic; FP64 for the water budget of
+ *            each level: the first-guess amounts ZQXFG, the explicit
+ *            source/sink matrix ZSOLQA, the sinks limited by ZQXFG, the sink
+ *            truncation (cloudsc.F90:3455-3536) and the right-hand side of the
+ *            implicit solve (:3576-3587).  The elimination and
+ *            CLOUD_SUPERSATCHECK stay in FP32.
  */
 #ifndef CLOUDSC_HIP_H
 #define CLOUDSC_HIP_H
