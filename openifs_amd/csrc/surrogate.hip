@@ -255,28 +255,74 @@ __global__ void fill_f32(float* p, float v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
 }
 
-__global__ void fill_i32(int* p, int v) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = v;
+// Gate score and rerun selection.  Score: max over levels with T < 238 K of q / qsat_mix (FOEEWM with FOEALFA,
+// fcttre.func.h).  With flags, each column's selection (seeded 1 % draw or score above the threshold) is stored and
+// each group of 256 columns writes how many it selected, for the compaction below.
+__global__ __launch_bounds__(256) void select_columns(KPtrs<float> p, Dims d, float frac, uint64_t seed, float thr,
+                                                      float* __restrict__ score, uint8_t* __restrict__ flags,
+                                                      int* __restrict__ group_counts) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  bool sel = false;
+  if (c < d.ngptot) {
+    float sc = 0.0f;
+    for (int k = 0; k < d.klev; ++k) {
+      const size_t i = fidx(c, k, 0, 1, d.klev, d.nproma);
+      const float t = p.pt[i];
+      if (t >= 238.0f) continue;
+      sc = fmaxf(sc, p.pq[i] / qsat_mix(d.th, t, p.pap[i]));
+    }
+    if (score) score[c] = sc;
+    if (flags) {
+      const synth::Draw u = synth::draw(seed, 0x52455255u /* "RERU" */, (uint32_t)c, 0u);
+      sel = u.u0 < (double)frac || sc > thr;
+      flags[c] = sel ? 1 : 0;
+    }
+  }
+  if (!flags) return;
+  __shared__ int wcount[4];
+  const uint64_t b = __ballot(sel);
+  if ((threadIdx.x & 63) == 0) wcount[threadIdx.x >> 6] = __popcll(b);
+  __syncthreads();
+  if (threadIdx.x == 0) group_counts[blockIdx.x] = wcount[0] + wcount[1] + wcount[2] + wcount[3];
 }
 
-// Gate score and rerun selection; selected columns are appended to list (order irrelevant: columns are
-// independent).  Score: max over levels with T < 238 K of q / qsat_mix (FOEEWM with FOEALFA, fcttre.func.h).
-__global__ __launch_bounds__(256) void select_columns(KPtrs<float> p, Dims d, float frac, uint64_t seed, float thr,
-                                                      float* __restrict__ score, int* __restrict__ list,
-                                                      int* __restrict__ count) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= d.ngptot) return;
-  float sc = 0.0f;
-  for (int k = 0; k < d.klev; ++k) {
-    const size_t i = fidx(c, k, 0, 1, d.klev, d.nproma);
-    const float t = p.pt[i];
-    if (t >= 238.0f) continue;
-    sc = fmaxf(sc, p.pq[i] / qsat_mix(d.th, t, p.pap[i]));
+// Exclusive prefix sum of the n group counts in place (one group of 1024 lanes); *total = their sum.
+__global__ __launch_bounds__(1024) void scan_counts(int* __restrict__ counts, int n, int* __restrict__ total) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, per = (n + 1023) / 1024, lo = min(n, t * per), hi = min(n, lo + per);
+  int sum = 0;
+  for (int i = lo; i < hi; ++i) sum += counts[i];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan of the per-lane sums
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
   }
-  if (score) score[c] = sc;
-  if (!list) return;
-  const synth::Draw u = synth::draw(seed, 0x52455255u /* "RERU" */, (uint32_t)c, 0u);
-  if (u.u0 < (double)frac || sc > thr) list[atomicAdd(count, 1)] = c;
+  int run = part[t] - sum;  // exclusive start of this lane's range
+  for (int i = lo; i < hi; ++i) {
+    const int ci = counts[i];
+    counts[i] = run;
+    run += ci;
+  }
+  if (t == 1023) *total = part[1023];
+}
+
+// The selected columns in ascending order: list[offset of the group + rank within it] = column.  Ascending order
+// keeps neighbouring lanes of the gather and scatter on neighbouring blocks of the fields.
+__global__ __launch_bounds__(256) void compact_columns(const uint8_t* __restrict__ flags, const int* __restrict__ offsets,
+                                                       int ngptot, int* __restrict__ list) {
+  const int c = blockIdx.x * 256 + threadIdx.x, w = threadIdx.x >> 6;
+  const bool sel = c < ngptot && flags[c];
+  const uint64_t b = __ballot(sel);
+  __shared__ int wcount[4];
+  if ((threadIdx.x & 63) == 0) wcount[w] = __popcll(b);
+  __syncthreads();
+  int base = offsets[blockIdx.x];
+  for (int i = 0; i < w; ++i) base += wcount[i];
+  const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(b >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b, 0));
+  if (sel) list[base + rank] = c;
 }
 
 // Column copies between a blocked field set (NPROMA = np_full) and a compact one (NPROMA = np_c), for one
@@ -596,7 +642,7 @@ int cloudsc_surrogate_gate_scores(cloudsc_surrogate_t* s, float* score, CLOUDSC_
   if (rc) return rc;
   if (!score) return fail(CLOUDSC_ENULL, "cloudsc_surrogate_gate_scores: NULL score");
   hipLaunchKernelGGL(surr::select_columns, dim3(nblk(ngptot, 256)), dim3(256), 0, st, p, d, 0.0f, (uint64_t)0, 0.0f,
-                     score, (int*)nullptr, (int*)nullptr);
+                     score, (uint8_t*)nullptr, (int*)nullptr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_surrogate_gate_scores", e);
   return CLOUDSC_OK;
@@ -615,17 +661,22 @@ int cloudsc_hybrid_sp_c(cloudsc_surrogate_t* s, float rerun_fraction, uint64_t r
   hipError_t e = predict(s, q, p, d, st);
   if (e != hipSuccess) return fail(CLOUDSC_EHIP, "cloudsc_hybrid_sp_c: prediction", e);
   // 2. columns for the plain kernel
+  // selection list (ngptot ints), per-group counts and offsets, per-column flags: one allocation
+  const int ngroups = (int)nblk(ngptot, 256);
   if (s->list_cap < ngptot) {
     if (s->list) (void)hipFree(s->list);
     s->list = nullptr;
     s->list_cap = 0;
-    e = hipMalloc((void**)&s->list, (size_t)ngptot * sizeof(int));
+    e = hipMalloc((void**)&s->list, (size_t)(ngptot + ngroups) * sizeof(int) + (size_t)ngptot);
     if (e != hipSuccess) return fail(CLOUDSC_ENOMEM, "cloudsc_hybrid_sp_c: hipMalloc", e);
     s->list_cap = ngptot;
   }
-  hipLaunchKernelGGL(surr::fill_i32, dim3(1), dim3(64), 0, st, s->count, 0);
-  hipLaunchKernelGGL(surr::select_columns, dim3(nblk(ngptot, 256)), dim3(256), 0, st, p, d, rerun_fraction,
-                     rerun_seed, gate_threshold, (float*)nullptr, s->list, s->count);
+  int* group_offsets = s->list + ngptot;
+  uint8_t* flags = reinterpret_cast<uint8_t*>(group_offsets + ngroups);
+  hipLaunchKernelGGL(surr::select_columns, dim3(ngroups), dim3(256), 0, st, p, d, rerun_fraction, rerun_seed,
+                     gate_threshold, (float*)nullptr, flags, group_offsets);
+  hipLaunchKernelGGL(surr::scan_counts, dim3(1), dim3(1024), 0, st, group_offsets, ngroups, s->count);
+  hipLaunchKernelGGL(surr::compact_columns, dim3(ngroups), dim3(256), 0, st, flags, group_offsets, ngptot, s->list);
   int nsel = 0;
   e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(&nsel, s->count, sizeof(int), hipMemcpyDeviceToHost, st);
