@@ -261,6 +261,36 @@ def test_hybrid_selection_reproduces_plain_and_network(params137):
     assert abs(int(sel.sum()) - n_mix) <= 3  # columns where both paths agree exactly are ambiguous
 
 
+def test_hybrid_is_reproducible_and_reruns_the_selected_columns(params137):
+    """Two hybrid calls on the same inputs give the same outputs bit for bit (the rerun list is a prefix-sum
+    compaction, not atomic appends), and exactly as many columns as reported equal the plain kernel in every
+    output."""
+    torch = _torch()
+    from openifs_amd import cloudsc as C
+    from openifs_amd import surrogate as S
+    from tests.helpers import columns
+
+    sur = S.Surrogate(137, hidden=256, hidden_layers=2, max_batch=1024, seed=4)
+    base = _fields(params137, 4000, 92)
+    plain, a, b = _clone(base), _clone(base), _clone(base)
+    C.cloudsc_blocked(plain, params137, 900.0)
+    na = sur.hybrid(a, params137, rerun_fraction=0.2, rerun_seed=11)
+    nb = sur.hybrid(b, params137, rerun_fraction=0.2, rerun_seed=11)
+    torch.cuda.synchronize()
+    assert na == nb and 0.15 * 4000 < na < 0.25 * 4000
+    names = [n for n, _, r in C.BLOCKED_ARRAYS if r == "o"] + ["plude"]
+    for n in names:
+        assert torch.equal(torch.nan_to_num(a[n], nan=7.0), torch.nan_to_num(b[n], nan=7.0)), n
+    # the rerun columns are those equal to the plain kernel in every output; their count is the reported one
+    same = None
+    for n in names:
+        x, p = columns(a[n].cpu().numpy(), base.nproma)[:4000], columns(plain[n].cpu().numpy(), base.nproma)[:4000]
+        ax = tuple(range(1, x.ndim))
+        eq = np.all((x == p) | (np.isnan(x) & np.isnan(p)), axis=ax)
+        same = eq if same is None else (same & eq)
+    assert int(same.sum()) == na
+
+
 def test_gate_scores_flag_cold_supersaturation(params137):
     torch = _torch()
     from openifs_amd import surrogate as S
