@@ -59,9 +59,14 @@ __global__ __launch_bounds__(256) void gemm_tn(const bf16* __restrict__ A, int l
   constexpr int FM = TBM / 64, FN = TBN / 64;        // 32x32 fragments per wave in M and N
   constexpr int CPR = TBK / 8, RPP = 256 / CPR;      // 16-byte chunks per row; rows per pass of 256 lanes
   constexpr int LA = TBM / RPP, LB = TBN / RPP;      // chunks per thread per K-step, A and B
-  constexpr int LDKT = TBK + 8;                      // LDS row stride (16-byte aligned, staggers banks)
+  // LDS rows of TBK bf16 (no padding); chunk c of row r sits in slot c ^ ((r >> SH) & (CPR - 1)), SH = log2(16 / CPR):
+  // the 16 rows one fragment read touches (one chunk each) and the rows one store pass writes then fall in 16
+  // distinct 16-byte slots of each 256-byte bank window -- no bank conflict on either side
+  constexpr int LDKT = TBK, SH = CPR == 4 ? 2 : (CPR == 8 ? 1 : 0);
+  static_assert(CPR == 4 || CPR == 8, "K-step 32 or 64");
   __shared__ bf16 As[2][TBM * LDKT];
   __shared__ bf16 Bs[2][TBN * LDKT];
+  auto slot = [](int r, int c) { return 8 * (c ^ ((r >> SH) & (CPR - 1))); };
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * TBM, n0 = blockIdx.x * TBN;
@@ -78,9 +83,15 @@ __global__ __launch_bounds__(256) void gemm_tn(const bf16* __restrict__ A, int l
   };
   auto sstore = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < LA; ++i) *reinterpret_cast<bf16x8*>(&As[buf][(r0 + RPP * i) * LDKT + kq]) = ra[i];
+    for (int i = 0; i < LA; ++i) {
+      const int r = r0 + RPP * i;
+      *reinterpret_cast<bf16x8*>(&As[buf][r * LDKT + slot(r, kq / 8)]) = ra[i];
+    }
 #pragma unroll
-    for (int i = 0; i < LB; ++i) *reinterpret_cast<bf16x8*>(&Bs[buf][(r0 + RPP * i) * LDKT + kq]) = rb[i];
+    for (int i = 0; i < LB; ++i) {
+      const int r = r0 + RPP * i;
+      *reinterpret_cast<bf16x8*>(&Bs[buf][r * LDKT + slot(r, kq / 8)]) = rb[i];
+    }
   };
   f32x16 acc[FM][FN];
 #pragma unroll
@@ -103,11 +114,15 @@ __global__ __launch_bounds__(256) void gemm_tn(const bf16* __restrict__ A, int l
     for (int ks = 0; ks < TBK; ks += 16) {
       bf16x8 a[FM], b[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
-        a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][(wm * (TBM / 2) + i * 32 + fr) * LDKT + ks + fk]);
+      for (int i = 0; i < FM; ++i) {
+        const int r = wm * (TBM / 2) + i * 32 + fr;
+        a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][r * LDKT + slot(r, (ks + fk) / 8)]);
+      }
 #pragma unroll
-      for (int j = 0; j < FN; ++j)
-        b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][(wn * (TBN / 2) + j * 32 + fr) * LDKT + ks + fk]);
+      for (int j = 0; j < FN; ++j) {
+        const int r = wn * (TBN / 2) + j * 32 + fr;
+        b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][r * LDKT + slot(r, (ks + fk) / 8)]);
+      }
 #pragma unroll
       for (int i = 0; i < FM; ++i)
 #pragma unroll
