@@ -32,6 +32,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // 128 x 128 for small products (training batches), 256 x 128 / 256 x 256 for large ones (inference), where the
 // larger wave tile reads fewer LDS bytes per MFMA (1 KB / 0.75 KB / 0.5 KB).
 constexpr int BM = 128, BN = 128, BK = 32;  // smallest tile: the granularity of every dimension
+constexpr int MAX_SPLIT = 4;                // stretches of a split weight-gradient contraction
 
 enum Epi : int {
   EPI_F32 = 0,      // C (fp32)
@@ -49,6 +50,7 @@ struct EpiArgs {
   bf16* actT;      // bf16 [N x M], leading dimension ldt
   int ldt;
   const float* z;  // fp32 [M x N] pre-activation (EPI_DHIDDEN), leading dimension ldc
+  long zstride;    // split contraction (EPI_F32, grid z > 1): elements between the partial products of the splits
 };
 
 __device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
@@ -70,6 +72,10 @@ __global__ __launch_bounds__(256) void gemm_tn(const bf16* __restrict__ A, int l
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int wm = w >> 1, wn = w & 1;
   const int m0 = blockIdx.y * TBM, n0 = blockIdx.x * TBN;
+  // split contraction: group z multiplies the z-th stretch of K columns into its own partial product
+  A += (size_t)blockIdx.z * K;
+  B += (size_t)blockIdx.z * K;
+  if (EPI == EPI_F32) e.c += (size_t)blockIdx.z * e.zstride;
   // global -> LDS in chunks of 8 bf16; thread t moves rows r0, r0 + RPP, ...
   const int r0 = tid / CPR, kq = (tid % CPR) * 8;
   const bf16* ag = A + (size_t)(m0 + r0) * lda + kq;
@@ -328,6 +334,15 @@ __global__ __launch_bounds__(256) void mse_grad(const float* __restrict__ y, con
   if ((threadIdx.x & 63) == 0 && l != 0.0f) atomicAdd(loss, l / (float)M);
 }
 
+// Sum of the S partial products of a split contraction, in split order: out[i] = sum_z part[z * n + i].
+__global__ void sum_splits(const float* __restrict__ part, size_t n, int S, float* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v = part[i];
+  for (int z = 1; z < S; ++z) v += part[(size_t)z * n + i];
+  out[i] = v;
+}
+
 // Bias gradient: db[n] = sum_m G^T[n][m] (fp32 sum of the bf16 gradient the weight product also sees).
 __global__ __launch_bounds__(256) void row_sums(const bf16* __restrict__ gT, int M, float* __restrict__ db) {
   const int n = blockIdx.x;
@@ -461,6 +476,27 @@ hipError_t forward(cloudsc_mlp* m, const bf16* x, int M, float* y, bool train, h
 
 namespace {
 
+// dW = G^T X over the batch (C[dout x din], contraction over M columns).  With few output tiles (one 128x128 tile
+// per CU or less) the contraction is split in S stretches over the grid's z dimension, each into its own partial
+// product in the workspace, which sum_splits then adds in split order (deterministic).
+hipError_t weight_gradient(cloudsc_mlp* m, const bf16* gT, const bf16* xT, int dout, int din, int M,
+                           const mlp::EpiArgs& ew, hipStream_t s) {
+  const long tiles = (long)(dout / mlp::BM) * (din / mlp::BN);
+  int S = 1;
+  while (S < mlp::MAX_SPLIT && tiles * S < 256 && (M / (2 * S)) % mlp::BK == 0 && M / (2 * S) >= 512) S *= 2;
+  if (S == 1 || dout % mlp::BM || din % mlp::BN || (size_t)S * dout * din > m->ws_elems)
+    return gemm<mlp::EPI_F32>(gT, M, xT, M, dout, din, M, ew, s);
+  mlp::EpiArgs ep{};
+  ep.c = m->ws;
+  ep.ldc = din;
+  ep.zstride = (long)dout * din;
+  hipLaunchKernelGGL((mlp::gemm_tn<mlp::EPI_F32, 128, 128, 32>), dim3(din / 128, dout / 128, S), dim3(256), 0, s, gT,
+                     M, xT, M, M / S, ep);
+  const size_t n = (size_t)dout * din;
+  hipLaunchKernelGGL(mlp::sum_splits, dim3(blocks(n, 256)), dim3(256), 0, s, m->ws, n, S, ew.c);
+  return hipGetLastError();
+}
+
 // Backward pass from the output gradient (g0, g0T hold dY and dY^T); fills gw, gb.
 hipError_t backward(cloudsc_mlp* m, const bf16* xT, int M, hipStream_t s) {
   hipError_t e = hipSuccess;
@@ -471,7 +507,7 @@ hipError_t backward(cloudsc_mlp* m, const bf16* xT, int M, hipStream_t s) {
     mlp::EpiArgs ew{};
     ew.c = m->gw[l];
     ew.ldc = m->din[l];
-    e = gemm<mlp::EPI_F32>(gT, M, xlT, M, m->dout[l], m->din[l], M, ew, s);
+    e = weight_gradient(m, gT, xlT, m->dout[l], m->din[l], M, ew, s);
     if (e != hipSuccess) break;
     hipLaunchKernelGGL(mlp::row_sums, dim3(m->dout[l]), dim3(256), 0, s, gT, M, m->gb[l]);
     if (l == 0) break;
@@ -525,6 +561,12 @@ int cloudsc_mlp_create(cloudsc_mlp_t** out, int n_in, int n_hidden, int n_hidden
   const size_t mg = (size_t)max_batch * std::max(n_hidden, n_out);
   if (!rc) rc = H(mg, &m->g0) || H(mg, &m->g0T) || H(mg, &m->g1) || H(mg, &m->g1T);
   if (!rc) rc = F((size_t)max_batch * n_out, &m->y) || F(1, &m->loss);
+  size_t wmax = 0;
+  for (int l = 0; l < m->nl; ++l) wmax = std::max(wmax, (size_t)m->din[l] * m->dout[l]);
+  if (!rc) {
+    m->ws_elems = (size_t)mlp::MAX_SPLIT * wmax;
+    rc = F(m->ws_elems, &m->ws);
+  }
   if (rc) {
     rc = CLOUDSC_ENOMEM;
     cloudsc_mlp_destroy(m);

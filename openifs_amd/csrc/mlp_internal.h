@@ -24,6 +24,8 @@ struct cloudsc_mlp {
   mlp::bf16 *g0 = nullptr, *g0T = nullptr, *g1 = nullptr, *g1T = nullptr;  // gradient ping-pong buffers
   float* y = nullptr;           // output [max_batch x n_out]
   float* loss = nullptr;        // device scalar
+  float* ws = nullptr;          // partial products of split weight-gradient contractions
+  size_t ws_elems = 0;
   std::vector<void*> allocs;
 };
 

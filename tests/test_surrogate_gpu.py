@@ -64,10 +64,11 @@ def _mlp_case(seed, n_in=256, hidden=128, layers=2, n_out=128, M=256):
     return m, x, t, w
 
 
-@pytest.mark.parametrize("layers", [1, 3])
-def test_mlp_gradients_match_cpu_reference(layers):
+# batch 4,096: the weight-gradient contractions are split in 4 stretches (few output tiles) and summed
+@pytest.mark.parametrize("layers,M", [(1, 256), (3, 256), (2, 4096)])
+def test_mlp_gradients_match_cpu_reference(layers, M):
     torch = _torch()
-    m, x, t, w = _mlp_case(3 + layers, layers=layers)
+    m, x, t, w = _mlp_case(3 + layers, layers=layers, M=M)
     m.gradients(x.to(DEV), x.T.contiguous().to(DEV), t.to(DEV), w.to(DEV))
     torch.cuda.synchronize()
     ws = [m.weight(l) for l in range(m.n_layers)]
