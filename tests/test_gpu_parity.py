@@ -173,10 +173,12 @@ def check_mixed_against_fp64(out, ref64, nproma, report=None):
     assert e <= MIXED_TOL
 
 
-@pytest.mark.parametrize("klev", [137, 91])
-def test_mixed_within_1e4_of_fp64(klev, params137, params91):
+# NPROMA 64: the tuned launch (wave-aligned blocks); NPROMA 100: blocks not aligned to waves (per-lane block
+# offsets, the compiler's register allocation) and a ragged last block
+@pytest.mark.parametrize("klev,nproma", [(137, 64), (91, 64), (137, 100)])
+def test_mixed_within_1e4_of_fp64(klev, nproma, params137, params91):
     params = params137 if klev == 137 else params91
-    ngptot, nproma = 8192, 64
+    ngptot = 8192
     inp, out = gpu_run(params, "mixed", ngptot, nproma, klev, 51)
     ref64 = oracle_blocked("dp", params, inp, ngptot, nproma, klev, nthreads=16)
     report = []
